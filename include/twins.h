@@ -1,0 +1,124 @@
+/* twins.h -- the C ABI of libtwins_core.so, the MI355X (gfx950) packet-simulation core of the
+ * TwiNS TCP model (`twins`).
+ *
+ * The reference has no library interface: its boundary is the `twins` command line, the flow /
+ * topology / worlds file formats and the fct.csv output, driven by one `main` (src/main.cu). Each
+ * entry point below replaces one stage of that `main`; the `twins` binary built here calls them,
+ * and so can any host program (ctypes, cgo, JNI: see INTEGRATION.md).
+ *
+ * Conventions: every function returns 0 on success and a negative code on error, with a message
+ * from tw_last_error() (per thread). Handles own their memory; caller buffers are copied and never
+ * retained. A batch owns its device memory and one HIP stream; use one host thread per batch.
+ * Times are nanoseconds at the boundary (the model runs in integer picoseconds).
+ */
+#ifndef TWINS_H
+#define TWINS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { TW_OK = 0, TW_EINVAL = -1, TW_EIO = -2, TW_ENOMEM = -3, TW_EDEVICE = -4, TW_ESTATE = -5 };
+enum { TW_TOPO_FATTREE = 0, TW_TOPO_LEAFSPINE = 1, TW_TOPO_GRAPH = 2 };
+enum { TW_CC_NEWRENO = 0, TW_CC_DCTCP = 1, TW_CC_CUBIC = 2 };
+
+typedef struct tw_world tw_world;
+typedef struct tw_batch tw_batch;
+
+/* One simulation ("world"): topology, flows and parameters, as host arrays.
+ * Replaces the set-up part of the reference's main (src/main.cu:223-1015). */
+typedef struct tw_world_desc
+{
+    int32_t topo;                  /* TW_TOPO_* */
+    int32_t k;                     /* FatTree: ports per switch (4..32, a power of two) */
+    int32_t leaves, spines, hpl;   /* leaf-spine: leaves, spines, hosts per leaf */
+    /* topology file (TW_TOPO_GRAPH): hosts, switches and the links in file order; an endpoint is
+     * a host h (>= 0) or switch j (encoded -1 - j); a host is the first endpoint of its one link */
+    int32_t n_hosts, n_switches, n_links;
+    const int32_t *link_a, *link_b;
+    /* link failures (TW_TOPO_GRAPH): at fail_ns[i] the first link between switches fail_a[i] and
+     * fail_b[i] goes down (fail_up[i] = 0) or up again (1) */
+    int32_t n_fail;
+    const int64_t *fail_ns;
+    const int32_t *fail_a, *fail_b;
+    const uint8_t *fail_up;
+    /* flows ("messages") in file order: hosts, bytes, start (ns), connection id (-1: its own
+     * connection; equal ids share one connection, in order) */
+    int32_t n_flows;
+    const int32_t *src, *dst;
+    const int64_t *size, *start_ns, *conn;
+    /* optional ns-3 ECMP hashes per flow (the reference's --ecmp_from); NULL: computed */
+    const uint32_t *hash_data, *hash_ack;
+    int32_t cc;                    /* TW_CC_* */
+    int32_t ecn_k, queue;          /* RED threshold K and queue limit, packets (0: 30, 200) */
+    int32_t fin;                   /* senders close their connections after the last flow */
+    int32_t spray;                 /* per-packet ECMP (MRG32k3a), with rng_seed / rng_run */
+    uint32_t rng_seed;
+    uint64_t rng_run;
+    int64_t max_slots;             /* 0: 2,000,000 (2 s simulated) */
+    int32_t rtt_ring, ooo_segs;    /* per-connection caps (0: 65536, 512) */
+} tw_world_desc;
+
+/* Builds a world: routes per failure epoch, connections, ns-3 ECMP hashes and source ports,
+ * the per-host application order. (src/main.cu:327-806, 870-963) */
+int tw_world_create(const tw_world_desc *desc, tw_world **out);
+
+/* The same from `twins` command-line arguments (flow and topology files; no --worlds):
+ * --k --flows --tcp --ecnK --queue --fin --topo --topofile --fail --leaves --spines --hpl --spray
+ * --RngSeed --RngRun --max_slots --ecmp_from --rtt_ring --ooo_segs (src/main.cu:155-222) */
+int tw_world_create_args(int argc, const char *const *argv, tw_world **out);
+
+typedef struct tw_world_info
+{
+    int32_t hosts, switches, ports_per_switch, links, conns, flows;
+    uint64_t device_bytes; /* device memory this world needs in a batch */
+} tw_world_info;
+int tw_world_info_get(const tw_world *w, tw_world_info *info);
+int tw_world_destroy(tw_world *w);
+
+/* Packing of worlds into one device batch and the launch shape (0: automatic). */
+typedef struct tw_pack_cfg
+{
+    int32_t chain_runs;      /* recorded event-order runs: 9 (`twins`) or 25 (`twins128`); 0: 9 */
+    int32_t block_threads;   /* threads per workgroup (a multiple of 64, <= 1024) */
+    int32_t blocks;          /* persistent workgroups */
+    int64_t slot_budget;     /* slots a world may run per kernel launch */
+} tw_pack_cfg;
+
+/* Packs worlds (copied; the handles stay the caller's) into device memory.
+ * (src/main.cu:267-326, 807-1015) */
+int tw_batch_create(tw_world *const *worlds, int n, const tw_pack_cfg *cfg, tw_batch **out);
+
+typedef struct tw_run_stats
+{
+    double wall_s;           /* host wall time of the run (the event loop; set-up excluded) */
+    double kernel_s;         /* summed kernel time (HIP events) */
+    int64_t launches;
+    int64_t slots_max;       /* largest per-world slot count executed */
+    uint64_t packet_events;  /* frames started on links, all worlds */
+    uint64_t flows_done;
+    int32_t worlds_done;
+} tw_run_stats;
+
+/* Runs every world of the batch to completion (all flows done or max_slots).
+ * (src/main.cu:1024-1048, the graph-launch loop) */
+int tw_batch_run(tw_batch *b, tw_run_stats *stats);
+
+/* Per flow of world `world`, in its file order: when the socket got it (established_ns) and when
+ * its last byte was acknowledged (completed_ns); -1 if not reached. (src/main.cu:1143-1153) */
+int tw_fct_get(tw_batch *b, int world, int64_t *established_ns, int64_t *completed_ns);
+
+/* The world's counters in the reference's numbering: drops, marks, retx, rto, tx-cap,
+ * active/ooo/due overflows, retx by state, max in-flight items, rtt/ring overflows, undecided
+ * ties (25), packet-events (26); executed slots in [31]. (src/main.cu:1054-1075) */
+int tw_counters_get(tw_batch *b, int world, uint64_t ev[32]);
+
+int tw_batch_destroy(tw_batch *b);
+const char *tw_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

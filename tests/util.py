@@ -1,0 +1,57 @@
+"""Helpers of the parity tests: the golden cases, running the oracle and the GPU command line,
+comparing outputs column by column as the reference's validation/check_exact.sh does."""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+ORACLE = os.path.join(ROOT, "oracle", "tcp_oracle")
+TWINS = os.path.join(ROOT, "twins_amd", "bin", "twins")
+
+
+def oracle_bin():
+    """The oracle binary, built here with make if it is missing (a CPU build, no GPU needed)."""
+    if not os.path.exists(ORACLE):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, capture_output=True)
+    return ORACLE
+
+
+def cases():
+    """(name, flows, ns-3 output, argument list) of every stored golden case."""
+    out = []
+    for line in open(os.path.join(GOLDEN, "manifest.txt")):
+        if line.startswith("#") or not line.strip():
+            continue
+        w, t, v, fl, ref, args = line.rstrip("\n").split("|")
+        out.append((f"{w}-{t}-{v}", fl, ref, args.split() + [f"--tcp={t}"]))
+    return out
+
+
+def run(binary, args, out_csv, extra=(), timeout=600):
+    """Runs a `twins`-style binary from tests/golden (relative paths); returns its stdout+stderr."""
+    r = subprocess.run([binary] + list(args) + [f"--out={out_csv}"] + list(extra), cwd=GOLDEN, capture_output=True, text=True,
+                       timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"{os.path.basename(binary)} {' '.join(args)} failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
+    return r.stdout + r.stderr
+
+
+def read_cols(path, cols):
+    rows = []
+    with open(path) as f:
+        next(f)
+        for line in f:
+            v = line.rstrip("\n").split(",")
+            rows.append(tuple(v[c] for c in cols))
+    return rows
+
+
+def same_times(ref_csv, out_csv):
+    """(flows identical in established_ns and completed_ns, flows) -- check_exact.sh's measure."""
+    a, b = read_cols(ref_csv, (4, 5)), read_cols(out_csv, (4, 5))
+    assert len(a) == len(b), f"{len(a)} reference rows, {len(b)} output rows"
+    return sum(x == y for x, y in zip(a, b)), len(a)
+
+
+def undecided(log):
+    return "warning" in log and "undecided" in log

@@ -1,0 +1,116 @@
+"""twins_amd: Python access to libtwins_core.so, the MI355X (gfx950) TCP packet-simulation core.
+
+This is a thin ctypes layer over the C ABI in include/twins.h (the same calls the `twins` command
+line makes): build worlds from `twins`-style arguments, pack them into one device batch, run it on
+the GPU, read per-flow establishment / completion times and the counters. It never simulates
+anything itself: without the built library, or without a GPU, every call raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtwins_core.so")
+CLI_PATH = os.path.join(HERE, "bin", "twins")
+
+EV_NAMES = {0: "drops", 1: "marks", 2: "retx", 3: "rto", 4: "tx-cap", 5: "active-ovf", 6: "ooo-ovf", 7: "due-ovf",
+            13: "max-inflight", 14: "rtt-ovf", 19: "ring-ovf", 25: "undecided-ties", 26: "packet-events", 31: "slots"}
+
+
+class TwError(RuntimeError):
+    pass
+
+
+class PackCfg(ctypes.Structure):
+    _fields_ = [("chain_runs", ctypes.c_int32), ("block_threads", ctypes.c_int32), ("blocks", ctypes.c_int32),
+                ("slot_budget", ctypes.c_int64)]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("kernel_s", ctypes.c_double), ("launches", ctypes.c_int64),
+                ("slots_max", ctypes.c_int64), ("packet_events", ctypes.c_uint64), ("flows_done", ctypes.c_uint64),
+                ("worlds_done", ctypes.c_int32)]
+
+
+class WorldInfo(ctypes.Structure):
+    _fields_ = [("hosts", ctypes.c_int32), ("switches", ctypes.c_int32), ("ports_per_switch", ctypes.c_int32),
+                ("links", ctypes.c_int32), ("conns", ctypes.c_int32), ("flows", ctypes.c_int32),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded core library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TwError(f"{LIB_PATH} is not built: run __graft_entry__.build() or make -C twins_amd")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+        L.tw_last_error.restype = ctypes.c_char_p
+        L.tw_world_create_args.argtypes = [i32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(vp)]
+        L.tw_world_info_get.argtypes = [vp, ctypes.POINTER(WorldInfo)]
+        L.tw_world_destroy.argtypes = [vp]
+        L.tw_batch_create.argtypes = [ctypes.POINTER(vp), i32, ctypes.POINTER(PackCfg), ctypes.POINTER(vp)]
+        L.tw_batch_run.argtypes = [vp, ctypes.POINTER(RunStats)]
+        L.tw_fct_get.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.tw_counters_get.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_uint64)]
+        L.tw_batch_destroy.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise TwError(f"twins core error {rc}: {lib().tw_last_error().decode()}")
+
+
+class World:
+    """One simulation, from `twins` arguments (flow and topology files)."""
+
+    def __init__(self, args):
+        L = lib()
+        arr = (ctypes.c_char_p * len(args))(*[a.encode() for a in args])
+        self._h = ctypes.c_void_p()
+        _check(L.tw_world_create_args(len(args), arr, ctypes.byref(self._h)))
+        self.info = WorldInfo()
+        _check(L.tw_world_info_get(self._h, ctypes.byref(self.info)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.tw_world_destroy(self._h)
+            self._h = None
+
+
+class Batch:
+    """Worlds packed into one device batch and run by the persistent world-queue kernel."""
+
+    def __init__(self, worlds, chain_runs=9, block_threads=0, blocks=0, slot_budget=0):
+        L = lib()
+        self.worlds = list(worlds)
+        hs = (ctypes.c_void_p * len(self.worlds))(*[w._h for w in self.worlds])
+        cfg = PackCfg(chain_runs, block_threads, blocks, slot_budget)
+        self._h = ctypes.c_void_p()
+        _check(L.tw_batch_create(hs, len(self.worlds), ctypes.byref(cfg), ctypes.byref(self._h)))
+
+    def run(self):
+        st = RunStats()
+        _check(lib().tw_batch_run(self._h, ctypes.byref(st)))
+        return st
+
+    def fct(self, w):
+        n = self.worlds[w].info.flows
+        est, done = (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)()
+        _check(lib().tw_fct_get(self._h, w, est, done))
+        return list(est), list(done)
+
+    def counters(self, w):
+        ev = (ctypes.c_uint64 * 32)()
+        _check(lib().tw_counters_get(self._h, w, ev))
+        return list(ev)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.tw_batch_destroy(self._h)
+            self._h = None

@@ -1,0 +1,212 @@
+// Device-resident state of a batch of TCP worlds: the layout in HBM shared by the host-side packer
+// (batch.cpp) and the gfx950 kernels (tcp_kernel.hip).
+//
+// A batch concatenates independent worlds. Every entity has a global id: hosts, switches,
+// switch ports, links (one per sending queue: host h sends on link h, switch port p on link
+// H + p, both world-local and offset by the world's link base), connections and messages. A
+// frame carries world-local host ids and global connection ids.
+//
+// Layout choices (MI355X): per-connection state is an array of records split by owner (the
+// sender's host and the receiver's host work on one connection in the same slot, each on its
+// own record), each record aligned to 128-byte cache lines so one lane's field accesses stay in
+// a few lines; queues, hosts, ports and link messages are arrays indexed by global id.
+#pragma once
+#include <cstdint>
+
+namespace tw
+{
+
+constexpr int64_t SLOT_PS = 1000000, PROP_PS = 1000000, PS_PER_BYTE = 80;
+constexpr uint32_t MSS = 1460;
+constexpr int CTRL_FRAME = 42, SYN_FRAME = 46;
+constexpr int MAX_TX = 300;    // frames one link can start in a slot (the model's cap)
+constexpr int NRING = 3;       // link message ring: slots t-2, t-1 (read) and t (written)
+constexpr int MAX_ACTIVE = 256, MAX_DUE = 16;
+constexpr int QDISC_CAP = 200, RED_K = 30;
+constexpr int64_t MIN_RTO_PS = 1000000000, DELACK_PS = 10000000, CONN_TIMEOUT_PS = 10000000000LL,
+                  CLOCK_GRAN_PS = 1000000, INIT_RTT_PS = 100000000;
+constexpr uint32_t DELACK_COUNT = 2, INIT_CWND = 10 * MSS;
+constexpr int DUPTHRESH = 3, DATA_RETRIES = 6;
+constexpr double DCTCP_G = 0.0625;
+
+constexpr uint8_t F_ACK = 1, F_SYN = 2, F_ECE = 4, F_ECT = 8, F_CE = 16, F_DATA = 32, F_CWR = 64, F_FIN = 128;
+constexpr uint8_t IT_SACKED = 1, IT_LOST = 2, IT_RETX = 4;
+constexpr uint8_t FIN_CLOSE = 1, FIN_SENT = 2, FIN_ACKED = 4, FIN_TW = 8;
+enum : uint8_t { R_OPEN = 0, R_LAST_ACK, R_CLOSED };
+enum : uint8_t { S_IDLE = 0, S_SYN_SENT, S_EST, S_DONE };
+enum : uint8_t { C_OPEN = 0, C_DISORDER, C_CWR, C_RECOVERY, C_LOSS };
+enum : uint8_t { E_IDLE = 0, E_CE_RCVD, E_SENDING_ECE, E_ECE_RCVD, E_CWR_SENT };
+enum : uint8_t { CC_NEWRENO = 0, CC_DCTCP = 1, CC_CUBIC = 2 };
+enum : int32_t { TOPO_FATTREE = 0, TOPO_LEAFSPINE = 1, TOPO_GRAPH = 2 };
+
+// counters (the reference's ev[] numbering)
+enum : int
+{
+    EV_DROP = 0,
+    EV_MARK = 1,
+    EV_RETX = 2,
+    EV_RTO = 3,
+    EV_TXCAP = 4,
+    EV_ACTIVE_OVF = 5,
+    EV_OOO_OVF = 6,
+    EV_DUE_OVF = 7,
+    EV_RETX_BY_STATE = 8, // 8..12
+    EV_MAX_INFLIGHT = 13,
+    EV_RTT_OVF = 14,
+    EV_RING_OVF = 19,
+    EV_UNDECIDED = 25,
+    EV_PKT_EVENTS = 26, // frames started on a link (packet-events)
+    EV_N = 32
+};
+
+// A frame: 24 bytes of header then the event-order chain (RUNS (code, count) pairs + a 2-word
+// root): 64 bytes with 9 runs, 128 with 25.
+template <int RUNS>
+struct alignas(16) FrameT
+{
+    int64_t ts;       // enqueue time at a queue; arrival time at the next hop on a link
+    uint32_t flow;    // connection (global id)
+    uint32_t sa;      // data: first byte; otherwise the acknowledgement number
+    uint16_t len;     // bytes on the wire
+    uint16_t dst;     // destination host (world-local)
+    uint8_t flags;
+    uint8_t to_sender;
+    uint8_t up;       // FatTree / leaf-spine: the ECMP index chosen at the sender
+    uint8_t pad;
+    uint16_t chain[2 * RUNS + 2];
+};
+static_assert(sizeof(FrameT<9>) == 64 && sizeof(FrameT<25>) == 128, "frame layout");
+
+// per connection, read-only
+struct ConnStatic
+{
+    int32_t src, dst; // global host ids
+    uint32_t hash_data, hash_ack;
+    int32_t m_first, m_cnt; // global message ids
+    uint32_t it_cap, oo_cap;
+    uint64_t it_off, oo_off; // ring offsets (sent list + RTT history; out-of-order blocks)
+    int32_t world;
+    uint8_t fin_on, pad[3];
+};
+
+// the sending side (touched only by the sender's host)
+struct alignas(128) SndState
+{
+    uint32_t tail, una, nxt, high, cwnd, ssthresh, dupacks, recover;
+    uint32_t it_head, it_n, sack_b, lost_b, rtx_b, sacked_n, cand_n, cand;
+    uint32_t hisack, h_head, h_n, cwnd_cnt, ack_rem, acked_total, acked_ecn, next_seq;
+    int32_t m_due, m_wr, m_dn;
+    uint8_t st, cong, fin, sack_seen, recover_active, first_partial, cwnd_limited, s_ce;
+    uint8_t ecn_s, next_seq_flag, syn_tries, cub_found;
+    int64_t srtt, rttvar, rto, rto_at, syn_at, timing_time, ecn_echo_seq, ecn_cwr_seq;
+    double alpha;
+    uint32_t cub_cnt, cub_lastmax, cub_origin, cub_endseq, cub_samples, cub_ackcnt, cub_tcpcwnd, pad0;
+    double cub_k;
+    int64_t cub_delaymin, cub_epoch, cub_roundstart, cub_lastack, cub_currrtt;
+};
+
+// the receiving side (touched only by the receiver's host)
+struct alignas(128) RcvState
+{
+    uint32_t rcv_nxt, ooo_n, ooo_h, prior_rcv_nxt;
+    uint8_t r_up, r_est, r_st, r_last_n, srv_syn_n, delack_count, ce_state, prior_flag;
+    uint8_t reserved, ecn_r, pad[6];
+    int64_t r_fin, r_synack_t, r_srtt, r_rttvar, r_last_at, r_frx_at, srv_syn_at, delack_at, ecn_ce_seq;
+};
+
+// MRG32k3a state (the layout of TwMrg in exact_math.h)
+struct TwMrgState
+{
+    double s[6];
+};
+
+struct Due
+{
+    int64_t at;
+    int32_t flow, kind; // 0 rto, 1 delack, 2 syn / deferred send, 3 SYN-ACK retx, 4 LastAck, 5 FIN+ACK retx
+};
+
+// one world of a batch
+struct WorldDesc
+{
+    int32_t kind, k, hk, H, S, ecmp_n, leaves, spines, hpl, n_fail;
+    int32_t host0, sw0, link0, L, conn0, nconn, msg0, M;
+    int32_t rng0;  // first switch's MRG32k3a state (-1: per-flow ECMP)
+    int32_t pad;
+    int64_t max_slots;
+    // topology files: offsets of the world's rows in the batch tables: host switch/port
+    // [H], route offsets and counts [epoch][S][S] (offsets relative to the world's port
+    // lists), port lists, down ports [epoch][S][k], failure times [n_fail]
+    int64_t o_host, o_rt, o_rtp, o_pdn, o_fail;
+};
+
+// a world's progress (device-written)
+struct WorldRun
+{
+    int64_t slot;         // the next slot to run
+    int64_t slots_run;    // slots executed
+    uint64_t next_wake;   // this slot: earliest host wake-up
+    uint32_t act_cur, act_prev; // entities with activity: this slot, the previous one
+    uint32_t flows_done;
+    int32_t finished;
+    int32_t error;
+    int32_t pad;
+    unsigned long long ev[EV_N];
+};
+
+// every device pointer of a batch (one copy in device memory, passed by pointer)
+template <int RUNS>
+struct BatchT
+{
+    typedef FrameT<RUNS> Frame;
+    static constexpr int CH = 2 * RUNS + 2;
+    int32_t nworlds, nhosts, nswitches, nlinks, nconns, nmsgs;
+    WorldDesc *worlds;
+    WorldRun *run;
+    // connections
+    ConnStatic *cs;
+    SndState *snd;
+    RcvState *rcv;
+    uint16_t *rchain, *schain, *dchain, *fchain; // [conn][CH] timer chains
+    uint32_t *it_start, *h_seq, *h_cnt, *oo_s, *oo_e;
+    uint8_t *it_fl;
+    int64_t *h_t;
+    // messages
+    int32_t *mg_conn;
+    uint32_t *mg_end;
+    int64_t *mg_start, *mg_est, *mg_done;
+    int32_t *pend_msg;
+    // hosts
+    int32_t *host_world, *pend_next, *pend_end, *n_snd, *n_rcv, *act_snd, *act_rcv;
+    uint8_t *host_cc;
+    int64_t *wake;
+    Due *due; // [host][MAX_DUE]
+    // per link: the link whose frames reach the entity that sends on it (-1: an unused port)
+    int32_t *in_link;
+    int32_t *sw_world; // switch -> world
+    // queues, one per link
+    Frame *qring;
+    uint64_t *q_base;
+    int32_t *q_cap, *q_head, *q_size, *q_red_k, *q_limit;
+    int64_t *q_link_free, *q_last_start;
+    uint16_t *q_last_chain; // [link][CH]
+    // link messages [NRING][link][MAX_TX] and their tagged counts slot << 16 | n
+    Frame *msg;
+    long long *msg_cnt;
+    // topology-file tables (all worlds)
+    int32_t *g_host_sw, *g_host_port;
+    uint32_t *g_rt_off;
+    uint8_t *g_rt_n, *g_rt_ports, *g_port_dn;
+    int64_t *g_fail_t;
+    TwMrgState *sw_rng; // per-packet ECMP streams
+    // kernel work queue: worlds in the order blocks take them
+    int32_t *world_order;
+    int32_t *queue_next;
+    // switch-phase scratch, per wave: [wave][64 * 2 * MAX_TX] arrival times, refs, drawn routes
+    int64_t *scr_ts;
+    uint32_t *scr_ref;
+    uint8_t *scr_up;
+    int32_t scr_waves;
+};
+
+} // namespace tw
