@@ -28,9 +28,15 @@ def cases():
 
 
 def run(binary, args, out_csv, extra=(), timeout=600):
-    """Runs a `twins`-style binary from tests/golden (relative paths); returns its stdout+stderr."""
+    """Runs a `twins`-style binary from tests/golden (relative paths); returns its stdout+stderr.
+    A GPU binary that fails in any way ends the test session: nothing more runs on the GPU."""
+    gpu = binary == TWINS
     r = subprocess.run([binary] + list(args) + [f"--out={out_csv}"] + list(extra), cwd=GOLDEN, capture_output=True, text=True,
                        timeout=timeout)
+    if r.returncode < 0 or r.returncode in (124, 134, 137, 139) or (gpu and r.returncode != 0):
+        import pytest
+        # a crash, abort or kill: nothing more runs on the GPU in this session
+        pytest.exit(f"{os.path.basename(binary)} {' '.join(args)} ended with {r.returncode}: {r.stderr[-2000:]}", returncode=3)
     if r.returncode != 0:
         raise RuntimeError(f"{os.path.basename(binary)} {' '.join(args)} failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
     return r.stdout + r.stderr

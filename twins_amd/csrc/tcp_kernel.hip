@@ -67,16 +67,22 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
             }
             for (int g = wv; g < ngroups; g += nwaves)
             {
-                const int sl = lane / k, q = lane % k, swl = g * spw + sl;
-                const bool active = sl < spw && swl < W.S;
-                const int base = (sl < spw ? sl : 0) * k;
+                const int sl = lane / k, base = (sl < spw ? sl : 0) * k;
+                typename Md::SwLane x;
+                x.swl = g * spw + sl;
+                x.q = lane % k;
+                x.active = sl < spw && x.swl < W.S;
+                x.lid = W.link0 + W.H + x.swl * k + x.q;
+                const size_t so = (size_t)base * 2 * MAX_TX;
+                x.scr_ts = scr_ts + so;
+                x.scr_ref = scr_ref + so;
+                x.scr_up = scr_up + so;
                 typename Md::SwLds L;
                 L.rows = wl + base * (k + 1);
                 L.in_win = wl + 64 * 33 + base * 3;
                 L.base = wl + 64 * 33 + 64 * 3 + base;
                 L.cnt = wl + 64 * 33 + 64 * 3 + 64 + base;
-                const size_t so = (size_t)base * 2 * MAX_TX;
-                act += Md::switch_step(B, W, R, swl, q, active, t, L, scr_ts + so, scr_ref + so, scr_up + so);
+                act += Md::switch_step(B, W, R, x, t, L);
             }
             if (act)
                 atomicAdd(&s_act, act);
@@ -87,7 +93,7 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
             {
                 // the ticket (src/kernel.cuh:79-112)
                 unsigned cur = s_act + R->act_cur, prev = R->act_prev;
-                bool fin = R->flows_done >= (unsigned)W.M || t + 1 >= W.max_slots;
+                bool fin = atomicAdd(&R->flows_done, 0u) >= (unsigned)W.M || t + 1 >= W.max_slots; // (atomics bypass L1)
                 int64_t next = t + 1;
                 unsigned long long nw = s_wake;
                 if (cur == 0 && prev == 0 && nw != ~0ull && (int64_t)nw > next)

@@ -744,7 +744,7 @@ struct Model
             cnt = 20;
         uint32_t scale = cub_u32(8 * (1024 + BETA * 1024) / 3 / (1024 - BETA * 1024));
         delta = (seg_cwnd * scale) >> 3;
-        while (s.cub_ackcnt > delta)
+        while (delta > 0 && s.cub_ackcnt > delta) // (delta >= 1 for any window of a segment or more)
         {
             s.cub_ackcnt -= delta;
             s.cub_tcpcwnd++;
@@ -2006,121 +2006,137 @@ struct Model
         in.m1 = B.msg + ((size_t)((t + NRING - 1) % NRING) * B.nlinks + link) * MAX_TX;
         return in;
     }
-    // lanes: q (port), active (this lane has a switch), sl/base lane of the switch in the wave;
-    // scratch: this switch's region (k * 2 * MAX_TX entries) of the wave's scratch
-    static __device__ int switch_step(const Batch &B, const WorldDesc &W, WorldRun *R, int swl, int q, bool active, int64_t t,
-                                      SwLds L, int64_t *scr_ts, uint32_t *scr_ref, uint8_t *scr_up)
+    // One lane of a switch's slot: port q of world-local switch swl (inactive lanes idle).
+    // The phases run in order for all lanes of a switch, with wave_sync() between them (a CPU
+    // check may run each phase over the lanes one after another). Scratch: this switch's region
+    // (k * 2 * MAX_TX entries) of the wave's arrival scratch.
+    struct SwLane
     {
-        const int k = W.k;
+        int swl, q, lid;
+        bool active;
+        int64_t *scr_ts;
+        uint32_t *scr_ref;
+        uint8_t *scr_up;
+    };
+    static __device__ In lane_input(const Batch &B, const WorldDesc &W, const SwLane &x, int64_t t, const SwLds &L, int q)
+    {
+        return input_of(B, B.in_link[W.link0 + W.H + x.swl * W.k + q], t, L.in_win + 3 * q);
+    }
+    static __device__ int lane_egress(const Batch &B, const WorldDesc &W, const SwLane &x, int j, const Frame &p)
+    {
+        int ep = W.n_fail ? epoch_of(B, W, p.ts) : 0;
+        if (port_down(B, W, x.swl, x.q, ep))
+            return -1; // dropped on arrival at a down interface
+        return W.rng0 >= 0 ? route_up(B, W, x.swl, p.dst, x.scr_up[x.q * 2 * MAX_TX + j], ep) : route(B, W, x.swl, p, ep);
+    }
+    // 1: the ingress window of port q (tail of slot t-2's message from S0 on, head of t-1's
+    // before E0) into LDS; the port's row of egress counts cleared
+    static __device__ void sw_window(const Batch &B, const WorldDesc &W, const SwLane &x, int64_t t, const SwLds &L)
+    {
+        if (!x.active)
+            return;
         const int64_t S0 = t * SLOT_PS, E0 = S0 + SLOT_PS;
-        const int lid = W.link0 + W.H + swl * k + q; // this port's sending link
-        int *my_row = L.rows + q * (k + 1);
-        const bool spray = W.rng0 >= 0;
-        In in{nullptr, nullptr, 0, 0, 0};
-        int in_link = -1;
-        if (active)
+        const int in_link = B.in_link[x.lid];
+        int w0 = 0, w1 = 0, w2 = 0;
+        if (in_link >= 0)
         {
-            in_link = B.in_link[lid];
-            int w[3] = {0, 0, 0};
-            if (in_link >= 0)
+            if (t >= 2)
             {
-                if (t >= 2)
-                {
-                    size_t r = (size_t)((t - 2) % NRING) * B.nlinks + in_link;
-                    long long v = B.msg_cnt[r];
-                    w[1] = (v >> 16) == t - 2 ? (int)(v & 0xFFFF) : 0;
-                    const Frame *m = B.msg + r * MAX_TX;
-                    while (w[0] < w[1] && m[w[0]].ts < S0)
-                        w[0]++;
-                }
-                if (t >= 1)
-                {
-                    size_t r = (size_t)((t - 1) % NRING) * B.nlinks + in_link;
-                    long long v = B.msg_cnt[r];
-                    int n = (v >> 16) == t - 1 ? (int)(v & 0xFFFF) : 0;
-                    const Frame *m = B.msg + r * MAX_TX;
-                    while (w[2] < n && m[w[2]].ts < E0)
-                        w[2]++;
-                }
+                size_t r = (size_t)((t - 2) % NRING) * B.nlinks + in_link;
+                long long v = B.msg_cnt[r];
+                w1 = (v >> 16) == t - 2 ? (int)(v & 0xFFFF) : 0;
+                const Frame *m = B.msg + r * MAX_TX;
+                while (w0 < w1 && m[w0].ts < S0)
+                    w0++;
             }
-            L.in_win[3 * q] = w[0];
-            L.in_win[3 * q + 1] = w[1];
-            L.in_win[3 * q + 2] = w[2];
-            in = input_of(B, in_link, t, w);
-            for (int i = 0; i <= k; i++)
-                my_row[i] = 0;
-        }
-        auto egress_of = [&](int j, const Frame &p) {
-            int ep = W.n_fail ? epoch_of(B, W, p.ts) : 0;
-            if (port_down(B, W, swl, q, ep))
-                return -1;
-            return spray ? route_up(B, W, swl, p.dst, scr_up[q * 2 * MAX_TX + j], ep) : route(B, W, swl, p, ep);
-        };
-        if (spray)
-        {
-            wave_sync();
-            if (active && q == 0)
-                spray_draws(B, W, R, swl, t, L, scr_up);
-            wave_sync();
-        }
-        if (active)
-        {
-            int n = in.count();
-            for (int j = 0; j < n; j++)
+            if (t >= 1)
             {
-                int e = egress_of(j, in.at(j));
-                if (e >= 0)
-                    my_row[e]++;
-                else
-                    count(R, EV_DROP);
+                size_t r = (size_t)((t - 1) % NRING) * B.nlinks + in_link;
+                long long v = B.msg_cnt[r];
+                int n = (v >> 16) == t - 1 ? (int)(v & 0xFFFF) : 0;
+                const Frame *m = B.msg + r * MAX_TX;
+                while (w2 < n && m[w2].ts < E0)
+                    w2++;
             }
         }
-        wave_sync();
-        if (active)
+        L.in_win[3 * x.q] = w0;
+        L.in_win[3 * x.q + 1] = w1;
+        L.in_win[3 * x.q + 2] = w2;
+        int *row = L.rows + x.q * (W.k + 1);
+        for (int i = 0; i <= W.k; i++)
+            row[i] = 0;
+    }
+    // 2: arrivals counted by egress (after the spray draws, if any)
+    static __device__ void sw_count(const Batch &B, const WorldDesc &W, WorldRun *R, const SwLane &x, int64_t t, const SwLds &L)
+    {
+        if (!x.active)
+            return;
+        In in = lane_input(B, W, x, t, L, x.q);
+        int *row = L.rows + x.q * (W.k + 1);
+        const int n = in.count();
+        for (int j = 0; j < n; j++)
         {
-            int off = 0;
-            for (int j = 0; j < k; j++)
-            {
-                int c = L.rows[j * (k + 1) + q];
-                L.rows[j * (k + 1) + q] = off;
-                off += c;
-            }
-            L.cnt[q] = off;
+            int e = lane_egress(B, W, x, j, in.at(j));
+            if (e >= 0)
+                row[e]++;
+            else
+                count(R, EV_DROP);
         }
-        wave_sync();
-        if (active)
+    }
+    // 3: for egress q, each ingress's offset in its region and the region's size
+    static __device__ void sw_prefix(const WorldDesc &W, const SwLane &x, const SwLds &L)
+    {
+        if (!x.active)
+            return;
+        int off = 0;
+        for (int j = 0; j < W.k; j++)
         {
-            int b = 0;
-            for (int j = 0; j < q; j++)
-                b += L.cnt[j];
-            L.base[q] = b;
+            int c = L.rows[j * (W.k + 1) + x.q];
+            L.rows[j * (W.k + 1) + x.q] = off;
+            off += c;
         }
-        wave_sync();
-        if (active)
+        L.cnt[x.q] = off;
+    }
+    // 4: the start of egress q's region
+    static __device__ void sw_base(const SwLane &x, const SwLds &L)
+    {
+        if (!x.active)
+            return;
+        int b = 0;
+        for (int j = 0; j < x.q; j++)
+            b += L.cnt[j];
+        L.base[x.q] = b;
+    }
+    // 5: every arrival of ingress q into its egress's region, in arrival order
+    static __device__ void sw_scatter(const Batch &B, const WorldDesc &W, const SwLane &x, int64_t t, const SwLds &L)
+    {
+        if (!x.active)
+            return;
+        In in = lane_input(B, W, x, t, L, x.q);
+        int *row = L.rows + x.q * (W.k + 1);
+        const int n = in.count();
+        for (int j = 0; j < n; j++)
         {
-            int n = in.count();
-            for (int j = 0; j < n; j++)
-            {
-                const Frame &p = in.at(j);
-                int e = egress_of(j, p);
-                if (e < 0)
-                    continue;
-                int at = L.base[e] + my_row[e]++;
-                scr_ts[at] = p.ts;
-                scr_ref[at] = (uint32_t)(q << 16 | j);
-            }
+            const Frame &p = in.at(j);
+            int e = lane_egress(B, W, x, j, p);
+            if (e < 0)
+                continue;
+            int at = L.base[e] + row[e]++;
+            x.scr_ts[at] = p.ts;
+            x.scr_ref[at] = (uint32_t)(x.q << 16 | j);
         }
-        wave_sync();
-        if (!active)
+    }
+    // 6: egress q's arrivals sorted by (time, frame_before), enqueued with the arrival event's
+    // chain (RED), and the slot's transmissions; returns 1 if the port sent or holds frames
+    static __device__ int sw_egress(const Batch &B, const WorldDesc &W, WorldRun *R, const SwLane &x, int64_t t, const SwLds &L)
+    {
+        if (!x.active)
             return 0;
-        const int n = L.cnt[q], b0 = L.base[q];
-        int64_t *ts = scr_ts + b0;
-        uint32_t *ref = scr_ref + b0;
-        auto frame_at = [&](uint32_t r) -> const Frame & {
-            int iq = (int)(r >> 16);
-            In src = input_of(B, B.in_link[W.link0 + W.H + swl * k + iq], t, L.in_win + 3 * iq);
-            return src.at((int)(r & 0xFFFF));
-        };
+        const int64_t E0 = (t + 1) * SLOT_PS;
+        const int n = L.cnt[x.q], b0 = L.base[x.q];
+        int64_t *ts = x.scr_ts + b0;
+        uint32_t *ref = x.scr_ref + b0;
+        auto frame_at = [&](uint32_t r) -> const Frame & { return lane_input(B, W, x, t, L, (int)(r >> 16)).at((int)(r & 0xFFFF)); };
         for (int i = 1; i < n; i++)
         {
             int64_t xv = ts[i];
@@ -2136,7 +2152,7 @@ struct Model
             ref[j + 1] = r;
         }
         Fifo eg;
-        eg.load(B, R, lid, t);
+        eg.load(B, R, x.lid, t);
         for (int i = 0; i < n; i++)
         {
             Frame p = frame_at(ref[i]);
@@ -2150,13 +2166,34 @@ struct Model
         eg.store(t);
         return (eg.sent > 0 || eg.size > 0) ? 1 : 0;
     }
+    // one slot of a switch on the GPU: the lanes of one wavefront, phases separated by wave_sync()
+    static __device__ int switch_step(const Batch &B, const WorldDesc &W, WorldRun *R, const SwLane &x, int64_t t, const SwLds &L)
+    {
+        sw_window(B, W, x, t, L);
+        if (W.rng0 >= 0)
+        {
+            wave_sync();
+            if (x.active && x.q == 0)
+                spray_draws(B, W, R, x.swl, t, L, x.scr_up);
+        }
+        wave_sync();
+        sw_count(B, W, R, x, t, L);
+        wave_sync();
+        sw_prefix(W, x, L);
+        wave_sync();
+        sw_base(x, L);
+        wave_sync();
+        sw_scatter(B, W, x, t, L);
+        wave_sync();
+        return sw_egress(B, W, R, x, t, L);
+    }
     // per-packet ECMP: one GetInteger draw per forwarded packet, in the node's arrival order
     // (all ports merged; src/step.cuh:2706-2745)
-    static __device__ __noinline__ void spray_draws(const Batch &B, const WorldDesc &W, WorldRun *R, int swl, int64_t t, SwLds L,
+    static __device__ __noinline__ void spray_draws(const Batch &B, const WorldDesc &W, WorldRun *R, int swl, int64_t t, const SwLds &L,
                                                    uint8_t *scr_up)
     {
         const int k = W.k;
-        int pos[64];
+        int pos[32];
         for (int q = 0; q < k; q++)
             pos[q] = 0;
         TwMrg g;

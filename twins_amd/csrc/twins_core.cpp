@@ -833,21 +833,11 @@ extern "C" int tw_world_create_args(int argc, const char *const *argv, tw_world 
 }
 
 // ------------------------------------------------------------------ batches
-struct DevBuf
-{
-    void *p = nullptr;
-    ~DevBuf()
-    {
-        if (p)
-            (void)hipFree(p);
-    }
-};
-
 struct tw_batch
 {
     int runs = 9;
     std::vector<std::unique_ptr<tw_world>> worlds; // copies
-    std::vector<DevBuf> bufs;
+    std::vector<void *> bufs; // every device allocation, freed with the batch
     void *batch_dev = nullptr; // BatchT<runs> in device memory
     std::vector<int32_t> msg0, conn0;
     hipStream_t stream = nullptr;
@@ -860,6 +850,8 @@ struct tw_batch
     int64_t *mg_est = nullptr, *mg_done = nullptr;
     ~tw_batch()
     {
+        for (void *p : bufs)
+            (void)hipFree(p);
         if (ev0)
             (void)hipEventDestroy(ev0);
         if (ev1)
@@ -880,12 +872,13 @@ struct tw_batch
 template <typename T>
 static int dev_alloc(tw_batch &b, T **out, size_t n)
 {
-    b.bufs.emplace_back();
     size_t bytes = sizeof(T) * std::max<size_t>(n, 1);
-    hipError_t e = hipMalloc(&b.bufs.back().p, bytes);
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess)
         return fail(TW_ENOMEM, "hipMalloc of %zu bytes: %s", bytes, hipGetErrorString(e));
-    *out = (T *)b.bufs.back().p;
+    b.bufs.push_back(p);
+    *out = (T *)p;
     return TW_OK;
 }
 template <typename T>
