@@ -83,7 +83,10 @@ typedef struct tw_pack_cfg
 {
     int32_t chain_runs;      /* recorded event-order runs: 9 (`twins`) or 25 (`twins128`); 0: 9 */
     int32_t block_threads;   /* threads per workgroup (a multiple of 64, <= 1024) */
-    int32_t blocks;          /* persistent workgroups */
+    int32_t blocks;          /* persistent workgroups (a multiple of team_blocks) */
+    int32_t team_blocks;     /* workgroups that run one world together (1: a world per workgroup) */
+    int32_t hosts_per_wave;  /* hosts a wavefront runs side by side (1..64) */
+    int32_t pad;
     int64_t slot_budget;     /* slots a world may run per kernel launch */
 } tw_pack_cfg;
 

@@ -110,12 +110,17 @@ static void run_cpu(BatchT<RUNS> &B, int64_t budget)
     }
 }
 
-hipError_t tcp_launch(int runs, const void *batch_dev, int, int, int64_t slot_budget, hipStream_t)
+hipError_t tcp_launch(int runs, const void *batch_dev, const LaunchShape &s, hipStream_t)
 {
     if (runs == 25)
-        run_cpu(*(BatchT<25> *)batch_dev, slot_budget);
+        run_cpu(*(BatchT<25> *)batch_dev, s.slot_budget);
     else
-        run_cpu(*(BatchT<9> *)batch_dev, slot_budget);
+        run_cpu(*(BatchT<9> *)batch_dev, s.slot_budget);
+    return hipSuccess;
+}
+hipError_t tcp_max_blocks(int, int, int *per_cu)
+{
+    *per_cu = 4;
     return hipSuccess;
 }
 } // namespace tw

@@ -22,6 +22,7 @@ class TwError(RuntimeError):
 
 class PackCfg(ctypes.Structure):
     _fields_ = [("chain_runs", ctypes.c_int32), ("block_threads", ctypes.c_int32), ("blocks", ctypes.c_int32),
+                ("team_blocks", ctypes.c_int32), ("hosts_per_wave", ctypes.c_int32), ("pad", ctypes.c_int32),
                 ("slot_budget", ctypes.c_int64)]
 
 
@@ -86,11 +87,11 @@ class World:
 class Batch:
     """Worlds packed into one device batch and run by the persistent world-queue kernel."""
 
-    def __init__(self, worlds, chain_runs=9, block_threads=0, blocks=0, slot_budget=0):
+    def __init__(self, worlds, chain_runs=9, block_threads=0, blocks=0, team_blocks=0, hosts_per_wave=0, slot_budget=0):
         L = lib()
         self.worlds = list(worlds)
         hs = (ctypes.c_void_p * len(self.worlds))(*[w._h for w in self.worlds])
-        cfg = PackCfg(chain_runs, block_threads, blocks, slot_budget)
+        cfg = PackCfg(chain_runs, block_threads, blocks, team_blocks, hosts_per_wave, 0, slot_budget)
         self._h = ctypes.c_void_p()
         _check(L.tw_batch_create(hs, len(self.worlds), ctypes.byref(cfg), ctypes.byref(self._h)))
 

@@ -5,7 +5,15 @@
 
 namespace tw
 {
-// one launch of the persistent world-queue kernel over a packed batch (BatchT<runs> in device
-// memory); each world runs at most slot_budget slots in this launch
-hipError_t tcp_launch(int runs, const void *batch_dev, int blocks, int threads, int64_t slot_budget, hipStream_t stream);
+// a launch of the persistent world-queue kernel: `blocks` workgroups of `threads`, in teams of
+// `team` workgroups per world; `hpw` hosts per wavefront work item; each world runs at most
+// slot_budget slots per launch
+struct LaunchShape
+{
+    int blocks, threads, team, hpw;
+    int64_t slot_budget;
+};
+hipError_t tcp_launch(int runs, const void *batch_dev, const LaunchShape &shape, hipStream_t stream);
+// workgroups of `threads` that one CU holds at once (the cooperative-launch limit)
+hipError_t tcp_max_blocks(int runs, int threads, int *per_cu);
 } // namespace tw

@@ -1,55 +1,113 @@
 // The TCP slot kernels for gfx950.
 //
-// tcp_worlds_kernel: a persistent world-queue kernel. Each workgroup takes the next world from
-// the batch's queue and runs its slots one after another until the world finishes or the
-// launch's slot budget is spent; the slot barrier is the workgroup barrier, so a world's whole
-// state is touched by one CU and no grid-wide synchronisation exists. Within a slot, lanes run
-// hosts (one serial event loop per lane) and wavefronts run switches (one lane per port of
-// 64/k switches), with no barrier between the two: both read only the two previous slots' link
-// messages and write their own links' message for this slot (reference: src/kernel.cuh:5-113).
-// The last step of a slot is the ticket (one lane): finish test and the exact idle-slot skip.
+// tcp_worlds_kernel: a persistent world-queue kernel run by teams of T workgroups. A team takes
+// the next world from the batch's queue and runs its slots one after another until the world
+// finishes or the launch's slot budget is spent. Within a slot the team's wavefronts share the
+// world's entities: "work items" are groups of `hpw` hosts (one serial event loop per lane) and
+// groups of 64/k switches (one lane per port), dealt round-robin over the team's waves, with no
+// barrier between hosts and switches: both read only the two previous slots' link messages and
+// write their own links' message for this slot (reference: src/kernel.cuh:5-113). The slot ends
+// with the team barrier, whose last arrival is the ticket (finish test, exact idle-slot skip).
+//
+// T = 1 (the batched sweep: a world per workgroup) needs only the workgroup barrier, so a
+// world's state stays within one CU. T > 1 (one large world over many CUs) synchronises through
+// an agent-scope arrive counter and generation word per team; the launch is cooperative, so all
+// of a team's workgroups are resident, and every wait is bounded (a timeout ends the world with
+// an error flag instead of hanging the queue).
 #include "tcp_model.h"
 #include "tcp_kernel.h"
 
 namespace tw
 {
 
+constexpr int LDS_PER_WAVE = 64 * 33 + 64 * 3 + 64 + 64; // switch phase: rows, in_win, base, cnt
+constexpr long SPIN_LIMIT = 1L << 25;                    // team-barrier polls before giving up
+
+__device__ __forceinline__ unsigned ld_acquire(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
+
 template <int RUNS>
-__global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__restrict__ Bp, int64_t slot_budget)
+__global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__restrict__ Bp, int T, int hpw, int64_t slot_budget)
 {
     typedef Model<RUNS> Md;
     const BatchT<RUNS> &B = *Bp;
-    extern __shared__ int lds[]; // per wave: rows[64 * 33], in_win[64 * 3], base[64], cnt[64]
-    __shared__ int s_qi;
+    extern __shared__ int lds[];
+    __shared__ int s_w;
     __shared__ int64_t s_t;
-    __shared__ int s_done;
+    __shared__ int s_done, s_err;
     __shared__ unsigned s_act;
     __shared__ unsigned long long s_wake;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwaves = blockDim.x >> 6;
-    constexpr int LDS_PER_WAVE = 64 * 33 + 64 * 3 + 64 + 64;
+    const int team = blockIdx.x / T, rank = blockIdx.x % T;
+    const int TW = T * nwaves, gw = rank * nwaves + wv; // the team's waves; this wave's place
+    TeamState *ts = B.teams + team;
     int *wl = lds + wv * LDS_PER_WAVE;
-    const int gwave = blockIdx.x * nwaves + wv; // this wave's scratch
+    const int gwave = blockIdx.x * nwaves + wv; // this wave's global scratch
     int64_t *scr_ts = B.scr_ts + (size_t)gwave * 64 * 2 * MAX_TX;
     uint32_t *scr_ref = B.scr_ref + (size_t)gwave * 64 * 2 * MAX_TX;
     uint8_t *scr_up = B.scr_up + (size_t)gwave * 64 * 2 * MAX_TX;
+    unsigned gen = 0; // the team barrier's generation this block has passed
+
+    // the team barrier; the last workgroup to arrive runs `last` first
+    auto team_barrier = [&](auto last) -> bool {
+        __syncthreads();
+        if (tid == 0)
+        {
+            if (T == 1)
+                last();
+            else
+            {
+                __threadfence();
+                unsigned g = gen;
+                if (atomicAdd(&ts->arrive, 1u) == (unsigned)T - 1)
+                {
+                    last();
+                    atomicExch(&ts->arrive, 0u);
+                    __threadfence();
+                    atomicAdd(&ts->gen, 1u);
+                }
+                else
+                {
+                    for (long spins = 0; ld_acquire(&ts->gen) == g; spins++)
+                    {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (spins > SPIN_LIMIT)
+                        {
+                            atomicExch(&ts->err, 1);
+                            break;
+                        }
+                    }
+                }
+                __threadfence();
+                gen = g + 1;
+            }
+            s_err = T > 1 && __hip_atomic_load(&ts->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        return s_err;
+    };
+
     for (;;)
     {
+        // the team's next world
+        if (team_barrier([&] { ts->world = atomicAdd(B.queue_next, 1); }))
+            return;
         if (tid == 0)
-            s_qi = atomicAdd(B.queue_next, 1);
+            s_w = __hip_atomic_load(&ts->world, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        const int qi = s_qi;
+        const int qi = s_w;
         if (qi >= B.nworlds)
             return;
         const int w = B.world_order[qi];
         const WorldDesc &W = B.worlds[w];
         WorldRun *R = B.run + w;
-        const int k = W.k, spw = 64 / k, ngroups = (W.S + spw - 1) / spw;
+        const int k = W.k, spw = 64 / k;
+        const int host_items = (W.H + hpw - 1) / hpw, items = host_items + (W.S + spw - 1) / spw;
         for (int64_t it = 0;; it++)
         {
             if (tid == 0)
             {
-                s_t = R->slot;
-                s_done = R->finished || it >= slot_budget;
+                s_t = __hip_atomic_load(&R->slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_done = __hip_atomic_load(&R->finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || it >= slot_budget;
                 s_act = 0;
                 s_wake = ~0ull;
             }
@@ -59,15 +117,20 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
             const int64_t t = s_t;
             unsigned act = 0;
             unsigned long long wk = ~0ull;
-            for (int hl = tid; hl < W.H; hl += blockDim.x)
+            for (int item = gw; item < items; item += TW)
             {
-                int64_t x = INT64_MAX;
-                act += Md::host_step(B, W, R, hl, t, &x);
-                wk = (unsigned long long)x < wk ? (unsigned long long)x : wk;
-            }
-            for (int g = wv; g < ngroups; g += nwaves)
-            {
-                const int sl = lane / k, base = (sl < spw ? sl : 0) * k;
+                if (item < host_items)
+                {
+                    const int hl = item * hpw + lane;
+                    if (lane < hpw && hl < W.H)
+                    {
+                        int64_t x = INT64_MAX;
+                        act += Md::host_step(B, W, R, hl, t, &x);
+                        wk = (unsigned long long)x < wk ? (unsigned long long)x : wk;
+                    }
+                    continue;
+                }
+                const int g = item - host_items, sl = lane / k, base = (sl < spw ? sl : 0) * k;
                 typename Md::SwLane x;
                 x.swl = g * spw + sl;
                 x.q = lane % k;
@@ -89,13 +152,20 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
             if (wk != ~0ull)
                 atomicMin(&s_wake, wk);
             __syncthreads();
-            if (tid == 0)
+            if (tid == 0 && T > 1)
             {
-                // the ticket (src/kernel.cuh:79-112)
-                unsigned cur = s_act + R->act_cur, prev = R->act_prev;
-                bool fin = atomicAdd(&R->flows_done, 0u) >= (unsigned)W.M || t + 1 >= W.max_slots; // (atomics bypass L1)
+                if (s_act)
+                    atomicAdd(&ts->act, s_act);
+                if (s_wake != ~0ull)
+                    atomicMin(&ts->wake, s_wake);
+            }
+            const bool err = team_barrier([&] {
+                // the ticket (src/kernel.cuh:79-112): finish, or the next slot with the exact idle skip
+                unsigned a = T > 1 ? atomicExch(&ts->act, 0u) : s_act;
+                unsigned long long nw = T > 1 ? atomicExch(&ts->wake, ~0ull) : s_wake;
+                unsigned cur = a + R->act_cur, prev = R->act_prev;
+                bool fin = atomicAdd(&R->flows_done, 0u) >= (unsigned)W.M || t + 1 >= W.max_slots;
                 int64_t next = t + 1;
-                unsigned long long nw = s_wake;
                 if (cur == 0 && prev == 0 && nw != ~0ull && (int64_t)nw > next)
                     next = (int64_t)nw < W.max_slots ? (int64_t)nw : W.max_slots;
                 R->act_prev = cur;
@@ -104,24 +174,48 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
                 R->slots_run++;
                 if (fin)
                     R->finished = 1;
+            });
+            if (err)
+            {
+                if (tid == 0)
+                {
+                    R->error = 1;
+                    R->finished = 1;
+                }
+                return;
             }
         }
     }
 }
 
 template <int RUNS>
-static hipError_t launch(const void *batch_dev, int blocks, int threads, int64_t slot_budget, hipStream_t stream)
+static hipError_t launch(const void *batch_dev, const LaunchShape &s, hipStream_t stream)
 {
-    size_t lds = (size_t)(threads / 64) * (64 * 33 + 64 * 3 + 64 + 64) * sizeof(int);
-    hipLaunchKernelGGL(tcp_worlds_kernel<RUNS>, dim3(blocks), dim3(threads), lds, stream,
-                       reinterpret_cast<const BatchT<RUNS> *>(batch_dev), slot_budget);
+    const size_t lds = (size_t)(s.threads / 64) * LDS_PER_WAVE * sizeof(int);
+    const BatchT<RUNS> *b = reinterpret_cast<const BatchT<RUNS> *>(batch_dev);
+    int T = s.team, hpw = s.hpw;
+    int64_t budget = s.slot_budget;
+    if (s.team > 1)
+    {
+        // every workgroup of a team must be resident: a cooperative launch refuses otherwise
+        void *args[] = {&b, &T, &hpw, &budget};
+        return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&tcp_worlds_kernel<RUNS>), dim3(s.blocks), dim3(s.threads),
+                                          args, (unsigned)lds, stream);
+    }
+    hipLaunchKernelGGL(tcp_worlds_kernel<RUNS>, dim3(s.blocks), dim3(s.threads), lds, stream, b, T, hpw, budget);
     return hipGetLastError();
 }
 
-hipError_t tcp_launch(int runs, const void *batch_dev, int blocks, int threads, int64_t slot_budget, hipStream_t stream)
+hipError_t tcp_launch(int runs, const void *batch_dev, const LaunchShape &shape, hipStream_t stream)
 {
-    return runs == 25 ? launch<25>(batch_dev, blocks, threads, slot_budget, stream)
-                      : launch<9>(batch_dev, blocks, threads, slot_budget, stream);
+    return runs == 25 ? launch<25>(batch_dev, shape, stream) : launch<9>(batch_dev, shape, stream);
+}
+
+hipError_t tcp_max_blocks(int runs, int threads, int *per_cu)
+{
+    const size_t lds = (size_t)(threads / 64) * LDS_PER_WAVE * sizeof(int);
+    return runs == 25 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, tcp_worlds_kernel<25>, threads, lds)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, tcp_worlds_kernel<9>, threads, lds);
 }
 
 } // namespace tw

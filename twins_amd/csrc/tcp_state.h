@@ -154,6 +154,20 @@ struct WorldRun
     unsigned long long ev[EV_N];
 };
 
+// a team's barrier and reductions (teams of several workgroups run one world), each word on its
+// own cache line
+struct alignas(128) TeamState
+{
+    unsigned arrive;
+    unsigned pad0[31];
+    unsigned gen;
+    unsigned pad1[31];
+    int32_t world, err;
+    unsigned act;
+    unsigned pad2;
+    unsigned long long wake;
+};
+
 // every device pointer of a batch (one copy in device memory, passed by pointer)
 template <int RUNS>
 struct BatchT
@@ -199,9 +213,10 @@ struct BatchT
     uint8_t *g_rt_n, *g_rt_ports, *g_port_dn;
     int64_t *g_fail_t;
     TwMrgState *sw_rng; // per-packet ECMP streams
-    // kernel work queue: worlds in the order blocks take them
+    // kernel work queue: worlds in the order teams take them; the teams' barriers
     int32_t *world_order;
     int32_t *queue_next;
+    TeamState *teams;
     // switch-phase scratch, per wave: [wave][64 * 2 * MAX_TX] arrival times, refs, drawn routes
     int64_t *scr_ts;
     uint32_t *scr_ref;

@@ -16,6 +16,7 @@ static int usage()
     fprintf(stderr, "usage: twins (--k=<4..32> | --topo=leafspine --leaves=L --spines=S --hpl=N | --topo=graph --topofile=F)\n"
                     "         (--flows=<csv> [--out=csv] [--tcp=dctcp|newreno|cubic] [--ecnK=30] [--queue=200] [--fin=0|1]\n"
                     "          [--ecmp_from=csv] [--spray=1 [--RngSeed=1 --RngRun=1]] [--chain_runs=9|25]\n"
+                    "          [--block_threads=256 --blocks=N --team_blocks=T --hosts_per_wave=H] (launch shape; 0: automatic)\n"
                     "          | --worlds=<file>: one world per line, flows=..,tcp=..,ecnK=..,queue=..,fin=..,ecmp_from=..,out=..)\n"
                     "         leaf-spine: max(N+S, L) <= 32\n");
     return 1;
@@ -25,7 +26,7 @@ int main(int argc, char **argv)
 {
     std::vector<std::string> common; // topology and defaults, passed to every world
     std::string flows, out = "fct.csv", worlds_file, tcp = "dctcp", ecmp_from;
-    int runs = 9, threads = 0, blocks = 0;
+    int runs = 9, threads = 0, blocks = 0, team = 0, hpw = 0;
     std::string ecn_k, queue, fin;
     for (int i = 1; i < argc; i++)
     {
@@ -54,6 +55,10 @@ int main(int argc, char **argv)
             threads = atoi(val("--block_threads=").c_str());
         else if (is("--blocks="))
             blocks = atoi(val("--blocks=").c_str());
+        else if (is("--team_blocks="))
+            team = atoi(val("--team_blocks=").c_str());
+        else if (is("--hosts_per_wave="))
+            hpw = atoi(val("--hosts_per_wave=").c_str());
         else if (a == "--general" || is("--graph_k=") || is("--host_lanes=") || is("--hash_check=") || is("--qtrace"))
             ; // the reference's kernel-shape and debugging switches: nothing to choose here
         else if (is("--k=") || is("--topo=") || is("--topofile=") || is("--fail=") || is("--leaves=") || is("--spines=") ||
@@ -158,6 +163,8 @@ int main(int argc, char **argv)
     cfg.chain_runs = runs;
     cfg.block_threads = threads;
     cfg.blocks = blocks;
+    cfg.team_blocks = team;
+    cfg.hosts_per_wave = hpw;
     tw_batch *b = nullptr;
     if (tw_batch_create(hw.data(), (int)hw.size(), &cfg, &b))
     {

@@ -842,8 +842,9 @@ struct tw_batch
     std::vector<int32_t> msg0, conn0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int blocks = 0, threads = 256;
-    int64_t slot_budget = 0;
+    LaunchShape shape{};
+    TeamState *teams = nullptr;
+    int nteams = 0;
     int32_t *queue_next = nullptr;
     WorldRun *run = nullptr;
     std::vector<WorldRun> run_host;
@@ -1098,20 +1099,44 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     });
     UP(world_order, order);
     ZERO(queue_next, 1);
-    // launch shape
-    int dev = 0, cus = 256;
+    // launch shape: teams of T workgroups (one world each), `hpw` hosts per wave work item
+    int dev = 0, cus = 256, per_cu = 1;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    b.threads = cfg.block_threads > 0 ? cfg.block_threads : 256;
-    if (b.threads % 64 || b.threads > 1024)
+    LaunchShape &sh = b.shape;
+    sh.threads = cfg.block_threads > 0 ? cfg.block_threads : 256;
+    if (sh.threads % 64 || sh.threads > 1024)
         return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 1024");
-    b.blocks = cfg.blocks > 0 ? cfg.blocks : std::min(W, 4 * cus);
-    b.slot_budget = cfg.slot_budget > 0 ? cfg.slot_budget : 200000;
-    const size_t waves = (size_t)b.blocks * (b.threads / 64);
+    HIPCHK(tcp_max_blocks(RUNS, sh.threads, &per_cu));
+    const int resident = std::max(1, per_cu) * cus;
+    int64_t largest = 0;
+    for (auto &w : b.worlds)
+        largest = std::max<int64_t>(largest, (int64_t)w->H + w->S);
+    if (cfg.team_blocks > 0)
+        sh.team = cfg.team_blocks;
+    else // spread few large worlds over the GPU; many worlds run a workgroup each
+        sh.team = W >= cus ? 1 : std::max(1, std::min<int>(resident / W, (int)((largest + 63) / 64)));
+    int teams = cfg.blocks > 0 ? cfg.blocks / sh.team : std::min(W, resident / sh.team);
+    if (teams < 1 || (sh.team > 1 && teams * sh.team > resident))
+        return fail(TW_EINVAL, "%d teams of %d workgroups do not fit (%d resident workgroups)", teams, sh.team, resident);
+    sh.blocks = teams * sh.team;
+    if (cfg.hosts_per_wave > 0)
+        sh.hpw = std::min(64, cfg.hosts_per_wave);
+    else
+    {
+        // about one work item per wave of the team
+        const int64_t tw = (int64_t)sh.team * (sh.threads / 64);
+        sh.hpw = sh.team == 1 ? 64 : (int)std::max<int64_t>(1, std::min<int64_t>(64, (largest + tw - 1) / tw));
+    }
+    sh.slot_budget = cfg.slot_budget > 0 ? cfg.slot_budget : 200000;
+    ZERO(teams, teams);
+    const size_t waves = (size_t)sh.blocks * (sh.threads / 64);
     B.scr_waves = (int32_t)waves;
     ZERO(scr_ts, waves * 64 * 2 * MAX_TX);
     ZERO(scr_ref, waves * 64 * 2 * MAX_TX);
     ZERO(scr_up, waves * 64 * 2 * MAX_TX);
+    b.teams = B.teams;
+    b.nteams = teams;
 #undef UP
 #undef ZERO
     b.queue_next = B.queue_next;
@@ -1176,8 +1201,9 @@ extern "C" int tw_batch_run(tw_batch *b, tw_run_stats *stats)
         if (!unfinished)
             break;
         HIPCHK(hipMemsetAsync(b->queue_next, 0, sizeof(int32_t), b->stream));
+        HIPCHK(hipMemsetAsync(b->teams, 0, sizeof(TeamState) * b->nteams, b->stream));
         HIPCHK(hipEventRecord(b->ev0, b->stream));
-        HIPCHK(tcp_launch(b->runs, b->batch_dev, b->blocks, b->threads, b->slot_budget, b->stream));
+        HIPCHK(tcp_launch(b->runs, b->batch_dev, b->shape, b->stream));
         HIPCHK(hipEventRecord(b->ev1, b->stream));
         HIPCHK(hipMemcpyAsync(b->run_host.data(), b->run, sizeof(WorldRun) * W, hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipStreamSynchronize(b->stream));
@@ -1189,6 +1215,9 @@ extern "C" int tw_batch_run(tw_batch *b, tw_run_stats *stats)
             return fail(TW_ESTATE, "no progress after %lld launches", (long long)st.launches);
     }
     st.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (auto &r : b->run_host)
+        if (r.error)
+            return fail(TW_EDEVICE, "a team barrier timed out: the world's results are incomplete");
     for (auto &r : b->run_host)
     {
         st.slots_max = std::max<int64_t>(st.slots_max, r.slots_run);
