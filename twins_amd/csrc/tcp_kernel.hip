@@ -117,17 +117,31 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
             const int64_t t = s_t;
             unsigned act = 0;
             unsigned long long wk = ~0ull;
+#ifdef TW_PROF
+            __shared__ unsigned long long p_host, p_sw;
+            const unsigned long long p0 = wall_clock64();
+            unsigned long long p_h = 0, p_s = 0;
+            if (tid == 0)
+                p_host = p_sw = 0;
+            __syncthreads();
+#endif
             for (int item = gw; item < items; item += TW)
             {
                 if (item < host_items)
                 {
                     const int hl = item * hpw + lane;
+#ifdef TW_PROF
+                    const unsigned long long q0 = wall_clock64();
+#endif
                     if (lane < hpw && hl < W.H)
                     {
                         int64_t x = INT64_MAX;
                         act += Md::host_step(B, W, R, hl, t, &x);
                         wk = (unsigned long long)x < wk ? (unsigned long long)x : wk;
                     }
+#ifdef TW_PROF
+                    p_h += wall_clock64() - q0;
+#endif
                     continue;
                 }
                 const int g = item - host_items, sl = lane / k, base = (sl < spw ? sl : 0) * k;
@@ -145,8 +159,18 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
                 L.in_win = wl + 64 * 33 + base * 3;
                 L.base = wl + 64 * 33 + 64 * 3 + base;
                 L.cnt = wl + 64 * 33 + 64 * 3 + 64 + base;
+#ifdef TW_PROF
+                const unsigned long long q0 = wall_clock64();
+#endif
                 act += Md::switch_step(B, W, R, x, t, L);
+#ifdef TW_PROF
+                p_s += wall_clock64() - q0;
+#endif
             }
+#ifdef TW_PROF
+            atomicMax(&p_host, p_h);
+            atomicMax(&p_sw, p_s);
+#endif
             if (act)
                 atomicAdd(&s_act, act);
             if (wk != ~0ull)
@@ -159,6 +183,14 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
                 if (s_wake != ~0ull)
                     atomicMin(&ts->wake, s_wake);
             }
+#ifdef TW_PROF
+            if (tid == 0)
+            {
+                atomicMax(&ts->prof[0], p_host);
+                atomicMax(&ts->prof[1], p_sw);
+                atomicMax(&ts->prof[2], wall_clock64() - p0);
+            }
+#endif
             const bool err = team_barrier([&] {
                 // the ticket (src/kernel.cuh:79-112): finish, or the next slot with the exact idle skip
                 unsigned a = T > 1 ? atomicExch(&ts->act, 0u) : s_act;
@@ -168,6 +200,17 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
                 int64_t next = t + 1;
                 if (cur == 0 && prev == 0 && nw != ~0ull && (int64_t)nw > next)
                     next = (int64_t)nw < W.max_slots ? (int64_t)nw : W.max_slots;
+#ifdef TW_PROF
+                {
+                    const unsigned long long now = wall_clock64();
+                    if (ts->prof[3])
+                        R->ev[EV_PROF_PERIOD] += now - ts->prof[3];
+                    ts->prof[3] = now;
+                    R->ev[EV_PROF_HOSTS] += atomicExch(&ts->prof[0], 0ull);
+                    R->ev[EV_PROF_SWITCHES] += atomicExch(&ts->prof[1], 0ull);
+                    R->ev[EV_PROF_WORK] += atomicExch(&ts->prof[2], 0ull);
+                }
+#endif
                 R->act_prev = cur;
                 R->act_cur = 0;
                 R->slot = next;

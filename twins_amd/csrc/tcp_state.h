@@ -56,6 +56,12 @@ enum : int
     EV_RING_OVF = 19,
     EV_UNDECIDED = 25,
     EV_PKT_EVENTS = 26, // frames started on a link (packet-events)
+    // TW_PROF builds: wall-clock ticks summed over slots: slot period, slowest workgroup's host
+    // work, slowest workgroup's switch work, slowest workgroup's total work
+    EV_PROF_PERIOD = 27,
+    EV_PROF_HOSTS = 28,
+    EV_PROF_SWITCHES = 29,
+    EV_PROF_WORK = 30,
     EV_N = 32
 };
 
@@ -166,6 +172,7 @@ struct alignas(128) TeamState
     unsigned act;
     unsigned pad2;
     unsigned long long wake;
+    unsigned long long prof[4]; // TW_PROF: this slot's maxima (hosts, switches, work); last ticket time
 };
 
 // every device pointer of a batch (one copy in device memory, passed by pointer)

@@ -203,6 +203,9 @@ int main(int argc, char **argv)
            M, (unsigned long long)st.flows_done, st.wall_s, ev[0], ev[1], ev[2], ev[3], ev[4], ev[5], ev[6], ev[7], ev[8], ev[9], ev[10],
            ev[11], ev[12], ev[13], ev[14], ev[19], ev[25]);
     printf("packet-events %llu kernel %.6f s launches %lld\n", (unsigned long long)st.packet_events, st.kernel_s, (long long)st.launches);
+    if (ev[27]) // a TW_PROF build: wall-clock ticks (100 MHz) summed over slots
+        printf("prof: per slot: period %.2f us, slowest workgroup: hosts %.2f us, switches %.2f us, work %.2f us\n", ev[27] / 100.0 / slots,
+               ev[28] / 100.0 / slots, ev[29] / 100.0 / slots, ev[30] / 100.0 / slots);
     if (ev[5] || ev[6] || ev[7] || ev[14] || ev[19])
         fprintf(stderr, "warning: a capacity overflowed (active/ooo/due/rtt/ring-ovf); the results may differ from ns-3\n");
     if (ev[25])
