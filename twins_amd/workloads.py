@@ -112,6 +112,144 @@ def config2_flows(seed=2, n=20000, k=16, load=0.5, alpha=1.2, xm=1000, cap=30_00
     return "src,dst,flow_size,start_ns,conn\n" + "".join(f"{a},{b},{c},{d},{e}\n" for a, b, c, d, e in rows)
 
 
+def leafspine_topology(leaves, spines, hpl):
+    """Leaf-spine as a topology file (hosts on their leaf's first ports, then leaf-spine links)."""
+    links = [(f"h{h}", f"s{h // hpl}") for h in range(leaves * hpl)]
+    links += [(f"s{l}", f"s{leaves + j}") for l in range(leaves) for j in range(spines)]
+    lines = [f"hosts {leaves * hpl}", f"switches {leaves + spines}"] + [f"link {a} {b}" for a, b in links]
+    return "\n".join(lines) + "\n"
+
+
+def jellyfish_topology(n, d, hps, seed):
+    """A Jellyfish network (Singla et al., NSDI 2012): random d-regular switch graph built by
+    joining random switch pairs with free ports that are not yet neighbours, breaking a random
+    link to absorb a switch left with two or more free ports; hps hosts per switch."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    free = [d] * n
+    adj = [set() for _ in range(n)]
+    edges = []
+    while True:
+        avail = [x for x in range(n) if free[x] > 0]
+        a = b = -1
+        for _ in range(64 if len(avail) > 1 else 0):  # a random joinable pair, sampled while plentiful
+            u, v = avail[int(rng.integers(len(avail)))], avail[int(rng.integers(len(avail)))]
+            if u != v and v not in adj[u]:
+                a, b = min(u, v), max(u, v)
+                break
+        if a < 0:
+            pairs = [(u, v) for i, u in enumerate(avail) for v in avail[i + 1:] if v not in adj[u]]
+            if pairs:
+                a, b = pairs[int(rng.integers(len(pairs)))]
+        if a < 0:
+            stuck = [x for x in range(n) if free[x] >= 2]
+            if not stuck:
+                break
+            x = stuck[int(rng.integers(len(stuck)))]
+            cand = [e for e in edges if x not in e and e[0] not in adj[x] and e[1] not in adj[x]]
+            if not cand:
+                break
+            y, z = cand[int(rng.integers(len(cand)))]
+            edges.remove((y, z))
+            adj[y].discard(z)
+            adj[z].discard(y)
+            free[y] += 1
+            free[z] += 1
+            for a, b in ((x, y), (x, z)):
+                adj[a].add(b)
+                adj[b].add(a)
+                edges.append((min(a, b), max(a, b)))
+                free[a] -= 1
+                free[b] -= 1
+            continue
+        adj[a].add(b)
+        adj[b].add(a)
+        edges.append((a, b))
+        free[a] -= 1
+        free[b] -= 1
+    H = n * hps
+    links = [(f"h{h}", f"s{h // hps}") for h in range(H)] + [(f"s{a}", f"s{b}") for a, b in edges]
+    return "\n".join([f"hosts {H}", f"switches {n}"] + [f"link {a} {b}" for a, b in links]) + "\n"
+
+
+def torus_topology(x, y, hps):
+    """A 2-D torus of x * y switches (4 network ports each), hps hosts per switch."""
+    n = x * y
+    sw = lambda i, j: (i % x) * y + (j % y)
+    edges = []
+    for i in range(x):
+        for j in range(y):
+            edges.append((sw(i, j), sw(i + 1, j)))
+            edges.append((sw(i, j), sw(i, j + 1)))
+    H = n * hps
+    links = [(f"h{h}", f"s{h // hps}") for h in range(H)] + [(f"s{a}", f"s{b}") for a, b in edges]
+    return "\n".join([f"hosts {H}", f"switches {n}"] + [f"link {a} {b}" for a, b in links]) + "\n"
+
+
+def pareto_flows(seed, n, H, load=0.4, alpha=1.2, xm=1000, cap=30_000_000):
+    """n Pareto-sized flows between random host pairs, per-host Poisson starts at `load`."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    mean = alpha * xm / (alpha - 1) * (1 - (xm / cap) ** (alpha - 1))
+    rate_per_ns = load * LINK_BPS / 8 / 1e9 / mean
+    per_host = n // H + 2
+    starts = np.floor(np.cumsum(rng.exponential(1.0 / rate_per_ns, (H, per_host)), axis=1)).astype(np.int64)
+    order = np.argsort(starts, axis=None, kind="stable")[:n]
+    hosts, idx = np.unravel_index(order, starts.shape)
+    src, start = hosts.astype(np.int64), starts[hosts, idx]
+    size = np.minimum(np.floor(xm * (1 + rng.pareto(alpha, n))), cap).astype(np.int64)
+    dst = rng.integers(0, H - 1, n)
+    dst = dst + (dst >= src)
+    rows = sorted(zip(src.tolist(), dst.tolist(), size.tolist(), start.tolist()), key=lambda r: (r[3], r[0]))
+    return "src,dst,flow_size,start_ns\n" + "".join(f"{a},{b},{c},{d}\n" for a, b, c, d in rows)
+
+
+def config4_worlds(outdir, n_worlds=256, seed=4):
+    """The batched sweep: n_worlds worlds drawn by seed. Returns a list of `twins` argument lists
+    (each with --flows, topology and parameters) and writes the files they name into outdir."""
+    os.makedirs(outdir, exist_ok=True)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    worlds = []
+    for w in range(n_worlds):
+        kind = int(rng.integers(4))
+        if kind == 0:  # FatTree
+            k = int(rng.choice([4, 8, 16], p=[0.4, 0.4, 0.2]))
+            topo, H = [f"--k={k}"], k ** 3 // 4
+        elif kind == 1:  # leaf-spine
+            leaves = int(rng.integers(8, 33))
+            spines, hpl = int(rng.choice([4, 8])), 16
+            tf = os.path.join(outdir, f"w{w:03d}_ls.txt")
+            text = leafspine_topology(leaves, spines, hpl)
+            topo, H = [f"--topo=leafspine", f"--leaves={leaves}", f"--spines={spines}", f"--hpl={hpl}"], leaves * hpl
+        elif kind == 2:  # Jellyfish
+            n = int(rng.integers(32, 257))
+            d = int(rng.choice([4, 6, 8]))
+            hps = int(rng.integers(2, 9))
+            tf = os.path.join(outdir, f"w{w:03d}_jf.txt")
+            text = jellyfish_topology(n, d, hps, seed * 1000 + w)
+            topo, H = ["--topo=graph", f"--topofile={tf}"], n * hps
+        else:  # torus
+            x = int(rng.integers(6, 17))
+            y = int(rng.integers(6, 17))
+            hps = int(rng.integers(2, 9))
+            tf = os.path.join(outdir, f"w{w:03d}_torus.txt")
+            text = torus_topology(x, y, hps)
+            topo, H = ["--topo=graph", f"--topofile={tf}"], x * y * hps
+        if kind in (2, 3):
+            if not os.path.exists(tf) or open(tf).read() != text:
+                with open(tf, "w") as f:
+                    f.write(text)
+        queue = int(rng.choice([100, 200, 400]))
+        ecn = int(rng.choice([k for k in (10, 30, 65, 100, 200) if k < queue]))
+        tcp = str(rng.choice(["newreno", "dctcp", "cubic"]))
+        nflows = int(rng.integers(1000, 20001))
+        ff = os.path.join(outdir, f"w{w:03d}_flows.csv")
+        text = pareto_flows(seed * 100000 + w, nflows, H)
+        if not os.path.exists(ff) or open(ff).read() != text:
+            with open(ff, "w") as f:
+                f.write(text)
+        worlds.append(topo + [f"--flows={ff}", f"--tcp={tcp}", f"--ecnK={ecn}", f"--queue={queue}"])
+    return worlds
+
+
 CONFIG2_ARGS = ["--topo=graph", "--spray=1", "--RngRun=1", "--fin=1", "--fail=300000:s256:s128,700000:s256:s128:up"]
 
 
