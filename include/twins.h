@@ -78,6 +78,20 @@ typedef struct tw_world_info
 int tw_world_info_get(const tw_world *w, tw_world_info *info);
 int tw_world_destroy(tw_world *w);
 
+/* An explicit launch plan (the batched sweep's autotuner writes one). Team j has team_blocks[j]
+ * workgroups and takes worlds from queue team_queue[j]; world i waits in queue world_queue[i]; a
+ * queue's worlds are taken in the order they appear in world_order (a permutation of 0..n-1;
+ * NULL: batch order). No counterpart in the reference, whose sweep (--worlds) runs every world in
+ * one fixed launch shape (src/main.cu:807-1015). */
+typedef struct tw_plan
+{
+    int32_t nteams, nqueues;
+    const int32_t *team_blocks;  /* [nteams] */
+    const int32_t *team_queue;   /* [nteams] */
+    const int32_t *world_queue;  /* [nworlds] */
+    const int32_t *world_order;  /* [nworlds] or NULL */
+} tw_plan;
+
 /* Packing of worlds into one device batch and the launch shape (0: automatic). */
 typedef struct tw_pack_cfg
 {
@@ -86,9 +100,14 @@ typedef struct tw_pack_cfg
     int32_t blocks;          /* persistent workgroups (a multiple of team_blocks) */
     int32_t team_blocks;     /* workgroups that run one world together (1: a world per workgroup) */
     int32_t hosts_per_wave;  /* hosts a wavefront runs side by side (1..64) */
-    int32_t pad;
+    int32_t no_graph;        /* 1: launch directly, not through the captured HIP graph */
     int64_t slot_budget;     /* slots a world may run per kernel launch */
+    const tw_plan *plan;     /* teams and queues (NULL: one queue, teams from the fields above) */
 } tw_pack_cfg;
+
+/* Workgroups of block_threads (0: 256) that the GPU holds at once: the most a plan with teams of
+ * several workgroups may use. */
+int tw_launch_capacity(int chain_runs, int block_threads, int32_t *resident_blocks);
 
 /* Packs worlds (copied; the handles stay the caller's) into device memory.
  * (src/main.cu:267-326, 807-1015) */
@@ -103,6 +122,7 @@ typedef struct tw_run_stats
     uint64_t packet_events;  /* frames started on links, all worlds */
     uint64_t flows_done;
     int32_t worlds_done;
+    int32_t graph;           /* 1: the launches went through a captured HIP graph */
 } tw_run_stats;
 
 /* Runs every world of the batch to completion (all flows done or max_slots).
@@ -117,6 +137,18 @@ int tw_fct_get(tw_batch *b, int world, int64_t *established_ns, int64_t *complet
  * active/ooo/due overflows, retx by state, max in-flight items, rtt/ring overflows, undecided
  * ties (25), packet-events (26); executed slots in [31]. (src/main.cu:1054-1075) */
 int tw_counters_get(tw_batch *b, int world, uint64_t ev[32]);
+
+/* One world's run: slots executed, the device time its team spent on it (seconds of the GPU's
+ * constant clock, summed over launches), the team's workgroups, whether it finished, and its
+ * packet-events. No counterpart in the reference (the autotuner's measurements). */
+typedef struct tw_world_stats
+{
+    int64_t slots;
+    double device_s;
+    int32_t team_blocks, finished;
+    uint64_t packet_events;
+} tw_world_stats;
+int tw_world_stats_get(tw_batch *b, int world, tw_world_stats *st);
 
 int tw_batch_destroy(tw_batch *b);
 const char *tw_last_error(void);

@@ -81,5 +81,16 @@ hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t)
     return hipSuccess;
 }
 hipError_t hipGetLastError(void) { return hipSuccess; }
+// no graphs on the CPU: the capture is refused and the batch launches directly
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorNotSupported; }
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g)
+{
+    *g = nullptr;
+    return hipErrorNotSupported;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, hipGraphNode_t *, char *, size_t) { return hipErrorNotSupported; }
+hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorNotSupported; }
+hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
 const char *hipGetErrorString(hipError_t) { return "host stand-in"; }
 }

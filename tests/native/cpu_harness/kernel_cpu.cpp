@@ -44,9 +44,10 @@ static void run_cpu(BatchT<RUNS> &B, int64_t budget)
 {
     typedef Model<RUNS> Md;
     std::vector<int> lds(64 * 33 + 64 * 3 + 64 + 64);
-    for (int qi = *B.queue_next; qi < B.nworlds; qi = ++*B.queue_next)
+    for (int wq = 0; wq < B.nwq; wq++)
+    for (int qi = B.queue_next[wq]; qi < B.wq_len[wq]; qi = ++B.queue_next[wq])
     {
-        const int w = B.world_order[qi];
+        const int w = B.world_order[B.wq_off[wq] + qi];
         const WorldDesc &W = B.worlds[w];
         WorldRun *R = B.run + w;
         const int k = W.k;
@@ -107,6 +108,7 @@ static void run_cpu(BatchT<RUNS> &B, int64_t budget)
             if (fin)
                 R->finished = 1;
         }
+        R->team = 1;
     }
 }
 
@@ -116,6 +118,12 @@ hipError_t tcp_launch(int runs, const void *batch_dev, const LaunchShape &s, hip
         run_cpu(*(BatchT<25> *)batch_dev, s.slot_budget);
     else
         run_cpu(*(BatchT<9> *)batch_dev, s.slot_budget);
+    return hipSuccess;
+}
+hipError_t tcp_reset(uint32_t *queue_next, int nq, void *teams, size_t team_bytes, hipStream_t)
+{
+    std::fill(queue_next, queue_next + nq, 0u);
+    std::fill((char *)teams, (char *)teams + team_bytes, (char)0);
     return hipSuccess;
 }
 hipError_t tcp_max_blocks(int, int, int *per_cu)

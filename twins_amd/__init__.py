@@ -20,16 +20,27 @@ class TwError(RuntimeError):
     pass
 
 
+class Plan(ctypes.Structure):
+    _fields_ = [("nteams", ctypes.c_int32), ("nqueues", ctypes.c_int32), ("team_blocks", ctypes.POINTER(ctypes.c_int32)),
+                ("team_queue", ctypes.POINTER(ctypes.c_int32)), ("world_queue", ctypes.POINTER(ctypes.c_int32)),
+                ("world_order", ctypes.POINTER(ctypes.c_int32))]
+
+
 class PackCfg(ctypes.Structure):
     _fields_ = [("chain_runs", ctypes.c_int32), ("block_threads", ctypes.c_int32), ("blocks", ctypes.c_int32),
-                ("team_blocks", ctypes.c_int32), ("hosts_per_wave", ctypes.c_int32), ("pad", ctypes.c_int32),
-                ("slot_budget", ctypes.c_int64)]
+                ("team_blocks", ctypes.c_int32), ("hosts_per_wave", ctypes.c_int32), ("no_graph", ctypes.c_int32),
+                ("slot_budget", ctypes.c_int64), ("plan", ctypes.POINTER(Plan))]
 
 
 class RunStats(ctypes.Structure):
     _fields_ = [("wall_s", ctypes.c_double), ("kernel_s", ctypes.c_double), ("launches", ctypes.c_int64),
                 ("slots_max", ctypes.c_int64), ("packet_events", ctypes.c_uint64), ("flows_done", ctypes.c_uint64),
-                ("worlds_done", ctypes.c_int32)]
+                ("worlds_done", ctypes.c_int32), ("graph", ctypes.c_int32)]
+
+
+class WorldStats(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int64), ("device_s", ctypes.c_double), ("team_blocks", ctypes.c_int32),
+                ("finished", ctypes.c_int32), ("packet_events", ctypes.c_uint64)]
 
 
 class WorldInfo(ctypes.Structure):
@@ -57,6 +68,8 @@ def lib():
         L.tw_batch_run.argtypes = [vp, ctypes.POINTER(RunStats)]
         L.tw_fct_get.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
         L.tw_counters_get.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_uint64)]
+        L.tw_world_stats_get.argtypes = [vp, i32, ctypes.POINTER(WorldStats)]
+        L.tw_launch_capacity.argtypes = [i32, i32, ctypes.POINTER(ctypes.c_int32)]
         L.tw_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
@@ -84,14 +97,39 @@ class World:
             self._h = None
 
 
-class Batch:
-    """Worlds packed into one device batch and run by the persistent world-queue kernel."""
+def launch_capacity(chain_runs=9, block_threads=0):
+    """Workgroups the GPU holds at once for the slot kernel (a plan's limit)."""
+    n = ctypes.c_int32()
+    _check(lib().tw_launch_capacity(chain_runs, block_threads, ctypes.byref(n)))
+    return n.value
 
-    def __init__(self, worlds, chain_runs=9, block_threads=0, blocks=0, team_blocks=0, hosts_per_wave=0, slot_budget=0):
+
+def _i32(xs):
+    return (ctypes.c_int32 * len(xs))(*xs)
+
+
+class Batch:
+    """Worlds packed into one device batch and run by the persistent world-queue kernel.
+
+    plan: None (one queue, teams of `team_blocks`, automatic when 0) or a dict with "team_blocks"
+    (workgroups per team), "team_queue", "world_queue" and optionally "world_order" (see tw_plan in
+    include/twins.h); the batched sweep's autotuner (twins_amd.autotune) makes them."""
+
+    def __init__(self, worlds, chain_runs=9, block_threads=0, blocks=0, team_blocks=0, hosts_per_wave=0, slot_budget=0,
+                 plan=None, graph=True):
         L = lib()
         self.worlds = list(worlds)
         hs = (ctypes.c_void_p * len(self.worlds))(*[w._h for w in self.worlds])
-        cfg = PackCfg(chain_runs, block_threads, blocks, team_blocks, hosts_per_wave, 0, slot_budget)
+        p = None
+        if plan is not None:
+            keep = [_i32(plan["team_blocks"]), _i32(plan["team_queue"]), _i32(plan["world_queue"])]
+            order = plan.get("world_order")
+            keep.append(_i32(order) if order is not None else None)
+            nq = max(max(plan["team_queue"]), max(plan["world_queue"])) + 1
+            p = Plan(len(plan["team_blocks"]), nq, keep[0], keep[1], keep[2],
+                     keep[3] if keep[3] is not None else ctypes.POINTER(ctypes.c_int32)())
+        cfg = PackCfg(chain_runs, block_threads, blocks, team_blocks, hosts_per_wave, 0 if graph else 1, slot_budget,
+                      ctypes.pointer(p) if p is not None else ctypes.POINTER(Plan)())
         self._h = ctypes.c_void_p()
         _check(L.tw_batch_create(hs, len(self.worlds), ctypes.byref(cfg), ctypes.byref(self._h)))
 
@@ -105,6 +143,11 @@ class Batch:
         est, done = (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)()
         _check(lib().tw_fct_get(self._h, w, est, done))
         return list(est), list(done)
+
+    def world_stats(self, w):
+        st = WorldStats()
+        _check(lib().tw_world_stats_get(self._h, w, ctypes.byref(st)))
+        return st
 
     def counters(self, w):
         ev = (ctypes.c_uint64 * 32)()

@@ -9,11 +9,13 @@
 // write their own links' message for this slot (reference: src/kernel.cuh:5-113). The slot ends
 // with the team barrier, whose last arrival is the ticket (finish test, exact idle-slot skip).
 //
-// T = 1 (the batched sweep: a world per workgroup) needs only the workgroup barrier, so a
-// world's state stays within one CU. T > 1 (one large world over many CUs) synchronises through
-// an agent-scope arrive counter and generation word per team; the launch is cooperative, so all
-// of a team's workgroups are resident, and every wait is bounded (a timeout ends the world with
-// an error flag instead of hanging the queue).
+// The launch plan gives each team its size T, its workgroups and the queue of worlds it takes
+// from, so one launch can run small worlds a workgroup each and large ones over many CUs.
+// T = 1 needs only the workgroup barrier, so a world's state stays within one CU. T > 1
+// synchronises through an agent-scope arrive counter and generation word per team; the grid is
+// at most what the GPU holds at once, so all of a team's workgroups are resident, and every wait
+// is bounded (a timeout ends the world with an error flag instead of hanging the queue). Each world's team
+// accounts the device clock time it spent on it (WorldRun::ticks), the input of the autotuner.
 #include "tcp_model.h"
 #include "tcp_kernel.h"
 
@@ -21,12 +23,12 @@ namespace tw
 {
 
 constexpr int LDS_PER_WAVE = 64 * 33 + 64 * 3 + 64 + 64; // switch phase: rows, in_win, base, cnt
-constexpr long SPIN_LIMIT = 1L << 25;                    // team-barrier polls before giving up
+constexpr unsigned long long WAIT_TICKS = 2000000000ull;  // a team-barrier wait gives up after 20 s (100 MHz clock)
 
 __device__ __forceinline__ unsigned ld_acquire(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
 
 template <int RUNS>
-__global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__restrict__ Bp, int T, int hpw, int64_t slot_budget)
+__global__ void __launch_bounds__(512) tcp_worlds_kernel(const BatchT<RUNS> *__restrict__ Bp, int64_t slot_budget)
 {
     typedef Model<RUNS> Md;
     const BatchT<RUNS> &B = *Bp;
@@ -37,7 +39,9 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
     __shared__ unsigned s_act;
     __shared__ unsigned long long s_wake;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwaves = blockDim.x >> 6;
-    const int team = blockIdx.x / T, rank = blockIdx.x % T;
+    const int team = B.team_of[blockIdx.x];
+    const TeamPlan P = B.plan[team];
+    const int T = P.T, rank = blockIdx.x - P.first, hpw = P.hpw;
     const int TW = T * nwaves, gw = rank * nwaves + wv; // the team's waves; this wave's place
     TeamState *ts = B.teams + team;
     int *wl = lds + wv * LDS_PER_WAVE;
@@ -67,10 +71,11 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
                 }
                 else
                 {
-                    for (long spins = 0; ld_acquire(&ts->gen) == g; spins++)
+                    const unsigned long long w0 = wall_clock64();
+                    while (ld_acquire(&ts->gen) == g)
                     {
                         __builtin_amdgcn_s_sleep(1);
-                        if (spins > SPIN_LIMIT)
+                        if (wall_clock64() - w0 > WAIT_TICKS)
                         {
                             atomicExch(&ts->err, 1);
                             break;
@@ -88,20 +93,21 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
 
     for (;;)
     {
-        // the team's next world
-        if (team_barrier([&] { ts->world = atomicAdd(B.queue_next, 1); }))
+        // the team's next world, from its queue
+        if (team_barrier([&] { ts->world = atomicAdd(B.queue_next + P.queue, 1); }))
             return;
         if (tid == 0)
             s_w = __hip_atomic_load(&ts->world, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const int qi = s_w;
-        if (qi >= B.nworlds)
+        if (qi >= B.wq_len[P.queue])
             return;
-        const int w = B.world_order[qi];
+        const int w = B.world_order[B.wq_off[P.queue] + qi];
         const WorldDesc &W = B.worlds[w];
         WorldRun *R = B.run + w;
         const int k = W.k, spw = 64 / k;
         const int host_items = (W.H + hpw - 1) / hpw, items = host_items + (W.S + spw - 1) / spw;
+        const unsigned long long c_begin = wall_clock64();
         for (int64_t it = 0;; it++)
         {
             if (tid == 0)
@@ -113,7 +119,15 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
             }
             __syncthreads();
             if (s_done)
+            {
+                // every workgroup of the team leaves at the same slot; the first accounts the time
+                if (rank == 0 && tid == 0)
+                {
+                    R->ticks += wall_clock64() - c_begin;
+                    R->team = T;
+                }
                 break;
+            }
             const int64_t t = s_t;
             unsigned act = 0;
             unsigned long long wk = ~0ull;
@@ -234,18 +248,35 @@ __global__ void __launch_bounds__(1024) tcp_worlds_kernel(const BatchT<RUNS> *__
 template <int RUNS>
 static hipError_t launch(const void *batch_dev, const LaunchShape &s, hipStream_t stream)
 {
+    // an ordinary launch: the grid never exceeds the workgroups the GPU holds at once (the plan
+    // checks it), so a team's workgroups are resident together, and a team barrier that still
+    // waited too long ends the world with an error. No cooperative launch: it takes a device-wide
+    // resource that one process can hold while another waits for it.
     const size_t lds = (size_t)(s.threads / 64) * LDS_PER_WAVE * sizeof(int);
     const BatchT<RUNS> *b = reinterpret_cast<const BatchT<RUNS> *>(batch_dev);
-    int T = s.team, hpw = s.hpw;
-    int64_t budget = s.slot_budget;
-    if (s.team > 1)
+    hipLaunchKernelGGL(tcp_worlds_kernel<RUNS>, dim3(s.blocks), dim3(s.threads), lds, stream, b, s.slot_budget);
+    return hipGetLastError();
+}
+
+// clears the world-queue counters and the teams' barrier words before a launch (a kernel, not a
+// memset, so that the pair can be captured in a graph)
+__global__ void tcp_reset_kernel(uint32_t *__restrict__ a, int na, uint32_t *__restrict__ b, int nb)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += gridDim.x * blockDim.x)
     {
-        // every workgroup of a team must be resident: a cooperative launch refuses otherwise
-        void *args[] = {&b, &T, &hpw, &budget};
-        return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&tcp_worlds_kernel<RUNS>), dim3(s.blocks), dim3(s.threads),
-                                          args, (unsigned)lds, stream);
+        if (i < na)
+            a[i] = 0;
+        else
+            b[i - na] = 0;
     }
-    hipLaunchKernelGGL(tcp_worlds_kernel<RUNS>, dim3(s.blocks), dim3(s.threads), lds, stream, b, T, hpw, budget);
+}
+
+hipError_t tcp_reset(uint32_t *queue_next, int nq, void *teams, size_t team_bytes, hipStream_t stream)
+{
+    const int nb = (int)(team_bytes / 4);
+    int blocks = (nq + nb + 255) / 256;
+    blocks = blocks < 1 ? 1 : blocks > 1024 ? 1024 : blocks;
+    hipLaunchKernelGGL(tcp_reset_kernel, dim3(blocks), dim3(256), 0, stream, queue_next, nq, (uint32_t *)teams, nb);
     return hipGetLastError();
 }
 

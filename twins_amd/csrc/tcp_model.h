@@ -25,77 +25,78 @@ struct Model
 {
     typedef BatchT<RUNS> Batch;
     typedef FrameT<RUNS> Frame;
-    static constexpr int CH = 2 * RUNS + 2;
+    static constexpr int CW = RUNS + 1; // a chain's words: RUNS runs (code | count << 16), the root
     struct Chain
     {
-        uint16_t w[CH];
+        uint32_t r[CW];
     };
 
     // -------------------------------------------------------------- event-order chains
-    // (src/step.cuh:58-159) run-length (code, count) pairs of the leads back to the root
-    static __device__ __forceinline__ uint16_t code_lead(int64_t lead_ps)
+    // (src/step.cuh:58-159) the leads from an event back to its root, run-length encoded; codes
+    // grow with the lead: ScheduleNow 1, Send's deferral 2, TransmitComplete len, arrival
+    // 2048 + len, timers 4096 + lead/16 ns (saturating)
+    static __device__ __forceinline__ uint32_t code_lead(int64_t lead_ps)
     {
         int64_t c = 4096 + lead_ps / 16000;
-        return (uint16_t)(c < 0xFFFE ? c : 0xFFFE);
+        return (uint32_t)(c < 0xFFFE ? c : 0xFFFE);
     }
-    static __device__ __forceinline__ void chain_load(Chain &c, const uint16_t *p)
+    static __device__ __forceinline__ void chain_load(Chain &c, const uint32_t *p)
     {
 #pragma unroll
-        for (int i = 0; i < CH; i++)
-            c.w[i] = p[i];
+        for (int i = 0; i < CW; i++)
+            c.r[i] = p[i];
     }
-    static __device__ __forceinline__ void chain_store(uint16_t *p, const Chain &c)
+    static __device__ __forceinline__ void chain_store(uint32_t *p, const Chain &c)
     {
 #pragma unroll
-        for (int i = 0; i < CH; i++)
-            p[i] = c.w[i];
+        for (int i = 0; i < CW; i++)
+            p[i] = c.r[i];
     }
-    static __device__ void chain_root(Chain &c, int host_local, int conn_local)
+    // an application start scheduled at set-up: host then connection (world-local)
+    static __device__ __forceinline__ void chain_root(Chain &c, int host_local, int conn_local)
     {
 #pragma unroll
-        for (int i = 0; i < 2 * RUNS; i++)
-            c.w[i] = 0;
-        c.w[2 * RUNS] = (uint16_t)(0x8000 | (host_local + 1));
-        c.w[2 * RUNS + 1] = (uint16_t)conn_local;
+        for (int i = 0; i < RUNS; i++)
+            c.r[i] = 0;
+        c.r[RUNS] = (uint32_t)(0x8000 | (host_local + 1)) | (uint32_t)(conn_local & 0xFFFF) << 16;
     }
-    static __device__ void chain_root_msg(Chain &c, int msg_local)
+    // a message write scheduled in main(), by message (world-local)
+    static __device__ __forceinline__ void chain_root_msg(Chain &c, int msg_local)
     {
 #pragma unroll
-        for (int i = 0; i < 2 * RUNS; i++)
-            c.w[i] = 0;
-        c.w[2 * RUNS] = (uint16_t)(1 + (msg_local >> 16));
-        c.w[2 * RUNS + 1] = (uint16_t)msg_local;
+        for (int i = 0; i < RUNS; i++)
+            c.r[i] = 0;
+        c.r[RUNS] = (uint32_t)((1 + (msg_local >> 16)) & 0xFFFF) | (uint32_t)(msg_local & 0xFFFF) << 16;
     }
-    // dst = [code] + parent (dst may alias parent)
-    static __device__ void chain_child(Chain &dst, uint16_t code, const Chain &parent)
+    // dst = [code] + parent (dst may alias parent): a repeated code extends the first run
+    static __device__ __forceinline__ void chain_child(Chain &dst, uint32_t code, const Chain &parent)
     {
-        Chain p = parent;
-        if (p.w[0] == code)
+        const Chain p = parent;
+        if ((p.r[0] & 0xFFFF) == code)
         {
             dst = p;
-            if (dst.w[1] < 0xFFFF)
-                dst.w[1]++;
+            if ((p.r[0] >> 16) < 0xFFFF)
+                dst.r[0] += 1u << 16;
             return;
         }
-        dst.w[0] = code;
-        dst.w[1] = 1;
+        dst.r[0] = code | 1u << 16;
 #pragma unroll
-        for (int i = 2; i < 2 * RUNS; i++)
-            dst.w[i] = p.w[i - 2];
-        dst.w[2 * RUNS] = p.w[2 * RUNS];
-        dst.w[2 * RUNS + 1] = p.w[2 * RUNS + 1];
+        for (int i = 1; i < RUNS; i++)
+            dst.r[i] = p.r[i - 1];
+        dst.r[RUNS] = p.r[RUNS];
     }
-    // < 0: a runs first, > 0: b, 0: undecided
-    static __device__ int chain_cmp(const uint16_t *a, const uint16_t *b)
+    // < 0: a runs first, > 0: b, 0: undecided. Larger codes (earlier parents) first; a complete
+    // chain (fewer than RUNS runs) ends at its root, before any other parent; else the roots.
+    static __device__ int chain_cmp(const uint32_t *a, const uint32_t *b)
     {
         int ia = 0, ib = 0;
-        uint32_t ra = a[1], rb = b[1];
+        uint32_t ra = a[0] >> 16, rb = b[0] >> 16;
         for (;;)
         {
-            uint16_t ca = a[2 * ia], cb = b[2 * ib];
-            if (ca == 0 && cb != 0 && a[2 * RUNS] != 0)
+            uint32_t ca = a[ia] & 0xFFFF, cb = b[ib] & 0xFFFF;
+            if (ca == 0 && cb != 0 && (a[RUNS] & 0xFFFF) != 0)
                 return -1;
-            if (cb == 0 && ca != 0 && b[2 * RUNS] != 0)
+            if (cb == 0 && ca != 0 && (b[RUNS] & 0xFFFF) != 0)
                 return 1;
             if (ca == 0 || cb == 0)
                 break;
@@ -108,16 +109,16 @@ struct Model
             {
                 if (++ia == RUNS)
                     break;
-                ra = a[2 * ia + 1];
+                ra = a[ia] >> 16;
             }
             if (rb == 0)
             {
                 if (++ib == RUNS)
                     break;
-                rb = b[2 * ib + 1];
+                rb = b[ib] >> 16;
             }
         }
-        uint32_t xa = (uint32_t)a[2 * RUNS] << 16 | a[2 * RUNS + 1], xb = (uint32_t)b[2 * RUNS] << 16 | b[2 * RUNS + 1];
+        uint32_t xa = a[RUNS] << 16 | a[RUNS] >> 16, xb = b[RUNS] << 16 | b[RUNS] >> 16; // the root keys
         if (xa == 0 || xb == 0 || xa == xb)
             return 0;
         return xa < xb ? -1 : 1;
@@ -134,10 +135,10 @@ struct Model
         Frame *ring;
         int cap, head, size, red_k, limit, sent;
         int64_t link_free, last_start, woke, idle_start, slot_end;
-        uint16_t *last_chain;
+        Chain last; // the chain of the transmission in progress (kept in registers for the slot)
         Frame *out;
 
-        __device__ void load(const Batch &b, WorldRun *r, int link, int64_t t)
+        __device__ __forceinline__ void load(const Batch &b, WorldRun *r, int link, int64_t t, int ring_t)
         {
             B = &b;
             R = r;
@@ -150,20 +151,21 @@ struct Model
             limit = b.q_limit[link];
             link_free = b.q_link_free[link];
             last_start = b.q_last_start[link];
-            last_chain = b.q_last_chain + (size_t)link * CH;
+            chain_load(last, b.q_last_chain + (size_t)link * CW);
             woke = idle_start = -1;
             slot_end = (t + 1) * SLOT_PS;
             sent = 0;
-            out = b.msg + ((size_t)(t % NRING) * b.nlinks + link) * MAX_TX;
+            out = b.msg + ((size_t)ring_t * b.nlinks + link) * MAX_TX;
         }
         // the slot's message and the queue state back to memory
-        __device__ void store(int64_t t)
+        __device__ __forceinline__ void store(int64_t t, int ring_t)
         {
-            B->msg_cnt[(size_t)(t % NRING) * B->nlinks + lid] = (long long)t << 16 | sent;
+            B->msg_cnt[(size_t)ring_t * B->nlinks + lid] = (long long)t << 16 | sent;
             B->q_head[lid] = head;
             B->q_size[lid] = size;
             B->q_link_free[lid] = link_free;
             B->q_last_start[lid] = last_start;
+            chain_store(B->q_last_chain + (size_t)lid * CW, last);
             if (sent)
                 count(R, EV_PKT_EVENTS, (unsigned long long)sent);
         }
@@ -182,16 +184,14 @@ struct Model
                 // else its own enqueue (a link that never sent is idle)
                 if (f.ts <= link_free && link_free > last_start)
                 {
-                    Chain prev, c;
-                    chain_load(prev, last_chain);
-                    chain_child(c, (uint16_t)((link_free - last_start) / PS_PER_BYTE), prev);
-                    chain_store(o.chain, c);
+                    chain_child(last, (uint32_t)((link_free - last_start) / PS_PER_BYTE), last);
+                    chain_store(o.chain, last);
                 }
                 else
+                {
                     idle_start = start;
-#pragma unroll
-                for (int i = 0; i < CH; i++)
-                    last_chain[i] = o.chain[i];
+                    chain_load(last, o.chain);
+                }
                 last_start = start;
                 o.ts = end + PROP_PS;
                 out[sent++] = o;
@@ -247,7 +247,7 @@ struct Model
         const Batch *B;
         const WorldDesc *W;
         WorldRun *R;
-        Fifo *nic;
+        Fifo nic;
         int h, hl; // global, world-local
         bool dctcp, cubic;
         Chain echain; // the chain of the event being processed
@@ -274,11 +274,11 @@ struct Model
         x.due[j] = {at, f, kind};
         x.nd++;
     }
-    static __device__ void timer_chain(Host &x, uint16_t *base, int f, int64_t lead)
+    static __device__ __forceinline__ void timer_chain(Host &x, uint32_t *base, int f, int64_t lead)
     {
         Chain c;
         chain_child(c, code_lead(lead), x.echain);
-        chain_store(base + (size_t)f * CH, c);
+        chain_store(base + (size_t)f * CW, c);
     }
     static __device__ void emit(Host &x, int f, uint8_t flags, uint32_t seq, uint32_t ack, uint16_t len, bool to_sender, int64_t now)
     {
@@ -294,7 +294,7 @@ struct Model
         fr.up = (uint8_t)((to_sender ? cs.hash_ack : cs.hash_data) % (uint32_t)x.W->ecmp_n);
         fr.pad = 0;
         chain_store(fr.chain, x.echain);
-        x.nic->enqueue(fr, now);
+        x.nic.enqueue(fr, now);
     }
     static __device__ void send_ack(Host &x, int f, uint32_t ack, bool force_ece, bool no_ece, int64_t now)
     {
@@ -713,7 +713,7 @@ struct Model
         s.cub_currrtt = INT64_MIN;
         s.cub_samples = 0;
     }
-    static __device__ __noinline__ uint32_t cub_update(SndState &s, uint32_t segs, int64_t now)
+    static __device__ __forceinline__ uint32_t cub_update(SndState &s, uint32_t segs, int64_t now)
     {
         const double C = 0.4, BETA = 0.7;
         uint32_t seg_cwnd = s.cwnd / MSS;
@@ -1025,7 +1025,7 @@ struct Model
         if (ack > s.nxt)
             s.nxt = ack;
     }
-    static __device__ __noinline__ void enter_recovery(Host &x, int f, int64_t now)
+    static __device__ __forceinline__ void enter_recovery(Host &x, int f, int64_t now)
     {
         SndState &s = x.B->snd[f];
         Tx t(*x.B, f);
@@ -1208,7 +1208,7 @@ struct Model
         }
         send_pending(x, f, now);
     }
-    static __device__ __noinline__ void on_rto(Host &x, int f, int64_t now)
+    static __device__ __forceinline__ void on_rto(Host &x, int f, int64_t now)
     {
         SndState &s = x.B->snd[f];
         s.rto_at = -1;
@@ -1549,7 +1549,7 @@ struct Model
             s.syn_at = now + 1;
             Chain ch;
             chain_child(ch, 2, x.echain);
-            chain_store(B.schain + (size_t)f * CH, ch);
+            chain_store(B.schain + (size_t)f * CW, ch);
             due_insert(x, now + 1, f, 2);
         }
     }
@@ -1608,7 +1608,7 @@ struct Model
             }
             if (r.r_est)
                 return;
-            x.nic->woke = now;
+            x.nic.woke = now;
             chain_child(x.echain, 1, x.echain);
             send_synack(x, f, now);
         }
@@ -1622,21 +1622,21 @@ struct Model
                 rcv_closed(r);
         }
     }
-    static __device__ const uint16_t *due_chain(const Batch &B, const Due &d)
+    static __device__ __forceinline__ const uint32_t *due_chain(const Batch &B, const Due &d)
     {
-        const uint16_t *base = d.kind == 1 || d.kind == 3 || d.kind == 4 ? B.dchain
+        const uint32_t *base = d.kind == 1 || d.kind == 3 || d.kind == 4 ? B.dchain
                                : d.kind == 5                            ? B.fchain
                                : (d.kind == 2 && B.snd[d.flow].st == S_EST) ? B.schain
                                                                             : B.rchain;
-        return base + (size_t)d.flow * CH;
+        return base + (size_t)d.flow * CW;
     }
     // a timer and an arrival at one instant: the one scheduled first runs first (undecided: the timer)
-    static __device__ __noinline__ bool timer_before_arrival(Host &x, const Due &d, const Frame &p)
+    static __device__ __forceinline__ bool timer_before_arrival(Host &x, const Due &d, const Frame &p)
     {
         Chain arr, pc;
         chain_load(pc, p.chain);
-        chain_child(arr, (uint16_t)(2048 + p.len), pc);
-        int r = chain_cmp(due_chain(*x.B, d), arr.w);
+        chain_child(arr, 2048u + p.len, pc);
+        int r = chain_cmp(due_chain(*x.B, d), arr.r);
         if (r == 0)
             count(x.R, EV_UNDECIDED);
         return r <= 0;
@@ -1673,13 +1673,12 @@ struct Model
             *wake = B.wake[h];
             return 0;
         }
-        Fifo nic;
-        nic.load(B, R, lid, t);
+        const int rt = (int)(t % NRING);
         Host x;
+        x.nic.load(B, R, lid, t, rt);
         x.B = &B;
         x.W = &W;
         x.R = R;
-        x.nic = &nic;
         x.h = h;
         x.hl = hl;
         x.dctcp = B.host_cc[h] == CC_DCTCP;
@@ -1749,7 +1748,7 @@ struct Model
                 break;
             if (now < S0)
                 now = S0;
-            nic.advance(now);
+            x.nic.advance(now);
             if (now == tp && B.cs[B.mg_conn[pm]].m_first != pm)
             {
                 // a later message of a connection (FlowSender::SendMessage)
@@ -1845,12 +1844,12 @@ struct Model
                 Frame p = i2 < n2 ? m2[i2++] : m1[i1++];
                 Chain pc;
                 chain_load(pc, p.chain);
-                chain_child(x.echain, (uint16_t)(2048 + p.len), pc);
+                chain_child(x.echain, 2048u + p.len, pc);
                 on_frame(x, p, now);
             }
         }
-        nic.advance(E0 - 1);
-        nic.store(t);
+        x.nic.advance(E0 - 1);
+        x.nic.store(t, rt);
         B.pend_next[h] = pn;
         // drop finished connections from the active lists; the next wake-up
         int64_t next = INT64_MAX;
@@ -1891,10 +1890,10 @@ struct Model
         }
         if (pn < pe && B.mg_start[B.pend_msg[pn]] < next)
             next = B.mg_start[B.pend_msg[pn]];
-        int64_t wk = nic.size > 0 ? t + 1 : (next == INT64_MAX ? INT64_MAX : next / SLOT_PS);
+        int64_t wk = x.nic.size > 0 ? t + 1 : (next == INT64_MAX ? INT64_MAX : next / SLOT_PS);
         B.wake[h] = wk;
         *wake = wk;
-        return (nic.sent > 0 || nic.size > 0) ? 1 : 0;
+        return (x.nic.sent > 0 || x.nic.size > 0) ? 1 : 0;
     }
 
     // -------------------------------------------------------------- routing (src/step.cuh:2541-2630)
@@ -2152,18 +2151,19 @@ struct Model
             ref[j + 1] = r;
         }
         Fifo eg;
-        eg.load(B, R, x.lid, t);
+        const int rt = (int)(t % NRING);
+        eg.load(B, R, x.lid, t, rt);
         for (int i = 0; i < n; i++)
         {
             Frame p = frame_at(ref[i]);
             Chain up, c;
             chain_load(up, p.chain);
-            chain_child(c, (uint16_t)(2048 + p.len), up);
+            chain_child(c, 2048u + p.len, up);
             chain_store(p.chain, c);
             eg.enqueue(p, p.ts);
         }
         eg.advance(E0 - 1);
-        eg.store(t);
+        eg.store(t, rt);
         return (eg.sent > 0 || eg.size > 0) ? 1 : 0;
     }
     // one slot of a switch on the GPU: the lanes of one wavefront, phases separated by wave_sync()
@@ -2189,7 +2189,7 @@ struct Model
     }
     // per-packet ECMP: one GetInteger draw per forwarded packet, in the node's arrival order
     // (all ports merged; src/step.cuh:2706-2745)
-    static __device__ __noinline__ void spray_draws(const Batch &B, const WorldDesc &W, WorldRun *R, int swl, int64_t t, const SwLds &L,
+    static __device__ __forceinline__ void spray_draws(const Batch &B, const WorldDesc &W, WorldRun *R, int swl, int64_t t, const SwLds &L,
                                                    uint8_t *scr_up)
     {
         const int k = W.k;

@@ -65,8 +65,9 @@ enum : int
     EV_N = 32
 };
 
-// A frame: 24 bytes of header then the event-order chain (RUNS (code, count) pairs + a 2-word
-// root): 64 bytes with 9 runs, 128 with 25.
+// A frame: 24 bytes of header then the event-order chain: RUNS runs, each a 16-bit code in the
+// low half and a 16-bit count in the high half of a word, then the root key's two 16-bit words
+// (high word first in memory): 64 bytes with 9 runs, 128 with 25.
 template <int RUNS>
 struct alignas(16) FrameT
 {
@@ -79,7 +80,7 @@ struct alignas(16) FrameT
     uint8_t to_sender;
     uint8_t up;       // FatTree / leaf-spine: the ECMP index chosen at the sender
     uint8_t pad;
-    uint16_t chain[2 * RUNS + 2];
+    uint32_t chain[RUNS + 1];
 };
 static_assert(sizeof(FrameT<9>) == 64 && sizeof(FrameT<25>) == 128, "frame layout");
 
@@ -156,8 +157,16 @@ struct WorldRun
     uint32_t flows_done;
     int32_t finished;
     int32_t error;
-    int32_t pad;
+    int32_t team;               // workgroups of the team that ran it last
+    unsigned long long ticks;   // device clock ticks (100 MHz) its team spent on it, over all launches
     unsigned long long ev[EV_N];
+};
+
+// a team of the launch plan: its workgroups blockIdx first .. first + T - 1, the hosts per wave work
+// item it uses, and the queue of worlds it takes from
+struct TeamPlan
+{
+    int32_t first, T, hpw, queue;
 };
 
 // a team's barrier and reductions (teams of several workgroups run one world), each word on its
@@ -180,7 +189,7 @@ template <int RUNS>
 struct BatchT
 {
     typedef FrameT<RUNS> Frame;
-    static constexpr int CH = 2 * RUNS + 2;
+    static constexpr int CW = RUNS + 1; // words of a chain
     int32_t nworlds, nhosts, nswitches, nlinks, nconns, nmsgs;
     WorldDesc *worlds;
     WorldRun *run;
@@ -188,7 +197,7 @@ struct BatchT
     ConnStatic *cs;
     SndState *snd;
     RcvState *rcv;
-    uint16_t *rchain, *schain, *dchain, *fchain; // [conn][CH] timer chains
+    uint32_t *rchain, *schain, *dchain, *fchain; // [conn][CW] timer chains
     uint32_t *it_start, *h_seq, *h_cnt, *oo_s, *oo_e;
     uint8_t *it_fl;
     int64_t *h_t;
@@ -210,7 +219,7 @@ struct BatchT
     uint64_t *q_base;
     int32_t *q_cap, *q_head, *q_size, *q_red_k, *q_limit;
     int64_t *q_link_free, *q_last_start;
-    uint16_t *q_last_chain; // [link][CH]
+    uint32_t *q_last_chain; // [link][CW]
     // link messages [NRING][link][MAX_TX] and their tagged counts slot << 16 | n
     Frame *msg;
     long long *msg_cnt;
@@ -220,9 +229,15 @@ struct BatchT
     uint8_t *g_rt_n, *g_rt_ports, *g_port_dn;
     int64_t *g_fail_t;
     TwMrgState *sw_rng; // per-packet ECMP streams
-    // kernel work queue: worlds in the order teams take them; the teams' barriers
+    // kernel work queues: queue q holds worlds world_order[wq_off[q] .. wq_off[q] + wq_len[q] - 1] in
+    // the order its teams take them, queue_next[q] counts those taken; the plan's teams (team_of:
+    // workgroup -> team) and their barriers
     int32_t *world_order;
+    int32_t *wq_off, *wq_len;
     int32_t *queue_next;
+    int32_t nwq;
+    int32_t *team_of;
+    TeamPlan *plan;
     TeamState *teams;
     // switch-phase scratch, per wave: [wave][64 * 2 * MAX_TX] arrival times, refs, drawn routes
     int64_t *scr_ts;

@@ -591,7 +591,7 @@ extern "C" int tw_world_destroy(tw_world *w)
 
 static uint64_t world_bytes(const tw_world &w, int runs)
 {
-    const uint64_t F = runs == 25 ? 128 : 64, CHB = (2 * runs + 2) * 2;
+    const uint64_t F = runs == 25 ? 128 : 64, CHB = (runs + 1) * 4;
     uint64_t b = 0;
     b += (uint64_t)NRING * w.L * MAX_TX * F + (uint64_t)NRING * w.L * 8;          // link messages
     b += (uint64_t)w.L * (std::max(w.queue, 1) + 2) * F + (uint64_t)w.L * (40 + CHB); // queues
@@ -844,13 +844,21 @@ struct tw_batch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     LaunchShape shape{};
     TeamState *teams = nullptr;
-    int nteams = 0;
+    int nteams = 0, nwq = 0;
+    std::vector<TeamPlan> plan;
     int32_t *queue_next = nullptr;
+    bool no_graph = false, graph_failed = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     WorldRun *run = nullptr;
     std::vector<WorldRun> run_host;
     int64_t *mg_est = nullptr, *mg_done = nullptr;
     ~tw_batch()
     {
+        if (gexec)
+            (void)hipGraphExecDestroy(gexec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
         for (void *p : bufs)
             (void)hipFree(p);
         if (ev0)
@@ -906,7 +914,7 @@ template <int RUNS>
 static int pack(tw_batch &b, const tw_pack_cfg &cfg)
 {
     typedef BatchT<RUNS> Batch;
-    constexpr int CH = Batch::CH;
+    constexpr int CW = Batch::CW;
     Batch B;
     memset(&B, 0, sizeof B);
     const int W = (int)b.worlds.size();
@@ -1037,10 +1045,10 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     UP(cs, cs);
     UP(snd, snd);
     UP(rcv, rcv);
-    ZERO(rchain, NC * CH);
-    ZERO(schain, NC * CH);
-    ZERO(dchain, NC * CH);
-    ZERO(fchain, NC * CH);
+    ZERO(rchain, NC * CW);
+    ZERO(schain, NC * CW);
+    ZERO(dchain, NC * CW);
+    ZERO(fchain, NC * CW);
     ZERO(it_start, IT);
     ZERO(h_seq, IT);
     ZERO(h_cnt, IT);
@@ -1075,7 +1083,7 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     UP(q_limit, q_limit);
     ZERO(q_link_free, L);
     ZERO(q_last_start, L);
-    ZERO(q_last_chain, L * CH);
+    ZERO(q_last_chain, L * CW);
     if ((rc = dev_alloc(b, &B.msg, (size_t)NRING * L * MAX_TX)))
         return rc;
     {
@@ -1090,53 +1098,127 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     UP(g_port_dn, g_pdn);
     UP(g_fail_t, g_fail);
     UP(sw_rng, rng);
-    // the world queue: largest worlds first, so that the long ones start early
-    std::vector<int32_t> order(W);
-    for (int i = 0; i < W; i++)
-        order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        return (int64_t)b.worlds[x]->H + b.worlds[x]->S > (int64_t)b.worlds[y]->H + b.worlds[y]->S;
-    });
-    UP(world_order, order);
-    ZERO(queue_next, 1);
-    // launch shape: teams of T workgroups (one world each), `hpw` hosts per wave work item
+    // launch plan: teams of workgroups, each taking worlds from one queue
     int dev = 0, cus = 256, per_cu = 1;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     LaunchShape &sh = b.shape;
     sh.threads = cfg.block_threads > 0 ? cfg.block_threads : 256;
-    if (sh.threads % 64 || sh.threads > 1024)
-        return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 1024");
+    if (sh.threads % 64 || sh.threads > 512)
+        return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 512");
     HIPCHK(tcp_max_blocks(RUNS, sh.threads, &per_cu));
     const int resident = std::max(1, per_cu) * cus;
-    int64_t largest = 0;
-    for (auto &w : b.worlds)
-        largest = std::max<int64_t>(largest, (int64_t)w->H + w->S);
-    if (cfg.team_blocks > 0)
-        sh.team = cfg.team_blocks;
-    else // spread few large worlds over the GPU; many worlds run a workgroup each
-        sh.team = W >= cus ? 1 : std::max(1, std::min<int>(resident / W, (int)((largest + 63) / 64)));
-    int teams = cfg.blocks > 0 ? cfg.blocks / sh.team : std::min(W, resident / sh.team);
-    if (teams < 1 || (sh.team > 1 && teams * sh.team > resident))
-        return fail(TW_EINVAL, "%d teams of %d workgroups do not fit (%d resident workgroups)", teams, sh.team, resident);
-    sh.blocks = teams * sh.team;
-    if (cfg.hosts_per_wave > 0)
-        sh.hpw = std::min(64, cfg.hosts_per_wave);
+    auto size_of = [&](int i) { return (int64_t)b.worlds[i]->H + b.worlds[i]->S; };
+    std::vector<int32_t> order(W), team_T, team_q, world_q(W, 0);
+    int nq = 1;
+    for (int i = 0; i < W; i++)
+        order[i] = i;
+    if (cfg.plan)
+    {
+        const tw_plan &p = *cfg.plan;
+        if (p.nteams < 1 || p.nqueues < 1 || !p.team_blocks || !p.team_queue || !p.world_queue)
+            return fail(TW_EINVAL, "plan: teams, queues and the world queues are required");
+        nq = p.nqueues;
+        team_T.assign(p.team_blocks, p.team_blocks + p.nteams);
+        team_q.assign(p.team_queue, p.team_queue + p.nteams);
+        world_q.assign(p.world_queue, p.world_queue + W);
+        for (int j = 0; j < p.nteams; j++)
+            if (team_T[j] < 1 || team_q[j] < 0 || team_q[j] >= nq)
+                return fail(TW_EINVAL, "plan: team %d has %d workgroups, queue %d", j, team_T[j], team_q[j]);
+        for (int i = 0; i < W; i++)
+            if (world_q[i] < 0 || world_q[i] >= nq)
+                return fail(TW_EINVAL, "plan: world %d in queue %d of %d", i, world_q[i], nq);
+        if (p.world_order)
+        {
+            std::vector<char> seen(W, 0);
+            for (int i = 0; i < W; i++)
+            {
+                int x = p.world_order[i];
+                if (x < 0 || x >= W || seen[x])
+                    return fail(TW_EINVAL, "plan: world_order is not a permutation of the worlds");
+                seen[x] = 1;
+                order[i] = x;
+            }
+        }
+    }
     else
     {
-        // about one work item per wave of the team
-        const int64_t tw = (int64_t)sh.team * (sh.threads / 64);
-        sh.hpw = sh.team == 1 ? 64 : (int)std::max<int64_t>(1, std::min<int64_t>(64, (largest + tw - 1) / tw));
+        // one queue, largest worlds first (the long ones start early); few large worlds are
+        // spread over the GPU, many run a workgroup each
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return size_of(x) > size_of(y); });
+        int64_t largest = size_of(order[0]);
+        int T = cfg.team_blocks > 0 ? cfg.team_blocks
+                                    : (W >= cus ? 1 : std::max(1, std::min<int>(resident / W, (int)((largest + 63) / 64))));
+        int teams = cfg.blocks > 0 ? cfg.blocks / T : std::min(W, resident / T);
+        if (teams < 1)
+            return fail(TW_EINVAL, "%d teams of %d workgroups do not fit (%d resident workgroups)", teams, T, resident);
+        team_T.assign(teams, T);
+        team_q.assign(teams, 0);
     }
+    // the queues' worlds, in order
+    std::vector<int32_t> worder, wq_off(nq), wq_len(nq, 0), q_teams(nq, 0);
+    std::vector<int64_t> q_largest(nq, 0);
+    for (int q = 0; q < nq; q++)
+    {
+        wq_off[q] = (int32_t)worder.size();
+        for (int i : order)
+            if (world_q[i] == q)
+            {
+                worder.push_back(i);
+                q_largest[q] = std::max(q_largest[q], size_of(i));
+            }
+        wq_len[q] = (int32_t)worder.size() - wq_off[q];
+    }
+    const int nteams = (int)team_T.size();
+    for (int j = 0; j < nteams; j++)
+        q_teams[team_q[j]]++;
+    for (int q = 0; q < nq; q++)
+        if (wq_len[q] && !q_teams[q])
+            return fail(TW_EINVAL, "plan: queue %d has %d worlds and no team", q, wq_len[q]);
+    std::vector<TeamPlan> plan(nteams);
+    std::vector<int32_t> team_of;
+    int blocks = 0;
+    sh.teams = 0;
+    for (int j = 0; j < nteams; j++)
+    {
+        TeamPlan &tp = plan[j];
+        tp.first = blocks;
+        tp.T = team_T[j];
+        tp.queue = team_q[j];
+        if (cfg.hosts_per_wave > 0)
+            tp.hpw = std::min(64, cfg.hosts_per_wave);
+        else
+        {
+            // about one work item per wave of the team for its queue's largest world
+            const int64_t tw = (int64_t)tp.T * (sh.threads / 64);
+            tp.hpw = tp.T == 1 ? 64 : (int)std::max<int64_t>(1, std::min<int64_t>(64, (q_largest[tp.queue] + tw - 1) / tw));
+        }
+        blocks += tp.T;
+        team_of.insert(team_of.end(), tp.T, j);
+        sh.teams |= tp.T > 1;
+    }
+    if (sh.teams && blocks > resident)
+        return fail(TW_EINVAL, "the plan's %d workgroups exceed the %d the GPU holds at once (a team's workgroups must all be resident)",
+                    blocks, resident);
+    sh.blocks = blocks;
     sh.slot_budget = cfg.slot_budget > 0 ? cfg.slot_budget : 200000;
-    ZERO(teams, teams);
+    UP(world_order, worder);
+    UP(wq_off, wq_off);
+    UP(wq_len, wq_len);
+    ZERO(queue_next, nq);
+    UP(team_of, team_of);
+    UP(plan, plan);
+    B.nwq = nq;
+    ZERO(teams, nteams);
     const size_t waves = (size_t)sh.blocks * (sh.threads / 64);
     B.scr_waves = (int32_t)waves;
     ZERO(scr_ts, waves * 64 * 2 * MAX_TX);
     ZERO(scr_ref, waves * 64 * 2 * MAX_TX);
     ZERO(scr_up, waves * 64 * 2 * MAX_TX);
+    b.plan = plan;
+    b.nwq = nq;
     b.teams = B.teams;
-    b.nteams = teams;
+    b.nteams = nteams;
 #undef UP
 #undef ZERO
     b.queue_next = B.queue_next;
@@ -1168,6 +1250,7 @@ extern "C" int tw_batch_create(tw_world *const *worlds, int n, const tw_pack_cfg
         return fail(TW_EDEVICE, "no GPU: libtwins_core runs the model on the GPU only");
     std::unique_ptr<tw_batch> b(new tw_batch);
     b->runs = runs;
+    b->no_graph = c.no_graph != 0;
     for (int i = 0; i < n; i++)
     {
         if (!worlds[i])
@@ -1184,6 +1267,36 @@ extern "C" int tw_batch_create(tw_world *const *worlds, int n, const tw_pack_cfg
     return TW_OK;
 }
 
+// the launch (queue and team reset, then the persistent kernel) captured once as a HIP graph and
+// replayed for every launch; a failed capture leaves the batch launching directly
+static void capture(tw_batch &b)
+{
+    b.graph_failed = true;
+    if (hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return;
+    }
+    hipError_t e1 = tcp_reset((uint32_t *)b.queue_next, b.nwq, b.teams, sizeof(TeamState) * b.nteams, b.stream);
+    hipError_t e2 = e1 == hipSuccess ? tcp_launch(b.runs, b.batch_dev, b.shape, b.stream) : e1;
+    hipGraph_t g = nullptr;
+    hipError_t e3 = hipStreamEndCapture(b.stream, &g);
+    if (e1 == hipSuccess && e2 == hipSuccess && e3 == hipSuccess && g)
+    {
+        hipGraphExec_t x = nullptr;
+        if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) == hipSuccess && x)
+        {
+            b.graph = g;
+            b.gexec = x;
+            b.graph_failed = false;
+            return;
+        }
+    }
+    if (g)
+        (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+}
+
 extern "C" int tw_batch_run(tw_batch *b, tw_run_stats *stats)
 {
     if (!b)
@@ -1192,6 +1305,9 @@ extern "C" int tw_batch_run(tw_batch *b, tw_run_stats *stats)
     tw_run_stats st;
     memset(&st, 0, sizeof st);
     HIPCHK(hipDeviceSynchronize());
+    std::vector<int64_t> slots_before(W);
+    for (int i = 0; i < W; i++)
+        slots_before[i] = b->run_host[i].slots_run;
     auto t0 = std::chrono::steady_clock::now();
     for (;;)
     {
@@ -1200,10 +1316,16 @@ extern "C" int tw_batch_run(tw_batch *b, tw_run_stats *stats)
             unfinished += !r.finished;
         if (!unfinished)
             break;
-        HIPCHK(hipMemsetAsync(b->queue_next, 0, sizeof(int32_t), b->stream));
-        HIPCHK(hipMemsetAsync(b->teams, 0, sizeof(TeamState) * b->nteams, b->stream));
+        if (!b->gexec && !b->no_graph && !b->graph_failed)
+            capture(*b);
         HIPCHK(hipEventRecord(b->ev0, b->stream));
-        HIPCHK(tcp_launch(b->runs, b->batch_dev, b->shape, b->stream));
+        if (b->gexec)
+            HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+        else
+        {
+            HIPCHK(tcp_reset((uint32_t *)b->queue_next, b->nwq, b->teams, sizeof(TeamState) * b->nteams, b->stream));
+            HIPCHK(tcp_launch(b->runs, b->batch_dev, b->shape, b->stream));
+        }
         HIPCHK(hipEventRecord(b->ev1, b->stream));
         HIPCHK(hipMemcpyAsync(b->run_host.data(), b->run, sizeof(WorldRun) * W, hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipStreamSynchronize(b->stream));
@@ -1211,8 +1333,14 @@ extern "C" int tw_batch_run(tw_batch *b, tw_run_stats *stats)
         HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
         st.kernel_s += ms / 1000.0;
         st.launches++;
-        if (st.launches > 100000)
-            return fail(TW_ESTATE, "no progress after %lld launches", (long long)st.launches);
+        // every launch runs at least one slot of every unfinished world it reaches
+        int64_t ran = 0;
+        for (int i = 0; i < W; i++)
+            ran += b->run_host[i].slots_run - slots_before[i];
+        if (ran == 0)
+            return fail(TW_ESTATE, "a launch ran no slot of the %d unfinished worlds", unfinished);
+        for (int i = 0; i < W; i++)
+            slots_before[i] = b->run_host[i].slots_run;
     }
     st.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (auto &r : b->run_host)
@@ -1225,6 +1353,7 @@ extern "C" int tw_batch_run(tw_batch *b, tw_run_stats *stats)
         st.flows_done += r.flows_done;
         st.worlds_done += r.finished != 0;
     }
+    st.graph = b->gexec != nullptr;
     if (stats)
         *stats = st;
     return TW_OK;
@@ -1249,6 +1378,36 @@ extern "C" int tw_fct_get(tw_batch *b, int world, int64_t *est_ns, int64_t *done
         if (done_ns)
             done_ns[i] = done[j] >= 0 ? done[j] / 1000 : -1;
     }
+    return TW_OK;
+}
+
+extern "C" int tw_world_stats_get(tw_batch *b, int world, tw_world_stats *st)
+{
+    if (!b || !st || world < 0 || world >= (int)b->worlds.size())
+        return fail(TW_EINVAL, "no world %d", world);
+    const WorldRun &r = b->run_host[world];
+    st->slots = r.slots_run;
+    st->device_s = (double)r.ticks / 100e6; // the 100 MHz constant clock
+    st->team_blocks = r.team;
+    st->finished = r.finished;
+    st->packet_events = r.ev[EV_PKT_EVENTS];
+    return TW_OK;
+}
+
+extern "C" int tw_launch_capacity(int chain_runs, int block_threads, int32_t *resident_blocks)
+{
+    if (!resident_blocks)
+        return fail(TW_EINVAL, "null output");
+    const int threads = block_threads > 0 ? block_threads : 256;
+    if (threads % 64 || threads > 512)
+        return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 512");
+    int ndev = 0, dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(TW_EDEVICE, "no GPU: libtwins_core runs the model on the GPU only");
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(tcp_max_blocks(chain_runs == 25 ? 25 : 9, threads, &per_cu));
+    *resident_blocks = std::max(1, per_cu) * cus;
     return TW_OK;
 }
 
