@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X TCP packet-simulation core on the config-2 workload.
+"""Benchmark of the MI355X TCP packet-simulation core: the config-2 workload (the headline line)
+and the config-4 batched sweep (the "sweep" object of the same JSON line).
 
 A step is one complete simulation of the workload: FatTree k=16 (1024 hosts, 320 switches, as a
 topology file), 20,000 Pareto-sized flows (a quarter in persistent connections), per-packet
@@ -14,9 +15,18 @@ frame written and read through a link message and through an egress queue) over 
 time, against MI355X's 8 TB/s. cpu_baseline: the oracle (the sequential CPU restatement of the
 model, one core) on the same three worlds.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Sweep (config 4): 256 worlds drawn by seed 4 (FatTree k in {4,8,16}, leaf-spine 8-32 leaves,
+Jellyfish and torus graphs; ECN K, queue depth and TCP drawn per world; 1k-20k Pareto flows each),
+one persistent launch per step through a HIP graph, with the launch plan (team sizes, queues) made
+by the autotuner from the warm-up runs' per-world device times. Every world's flows are compared
+with the oracle's; the plain per-world launch rate (each world alone, one after another) is
+measured on every 8th world; the CPU baseline is the oracle on every world, on all the host cores
+given, worlds in parallel.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--sweep 0|1]
 """
 import argparse
+import concurrent.futures as cf
 import json
 import os
 import re
@@ -44,6 +54,180 @@ def oracle_run(flows, args, tcp, out):
     return sec, ev, rows
 
 
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def oracle_world(args, out):
+    """The oracle on one sweep world: (event-loop seconds, packet-events, fct rows)."""
+    r = subprocess.run([os.path.join(ROOT, "oracle", "tcp_oracle")] + args + [f"--out={out}"], capture_output=True, text=True, check=True)
+    sec = float(re.search(r"Simulation Duration: ([0-9.]+)s", r.stdout).group(1))
+    ev = int(re.search(r"packet-events (\d+)", r.stdout).group(1))
+    rows = [tuple(int(x) for x in line.split(",")[4:6]) for line in open(out).read().splitlines()[1:]]
+    os.remove(out)
+    return sec, ev, rows
+
+
+def sweep_leg(a):
+    """Config 4: the batched sweep; returns the "sweep" object of the JSON line."""
+    import twins_amd
+    from twins_amd import autotune, workloads
+    wdir = os.path.join(a.sweep_dir, "worlds")
+    t0 = time.time()
+    wargs = workloads.config4_worlds(wdir, a.sweep_worlds, seed=4)
+    worlds = [twins_amd.World(x) for x in wargs]
+    log(f"sweep: {len(worlds)} worlds generated and built in {time.time() - t0:.1f} s")
+    tuner = autotune.Autotuner([w.info for w in worlds], twins_amd.launch_capacity())
+    for i in range(a.sweep_tune):
+        plan = tuner.plan()
+        t1 = time.time()
+        b = twins_amd.Batch(worlds, plan=plan)
+        t2 = time.time()
+        st = b.run()
+        tuner.observe(b)
+        tuner.record(plan, st.wall_s)
+        log(f"sweep tuning run {i}: teams {autotune.describe(plan)} pack {t2 - t1:.1f} s, run {st.wall_s:.3f} s, "
+            f"{st.packet_events / st.wall_s:.3e} events/s, graph {st.graph}")
+        del b
+    plan = tuner.best()
+    wall = kern = 0.0
+    events = 0
+    for i in range(a.steps):
+        b = twins_amd.Batch(worlds, plan=plan)
+        st = b.run()
+        wall += st.wall_s
+        kern += st.kernel_s
+        events += st.packet_events
+        log(f"sweep step {i}: {st.wall_s:.3f} s, {st.packet_events} packet-events, worlds done {st.worlds_done}")
+        if i + 1 < a.steps:
+            del b
+    ws = [b.world_stats(w) for w in range(len(worlds))]
+    top = sorted(range(len(worlds)), key=lambda w: -ws[w].device_s)[:5]
+    for w in top:
+        x = worlds[w].info
+        log(f"  longest: world {w} ({x.hosts} hosts, {x.switches} switches, {x.flows} flows) team {ws[w].team_blocks}, "
+            f"{ws[w].slots} slots, {ws[w].device_s:.3f} s ({1e6 * ws[w].device_s / max(1, ws[w].slots):.1f} us/slot)")
+    # plain per-world launches (each world alone, the reference's one-world-per-launch form) on a sample
+    sample = list(range(0, len(worlds), 8))
+    p_wall = p_ev = 0.0
+    for w in sample:
+        st1 = twins_amd.Batch([worlds[w]]).run()
+        p_wall += st1.wall_s
+        p_ev += st1.packet_events
+    log(f"sweep plain launches: {len(sample)} worlds, {p_wall:.2f} s")
+    # the oracle on every world, all host cores: parity and the CPU baseline
+    cores = a.cpu_cores
+    t3 = time.time()
+    with cf.ThreadPoolExecutor(cores) as ex:
+        res = list(ex.map(lambda i: oracle_world(wargs[i], os.path.join(a.sweep_dir, f"o{i:03d}.csv")), range(len(worlds))))
+    cpu_wall = time.time() - t3
+    same = total = worlds_same = 0
+    cpu_ev = 0
+    for i, (sec, ev, rows) in enumerate(res):
+        est, done = b.fct(i)
+        s1 = sum(r == g for r, g in zip(rows, zip(est, done)))
+        same += s1
+        total += len(rows)
+        worlds_same += s1 == len(rows) and b.counters(i)[26] == ev
+        cpu_ev += ev
+    log(f"sweep oracle: {cpu_wall:.1f} s on {cores} cores; parity {same}/{total}")
+    value = events / wall
+    achieved = BYTES_PER_EVENT * events / kern / 1e9
+    return {
+        "workload": f"config 4: {len(worlds)} worlds (seed 4) of FatTree k 4/8/16, leaf-spine 8-32 leaves, Jellyfish 32-256 and "
+                    f"torus 36-256 switches; ECN K, queue, TCP drawn per world; 1k-20k Pareto(1.2) flows each; one persistent "
+                    f"launch per step",
+        "metric": "simulated packet-events/s at bit-exact FCT",
+        "value": round(value, 1),
+        "unit": "packet-events/s",
+        "ms_per_step": round(1000 * wall / a.steps, 3),
+        "steps": a.steps,
+        "tuning_runs": a.sweep_tune,
+        "packet_events_per_step": events // a.steps,
+        "plan": {"teams": {str(k): v for k, v in autotune.describe(plan).items()}, "queues": len(set(plan["team_queue"]))},
+        "graph": bool(st.graph),
+        "parity": {"flows_identical_to_oracle": same, "flows": total, "worlds_identical": worlds_same, "worlds": len(worlds)},
+        "plain_per_world": {"value": round(p_ev / p_wall, 1), "unit": "packet-events/s", "sample": f"every 8th world ({len(sample)}), "
+                            "each alone in its own batch with the automatic launch shape, one after another"},
+        "speedup_vs_plain": round(value / (p_ev / p_wall), 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "tcp_worlds_kernel",
+                     "kernel_ms_per_step": round(1000 * kern / a.steps, 3), "bytes_per_event": BYTES_PER_EVENT},
+        "cpu_baseline": {"value": round(cpu_ev / cpu_wall, 1), "unit": "packet-events/s", "cores": cores, "kind": "port",
+                         "sample": f"the oracle on all {len(worlds)} worlds, one process per world, {cores} at a time (wall time "
+                                   "including process start and input parsing)"},
+    }
+
+
+def serve_leg(a):
+    """Config 5: the serving loop; returns the "serve" object of the JSON line."""
+    import twins_amd
+    from twins_amd import workloads
+    t0 = time.time()
+    reqs = workloads.config5_requests(os.path.join(a.sweep_dir, "requests"), a.serve_requests, seed=5, rate_per_s=a.serve_rate)
+    with cf.ThreadPoolExecutor(a.cpu_cores) as ex:
+        worlds = list(ex.map(lambda r: twins_amd.World(r[1]), reqs))
+    log(f"serve: {len(reqs)} requests generated and built in {time.time() - t0:.1f} s")
+    srv = twins_amd.Server(idle_s=2.0, large_teams=8, mid_teams=24)
+    n = len(reqs)
+    arrival, finish, results = {}, {}, {}
+    events = 0
+    submitted = 0
+    t_start = time.perf_counter()
+    last_log = t_start
+    while len(finish) < n:
+        now = time.perf_counter() - t_start
+        while submitted < n and reqs[submitted][0] <= now:
+            rid = srv.submit(worlds[submitted])
+            arrival[rid] = reqs[submitted][0]
+            submitted += 1
+        st = srv.poll()
+        for rid in list(arrival):
+            if rid in finish:
+                continue
+            s_ = srv.state(rid)
+            if s_ in (twins_amd.REQ_DONE, twins_amd.REQ_FAILED):
+                est, done, ev = srv.result(rid)
+                finish[rid] = time.perf_counter() - t_start
+                events += ev[26]
+                if rid % a.serve_check == 0:
+                    results[rid] = list(zip(est, done))
+        if time.perf_counter() - last_log > 10:
+            last_log = time.perf_counter()
+            log(f"serve: {now:.0f} s, {submitted} submitted, {len(finish)} done, resident {st.resident}, waiting {st.waiting}, "
+                f"arena {st.arena_used / 2**30:.1f}/{st.arena_bytes / 2**30:.1f} GiB")
+        time.sleep(0.0005)
+    wall = max(finish.values()) - reqs[0][0]
+    srv.close()
+    lat = sorted(finish[r] - arrival[r] for r in finish)
+    pct = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))]
+    log(f"serve: {n} requests in {wall:.1f} s, p50 {pct(0.5):.3f} s, p99 {pct(0.99):.3f} s")
+    # every serve_check-th request against the oracle (all host cores)
+    def check(rid):
+        out = os.path.join(a.sweep_dir, f"s{rid:04d}.csv")
+        _, ev, rows = oracle_world(reqs[rid][1], out)
+        return rid, ev, rows
+    same = total = 0
+    with cf.ThreadPoolExecutor(a.cpu_cores) as ex:
+        for rid, ev, rows in ex.map(check, sorted(results)):
+            same += sum(x == y for x, y in zip(rows, results[rid]))
+            total += len(rows)
+    log(f"serve: oracle parity {same}/{total} flows of {len(results)} requests")
+    return {
+        "workload": f"config 5: {n} requests arriving Poisson at {a.serve_rate:g}/s (seed 5), each a FatTree k in {{4,8,16,32}} "
+                    f"world with U{{1000..64000}} Pareto(1.2) flows and a TCP variant drawn per request, a quarter with a link "
+                    f"failure and recovery; TCP only (the RDMA model is not built yet)",
+        "metric": "sweep-request p50/p99 latency (arrival to flow times ready) and packet-events/s",
+        "p50_s": round(pct(0.5), 4),
+        "p99_s": round(pct(0.99), 4),
+        "value": round(events / wall, 1),
+        "unit": "packet-events/s",
+        "requests": n,
+        "wall_s": round(wall, 3),
+        "parity": {"flows_identical_to_oracle": same, "flows": total, "requests_checked": len(results)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +236,15 @@ def main():
     ap.add_argument("--team_blocks", type=int, default=0)
     ap.add_argument("--hosts_per_wave", type=int, default=0)
     ap.add_argument("--block_threads", type=int, default=0)
+    ap.add_argument("--sweep", type=int, default=1, help="also run the config-4 batched sweep (the \"sweep\" object)")
+    ap.add_argument("--sweep_worlds", type=int, default=256)
+    ap.add_argument("--sweep_tune", type=int, default=3, help="autotuning runs before the timed sweep steps")
+    ap.add_argument("--sweep_dir", default="/tmp/twins_sweep")
+    ap.add_argument("--cpu_cores", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--serve", type=int, default=1, help="also run the config-5 serving loop (the \"serve\" object)")
+    ap.add_argument("--serve_requests", type=int, default=300)
+    ap.add_argument("--serve_rate", type=float, default=100.0)
+    ap.add_argument("--serve_check", type=int, default=10, help="every n-th request is checked against the oracle")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU (the worlds of a sweep are independent and would be "
@@ -133,8 +326,13 @@ def main():
             "sample": "the oracle (sequential CPU restatement of the model) on the same 3 worlds, event loop only",
         },
     }
+    if a.sweep:
+        line["sweep"] = sweep_leg(a)
+    if a.serve:
+        line["serve"] = serve_leg(a)
     print(json.dumps(line), flush=True)
-    if same != total:
+    bad = [k for k in ("sweep", "serve") if k in line and line[k]["parity"]["flows_identical_to_oracle"] != line[k]["parity"]["flows"]]
+    if same != total or bad:
         sys.exit(1)
 
 

@@ -151,6 +151,41 @@ typedef struct tw_world_stats
 int tw_world_stats_get(tw_batch *b, int world, tw_world_stats *st);
 
 int tw_batch_destroy(tw_batch *b);
+
+/* The serving loop: one persistent kernel serves requests (one world each) as they arrive.
+ * A request is admitted in arrival order once its device footprint fits the server's arena,
+ * packed there (it stays resident until its result is taken), and handed to the kernel's queue
+ * for its size (teams of 1, 4 or 16 workgroups); requests run side by side, and each one's
+ * results are those of the same world run alone (tw_batch_run). No counterpart in the
+ * reference, which runs one simulation per process (src/main.cu). */
+typedef struct tw_server tw_server;
+typedef struct tw_server_cfg
+{
+    int32_t chain_runs;       /* 9 or 25 (0: 9) */
+    int32_t block_threads;    /* 0: 256 */
+    int32_t small_teams;      /* teams of one workgroup (0: what the GPU holds after the others and a free share) */
+    int32_t mid_teams;        /* teams of 4 workgroups, for worlds of 513-4096 hosts and switches (0: 16) */
+    int32_t large_teams;      /* teams of 16, for larger worlds (0: 4) */
+    int32_t max_requests;     /* requests over the server's life (0: 65536) */
+    uint64_t arena_bytes;     /* device memory for resident requests (0: 60% of the free memory) */
+    double idle_s;            /* a kernel team idle this long leaves; the kernel is relaunched on demand (0: 2 s) */
+} tw_server_cfg;
+enum { TW_REQ_WAITING = 0, TW_REQ_RESIDENT = 1, TW_REQ_DONE = 2, TW_REQ_FAILED = 3, TW_REQ_TAKEN = 4 };
+typedef struct tw_server_stats
+{
+    int32_t waiting, resident, done, failed;
+    uint64_t arena_used, arena_bytes, relaunches;
+} tw_server_stats;
+int tw_server_create(const tw_server_cfg *cfg, tw_server **out);
+/* Queues a request (the world is copied); admits what fits. */
+int tw_server_submit(tw_server *s, const tw_world *w, int32_t *id);
+/* Collects completions, admits waiting requests, relaunches the kernel if it left. */
+int tw_server_poll(tw_server *s, tw_server_stats *st);
+int tw_server_state(tw_server *s, int32_t id, int32_t *state);
+/* A finished request's flow times (as tw_fct_get) and counters (as tw_counters_get); frees its
+ * device memory. */
+int tw_server_result(tw_server *s, int32_t id, int64_t *established_ns, int64_t *completed_ns, uint64_t ev[32]);
+int tw_server_destroy(tw_server *s);
 const char *tw_last_error(void);
 
 #ifdef __cplusplus

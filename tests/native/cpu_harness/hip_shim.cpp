@@ -91,6 +91,13 @@ hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g)
 hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, hipGraphNode_t *, char *, size_t) { return hipErrorNotSupported; }
 hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorNotSupported; }
 hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+// the serving loop needs a GPU: its calls are refused here
+hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = (hipStream_t)malloc(8); return hipSuccess; }
+hipError_t hipStreamQuery(hipStream_t) { return hipSuccess; }
+hipError_t hipHostMalloc(void **, size_t, unsigned) { return hipErrorNotSupported; }
+hipError_t hipHostFree(void *) { return hipSuccess; }
+hipError_t hipHostGetDevicePointer(void **, void *, unsigned) { return hipErrorNotSupported; }
+hipError_t hipMemGetInfo(size_t *f, size_t *t) { *f = *t = 0; return hipSuccess; }
 hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
 const char *hipGetErrorString(hipError_t) { return "host stand-in"; }
 }

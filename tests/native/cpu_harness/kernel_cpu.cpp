@@ -126,6 +126,7 @@ hipError_t tcp_reset(uint32_t *queue_next, int nq, void *teams, size_t team_byte
     std::fill((char *)teams, (char *)teams + team_bytes, (char)0);
     return hipSuccess;
 }
+hipError_t tcp_server_launch(int, const void *, int, int, hipStream_t) { return hipErrorNotSupported; }
 hipError_t tcp_max_blocks(int, int, int *per_cu)
 {
     *per_cu = 4;

@@ -103,9 +103,11 @@ def test_solved_plan_is_valid_and_not_worse_than_one_workgroup_each(seed):
 
 def test_saturation_counts_work_items():
     s = autotune.WorldShape(info(16, 20, 4))
-    # k=4 FatTree: one host item (16 hosts on one wave) and two switch items (16 switches of 4 ports per item)
-    assert s.items(1, 4) == 1 + 2
-    assert s.saturation(4) == 1
+    # k=4 FatTree on one workgroup of 4 waves: 9 entities per item -> 2 host items, 3 switch items
+    assert s.items(1, 4) == 2 + 3
+    # 36 entities: one per wave from 9 workgroups of 4 waves (team sizes are powers of two: 16)
+    assert s.saturation(4) == 16
     big = autotune.WorldShape(info(1024, 320, 16))
+    # one workgroup: 64 hosts per item (16 items), 4 switches of 16 ports per item (80 items)
     assert big.items(1, 4) == 16 + 80
-    assert big.saturation(4) > 1
+    assert big.saturation(4) == 64

@@ -49,6 +49,19 @@ class WorldInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64)]
 
 
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("chain_runs", ctypes.c_int32), ("block_threads", ctypes.c_int32), ("small_teams", ctypes.c_int32),
+                ("mid_teams", ctypes.c_int32), ("large_teams", ctypes.c_int32), ("max_requests", ctypes.c_int32),
+                ("arena_bytes", ctypes.c_uint64), ("idle_s", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("waiting", ctypes.c_int32), ("resident", ctypes.c_int32), ("done", ctypes.c_int32), ("failed", ctypes.c_int32),
+                ("arena_used", ctypes.c_uint64), ("arena_bytes", ctypes.c_uint64), ("relaunches", ctypes.c_uint64)]
+
+
+REQ_WAITING, REQ_RESIDENT, REQ_DONE, REQ_FAILED, REQ_TAKEN = range(5)
+
 _lib = None
 
 
@@ -70,6 +83,13 @@ def lib():
         L.tw_counters_get.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_uint64)]
         L.tw_world_stats_get.argtypes = [vp, i32, ctypes.POINTER(WorldStats)]
         L.tw_launch_capacity.argtypes = [i32, i32, ctypes.POINTER(ctypes.c_int32)]
+        L.tw_server_create.argtypes = [ctypes.POINTER(ServerCfg), ctypes.POINTER(vp)]
+        L.tw_server_submit.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int32)]
+        L.tw_server_poll.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.tw_server_state.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int32)]
+        L.tw_server_result.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_uint64)]
+        L.tw_server_destroy.argtypes = [vp]
         L.tw_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
@@ -158,3 +178,47 @@ class Batch:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.tw_batch_destroy(self._h)
             self._h = None
+
+
+class Server:
+    """The serving loop (tw_server_* in include/twins.h): one persistent kernel runs requests (one
+    world each) as they are submitted; poll() collects completions and admits waiting requests by
+    device footprint; result() returns a finished request's flow times and counters."""
+
+    def __init__(self, chain_runs=9, block_threads=0, small_teams=0, mid_teams=0, large_teams=0, max_requests=0,
+                 arena_bytes=0, idle_s=0.0):
+        L = lib()
+        cfg = ServerCfg(chain_runs, block_threads, small_teams, mid_teams, large_teams, max_requests, arena_bytes, idle_s)
+        self._h = ctypes.c_void_p()
+        self._worlds = {}
+        _check(L.tw_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
+
+    def submit(self, world):
+        i = ctypes.c_int32()
+        _check(lib().tw_server_submit(self._h, world._h, ctypes.byref(i)))
+        self._worlds[i.value] = world.info.flows
+        return i.value
+
+    def poll(self):
+        st = ServerStats()
+        _check(lib().tw_server_poll(self._h, ctypes.byref(st)))
+        return st
+
+    def state(self, rid):
+        x = ctypes.c_int32()
+        _check(lib().tw_server_state(self._h, rid, ctypes.byref(x)))
+        return x.value
+
+    def result(self, rid):
+        n = self._worlds[rid]
+        est, done, ev = (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)(), (ctypes.c_uint64 * 32)()
+        _check(lib().tw_server_result(self._h, rid, est, done, ev))
+        return list(est), list(done), list(ev)
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.tw_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

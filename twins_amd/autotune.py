@@ -15,7 +15,8 @@ fitted to those measurements:
     per-slot time s_w(T) = c_w / min(T, sat_w) + o(T)
 
 with c_w the world's work per slot on one workgroup, sat_w the team size beyond which its work
-items (one per wavefront) run out, and o(T) the barrier and ticket overhead (one workgroup's
+items (about one per wavefront, with as few hosts or switches each as the team's waves allow)
+run out, and o(T) the barrier and ticket overhead (one workgroup's
 barrier is cheap; a team's is an agent-scope barrier). c_w is refitted from each new measurement.
 The plan is then solved for the least makespan M: each world takes the smallest team size whose
 predicted time fits M, worlds of one size share a queue in longest-first order, and each queue
@@ -39,19 +40,17 @@ class WorldShape:
         self.flows = info.flows
 
     def items(self, T, waves):
-        """Work items of one slot for a team of T workgroups of `waves` wavefronts (hosts per item as
-        the library chooses them: 64 on one workgroup, about one item per wave on a team)."""
-        if T == 1:
-            hpw = 64
-        else:
-            hpw = max(1, min(64, -(-(self.hosts + self.switches) // (T * waves))))
-        spw = max(1, 64 // self.k)
-        return -(-self.hosts // hpw) + -(-self.switches // spw)
+        """Work items of one slot for a team of T workgroups of `waves` wavefronts, as the kernel
+        deals them: about one per wave, ceil((hosts + switches) / (T waves)) hosts or switches each
+        (at most 64 hosts, 64 / k switches)."""
+        per = max(1, min(64, -(-(self.hosts + self.switches) // (T * waves))))
+        spw = min(per, max(1, 64 // self.k))
+        return -(-self.hosts // per) + -(-self.switches // spw)
 
     def saturation(self, waves):
-        """Team size beyond which more workgroups add no wavefront work (one item per wave)."""
+        """Team size beyond which more workgroups add nothing: one host or switch per item."""
         for T in TEAM_SIZES:
-            if self.items(T, waves) <= T * waves:
+            if T * waves >= self.hosts + self.switches:
                 return T
         return TEAM_SIZES[-1]
 

@@ -14,6 +14,8 @@ struct LaunchShape
     int64_t slot_budget;
 };
 hipError_t tcp_launch(int runs, const void *batch_dev, const LaunchShape &shape, hipStream_t stream);
+// the serving loop's persistent kernel over a ServerCtl in device memory
+hipError_t tcp_server_launch(int runs, const void *ctl_dev, int blocks, int threads, hipStream_t stream);
 // zeroes the world-queue counters and the team barrier state before a launch
 hipError_t tcp_reset(uint32_t *queue_next, int nq, void *teams, size_t team_bytes, hipStream_t stream);
 // workgroups of `threads` that one CU holds at once (times the CUs: the limit of a plan with teams)

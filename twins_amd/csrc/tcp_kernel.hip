@@ -24,32 +24,62 @@ namespace tw
 
 constexpr int LDS_PER_WAVE = 64 * 33 + 64 * 3 + 64 + 64; // switch phase: rows, in_win, base, cnt
 constexpr unsigned long long WAIT_TICKS = 2000000000ull;  // a team-barrier wait gives up after 20 s (100 MHz clock)
+constexpr unsigned long long SERVER_MAX_WAIT = 60000000000ull; // a server team waits 10 min at most for a request
 
 __device__ __forceinline__ unsigned ld_acquire(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
 
-template <int RUNS>
-__global__ void __launch_bounds__(512) tcp_worlds_kernel(const BatchT<RUNS> *__restrict__ Bp, int64_t slot_budget)
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p)
+{
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T *)((uint64_t)hi << 32 | lo);
+}
+
+// Work items when the plan leaves them to the world: about one item per wave of the team, i.e.
+// ceil((hosts + switches) / waves) hosts side by side in a host item and as many switches (at
+// most 64 / k) in a switch item. Fewer entities per wave cost lanes but save the divergence of
+// unlike event loops run in lockstep, which dominates a slot's time.
+__device__ __forceinline__ void auto_items(const WorldDesc &W, int T, int nwaves, int *hpw, int *spw)
+{
+    const int tw = T * nwaves, n = W.H + W.S;
+    int per = (n + tw - 1) / tw;
+    per = per < 1 ? 1 : per > 64 ? 64 : per;
+    *hpw = per;
+    *spw = per < 64 / W.k ? per : 64 / W.k;
+}
+
+// The world loop of both persistent kernels: SRV = false takes worlds from a batch's queues
+// (tcp_worlds_kernel), SRV = true takes requests from the serving loop's queues
+// (tcp_server_kernel). One body, specialised at compile time, so the batch kernel carries nothing
+// of the server's.
+template <int RUNS, bool SRV>
+__device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, const ServerCtl *__restrict__ Sp, int64_t slot_budget)
 {
     typedef Model<RUNS> Md;
-    const BatchT<RUNS> &B = *Bp;
     extern __shared__ int lds[];
     __shared__ int s_w;
     __shared__ int64_t s_t;
     __shared__ int s_done, s_err;
     __shared__ unsigned s_act;
     __shared__ unsigned long long s_wake;
+    __shared__ const BatchT<RUNS> *s_b;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwaves = blockDim.x >> 6;
-    const int team = B.team_of[blockIdx.x];
-    const TeamPlan P = B.plan[team];
-    const int T = P.T, rank = blockIdx.x - P.first, hpw = P.hpw;
+    const int32_t *team_of = SRV ? Sp->team_of : Bp->team_of;
+    const TeamPlan *plans = SRV ? Sp->plan : Bp->plan;
+    const int team = team_of[blockIdx.x];
+    const TeamPlan P = plans[team];
+    const int T = P.T, rank = blockIdx.x - P.first;
     const int TW = T * nwaves, gw = rank * nwaves + wv; // the team's waves; this wave's place
-    TeamState *ts = B.teams + team;
+    TeamState *ts = (SRV ? Sp->teams : Bp->teams) + team;
     int *wl = lds + wv * LDS_PER_WAVE;
     const int gwave = blockIdx.x * nwaves + wv; // this wave's global scratch
-    int64_t *scr_ts = B.scr_ts + (size_t)gwave * 64 * 2 * MAX_TX;
-    uint32_t *scr_ref = B.scr_ref + (size_t)gwave * 64 * 2 * MAX_TX;
-    uint8_t *scr_up = B.scr_up + (size_t)gwave * 64 * 2 * MAX_TX;
+    int64_t *scr_ts = (SRV ? Sp->scr_ts : Bp->scr_ts) + (size_t)gwave * 64 * 2 * MAX_TX;
+    uint32_t *scr_ref = (SRV ? Sp->scr_ref : Bp->scr_ref) + (size_t)gwave * 64 * 2 * MAX_TX;
+    uint8_t *scr_up = (SRV ? Sp->scr_up : Bp->scr_up) + (size_t)gwave * 64 * 2 * MAX_TX;
     unsigned gen = 0; // the team barrier's generation this block has passed
+    // a barrier wait's limit; the server's claim waits as long as a request may take to come
+    unsigned long long wait_limit = WAIT_TICKS;
 
     // the team barrier; the last workgroup to arrive runs `last` first
     auto team_barrier = [&](auto last) -> bool {
@@ -75,7 +105,7 @@ __global__ void __launch_bounds__(512) tcp_worlds_kernel(const BatchT<RUNS> *__r
                     while (ld_acquire(&ts->gen) == g)
                     {
                         __builtin_amdgcn_s_sleep(1);
-                        if (wall_clock64() - w0 > WAIT_TICKS)
+                        if (wall_clock64() - w0 > (SRV ? wait_limit : WAIT_TICKS))
                         {
                             atomicExch(&ts->err, 1);
                             break;
@@ -93,19 +123,69 @@ __global__ void __launch_bounds__(512) tcp_worlds_kernel(const BatchT<RUNS> *__r
 
     for (;;)
     {
-        // the team's next world, from its queue
-        if (team_barrier([&] { ts->world = atomicAdd(B.queue_next + P.queue, 1); }))
-            return;
-        if (tid == 0)
-            s_w = __hip_atomic_load(&ts->world, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        const int qi = s_w;
-        if (qi >= B.wq_len[P.queue])
-            return;
-        const int w = B.world_order[B.wq_off[P.queue] + qi];
+        int w = 0;
+        if constexpr (!SRV)
+        {
+            // the team's next world, from its queue
+            if (team_barrier([&] { ts->world = atomicAdd(Bp->queue_next + P.queue, 1); }))
+                return;
+            if (tid == 0)
+                s_w = __hip_atomic_load(&ts->world, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            const int qi = s_w;
+            if (qi >= Bp->wq_len[P.queue])
+                return;
+            w = Bp->world_order[Bp->wq_off[P.queue] + qi];
+        }
+        else
+        {
+            // claim the queue's next index, then wait until the host publishes its request: -1
+            // when the server shuts down, or when idle and the claim could be given back
+            const ServerCtl &S = *Sp;
+            const int q = P.queue;
+            wait_limit = SERVER_MAX_WAIT + WAIT_TICKS;
+            const bool err = team_barrier([&] {
+                unsigned i = atomicAdd(S.next + q, 1u);
+                int got = -1;
+                const unsigned long long w0 = wall_clock64();
+                for (;;)
+                {
+                    if (__hip_atomic_load(S.shutdown, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+                        break;
+                    if (i < __hip_atomic_load(S.tail + q, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM))
+                    {
+                        got = (int)i;
+                        break;
+                    }
+                    const unsigned long long waited = wall_clock64() - w0;
+                    if ((waited > S.idle_ticks && atomicCAS(S.next + q, i + 1, i) == i + 1) || waited > SERVER_MAX_WAIT)
+                        break;
+                    __builtin_amdgcn_s_sleep(8);
+                }
+                ts->world = got;
+            });
+            wait_limit = WAIT_TICKS;
+            if (err)
+                return;
+            if (tid == 0)
+            {
+                s_w = __hip_atomic_load(&ts->world, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (s_w >= 0)
+                    s_b = reinterpret_cast<const BatchT<RUNS> *>(S.req[(size_t)q * S.cap + s_w].batch);
+            }
+            __syncthreads();
+            if (s_w < 0)
+                return;
+        }
+        // (the request's batch pointer made wave-uniform, so its fields load into scalar registers
+        // as a kernel argument's do)
+        const BatchT<RUNS> &B = SRV ? *uniform_ptr(s_b) : *Bp;
         const WorldDesc &W = B.worlds[w];
+        int hpw = P.hpw, spw = 64 / W.k;
+        if (hpw <= 0)
+            auto_items(W, T, nwaves, &hpw, &spw);
         WorldRun *R = B.run + w;
-        const int k = W.k, spw = 64 / k;
+        const int k = W.k;
         const int host_items = (W.H + hpw - 1) / hpw, items = host_items + (W.S + spw - 1) / spw;
         const unsigned long long c_begin = wall_clock64();
         for (int64_t it = 0;; it++)
@@ -121,10 +201,17 @@ __global__ void __launch_bounds__(512) tcp_worlds_kernel(const BatchT<RUNS> *__r
             if (s_done)
             {
                 // every workgroup of the team leaves at the same slot; the first accounts the time
+                // (and, serving, tells the host)
                 if (rank == 0 && tid == 0)
                 {
                     R->ticks += wall_clock64() - c_begin;
                     R->team = T;
+                    if constexpr (SRV)
+                    {
+                        __threadfence_system();
+                        __hip_atomic_store(Sp->done + Sp->req[(size_t)P.queue * Sp->cap + s_w].id, 1, __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_SYSTEM);
+                    }
                 }
                 break;
             }
@@ -238,11 +325,37 @@ __global__ void __launch_bounds__(512) tcp_worlds_kernel(const BatchT<RUNS> *__r
                 {
                     R->error = 1;
                     R->finished = 1;
+                    if constexpr (SRV)
+                        if (rank == 0)
+                        {
+                            __threadfence_system();
+                            __hip_atomic_store(Sp->done + Sp->req[(size_t)P.queue * Sp->cap + s_w].id, 2, __ATOMIC_RELEASE,
+                                               __HIP_MEMORY_SCOPE_SYSTEM);
+                        }
                 }
                 return;
             }
         }
     }
+}
+
+template <int RUNS>
+__global__ void __launch_bounds__(512) tcp_worlds_kernel(const BatchT<RUNS> *__restrict__ Bp, int64_t slot_budget)
+{
+    world_loop<RUNS, false>(Bp, nullptr, slot_budget);
+}
+
+// The serving loop's persistent kernel. Requests are single-world batches already in device
+// memory; the host appends them to per-queue rings in pinned host memory and publishes each
+// queue's tail there. A team claims the next index of its queue, waits until the host has
+// published it (or the server shuts down), runs the world to its end and flags it done in
+// pinned memory. Every wait is bounded: an idle team gives its claim back and leaves after
+// ServerCtl::idle_ticks, the host's shutdown flag ends every wait, and the host relaunches the
+// kernel when it finds it gone.
+template <int RUNS>
+__global__ void __launch_bounds__(512) tcp_server_kernel(const ServerCtl *__restrict__ Sp)
+{
+    world_loop<RUNS, true>(nullptr, Sp, INT64_MAX);
 }
 
 template <int RUNS>
@@ -277,6 +390,17 @@ hipError_t tcp_reset(uint32_t *queue_next, int nq, void *teams, size_t team_byte
     int blocks = (nq + nb + 255) / 256;
     blocks = blocks < 1 ? 1 : blocks > 1024 ? 1024 : blocks;
     hipLaunchKernelGGL(tcp_reset_kernel, dim3(blocks), dim3(256), 0, stream, queue_next, nq, (uint32_t *)teams, nb);
+    return hipGetLastError();
+}
+
+hipError_t tcp_server_launch(int runs, const void *ctl_dev, int blocks, int threads, hipStream_t stream)
+{
+    const size_t lds = (size_t)(threads / 64) * LDS_PER_WAVE * sizeof(int);
+    const ServerCtl *c = reinterpret_cast<const ServerCtl *>(ctl_dev);
+    if (runs == 25)
+        hipLaunchKernelGGL(tcp_server_kernel<25>, dim3(blocks), dim3(threads), lds, stream, c);
+    else
+        hipLaunchKernelGGL(tcp_server_kernel<9>, dim3(blocks), dim3(threads), lds, stream, c);
     return hipGetLastError();
 }
 

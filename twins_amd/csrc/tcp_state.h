@@ -184,6 +184,32 @@ struct alignas(128) TeamState
     unsigned long long prof[4]; // TW_PROF: this slot's maxima (hosts, switches, work); last ticket time
 };
 
+// The serving loop (tcp_server_kernel): requests are single-world batches resident in device
+// memory. Queue q's requests are req[q * cap + i] in order; the host publishes tail[q] and the
+// shutdown flag, and reads done[id] (1: finished, 2: failed), all in pinned host memory; the
+// teams claim indices with next[q] (device memory).
+struct ServerReq
+{
+    const void *batch; // BatchT in device memory
+    int32_t id, pad;
+};
+struct ServerCtl
+{
+    ServerReq *req;
+    unsigned *tail;
+    int *shutdown;
+    int *done;
+    unsigned *next;
+    int32_t nq, cap;
+    unsigned long long idle_ticks; // a team waiting this long for a request gives its claim back
+    int32_t *team_of;
+    TeamPlan *plan;
+    TeamState *teams;
+    int64_t *scr_ts;
+    uint32_t *scr_ref;
+    uint8_t *scr_up;
+};
+
 // every device pointer of a batch (one copy in device memory, passed by pointer)
 template <int RUNS>
 struct BatchT
@@ -210,6 +236,7 @@ struct BatchT
     int32_t *host_world, *pend_next, *pend_end, *n_snd, *n_rcv, *act_snd, *act_rcv;
     uint8_t *host_cc;
     int64_t *wake;
+    int64_t *tmin_s, *tmin_r; // per host: lower bounds of its connections' sender / receiver timers
     Due *due; // [host][MAX_DUE]
     // per link: the link whose frames reach the entity that sends on it (-1: an unused port)
     int32_t *in_link;

@@ -4,6 +4,8 @@
 // Reference: the set-up and report parts of src/main.cu.
 #include <algorithm>
 #include <chrono>
+#include <map>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -833,6 +835,64 @@ extern "C" int tw_world_create_args(int argc, const char *const *argv, tw_world 
 }
 
 // ------------------------------------------------------------------ batches
+// A device-memory arena: one allocation made up front, carved first-fit in 256-byte units (the
+// serving loop's requests: no hipMalloc or hipFree, which may wait for the device, while its
+// persistent kernel runs).
+struct Arena
+{
+    char *base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_; // offset -> bytes
+    size_t used = 0;
+    std::mutex mu;
+    void *alloc(size_t n)
+    {
+        n = (n + 255) & ~(size_t)255;
+        std::lock_guard<std::mutex> g(mu);
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= n)
+            {
+                size_t off = it->first, left = it->second - n;
+                free_.erase(it);
+                if (left)
+                    free_[off + n] = left;
+                used += n;
+                sizes[off] = n;
+                return base + off;
+            }
+        return nullptr;
+    }
+    void release(void *p)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        size_t off = (char *)p - base;
+        auto sz = sizes.find(off);
+        if (sz == sizes.end())
+            return;
+        size_t n = sz->second;
+        sizes.erase(sz);
+        used -= n;
+        auto nx = free_.lower_bound(off);
+        if (nx != free_.end() && nx->first == off + n)
+        {
+            n += nx->second;
+            free_.erase(nx);
+        }
+        auto pv = free_.lower_bound(off);
+        if (pv != free_.begin())
+        {
+            --pv;
+            if (pv->first + pv->second == off)
+            {
+                pv->second += n;
+                return;
+            }
+        }
+        free_[off] = n;
+    }
+    std::map<size_t, size_t> sizes;
+};
+
 struct tw_batch
 {
     int runs = 9;
@@ -848,6 +908,9 @@ struct tw_batch
     std::vector<TeamPlan> plan;
     int32_t *queue_next = nullptr;
     bool no_graph = false, graph_failed = false;
+    Arena *arena = nullptr;     // allocations come from it (else hipMalloc)
+    bool for_server = false;    // a serving-loop request (no launch plan of its own)
+    hipStream_t up = nullptr;   // with an arena: uploads go on this stream
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     WorldRun *run = nullptr;
@@ -860,7 +923,10 @@ struct tw_batch
         if (graph)
             (void)hipGraphDestroy(graph);
         for (void *p : bufs)
-            (void)hipFree(p);
+            if (arena)
+                arena->release(p);
+            else
+                (void)hipFree(p);
         if (ev0)
             (void)hipEventDestroy(ev0);
         if (ev1)
@@ -883,9 +949,18 @@ static int dev_alloc(tw_batch &b, T **out, size_t n)
 {
     size_t bytes = sizeof(T) * std::max<size_t>(n, 1);
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess)
-        return fail(TW_ENOMEM, "hipMalloc of %zu bytes: %s", bytes, hipGetErrorString(e));
+    if (b.arena)
+    {
+        p = b.arena->alloc(bytes);
+        if (!p)
+            return fail(TW_ENOMEM, "the device arena has no %zu free bytes", bytes);
+    }
+    else
+    {
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess)
+            return fail(TW_ENOMEM, "hipMalloc of %zu bytes: %s", bytes, hipGetErrorString(e));
+    }
     b.bufs.push_back(p);
     *out = (T *)p;
     return TW_OK;
@@ -897,7 +972,12 @@ static int dev_upload(tw_batch &b, T **out, const std::vector<T> &v)
     if (rc)
         return rc;
     if (!v.empty())
-        HIPCHK(hipMemcpy(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    {
+        if (b.up)
+            HIPCHK(hipMemcpyAsync(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, b.up));
+        else
+            HIPCHK(hipMemcpy(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    }
     return TW_OK;
 }
 template <typename T>
@@ -906,7 +986,10 @@ static int dev_zero(tw_batch &b, T **out, size_t n)
     int rc = dev_alloc(b, out, n);
     if (rc)
         return rc;
-    HIPCHK(hipMemset(*out, 0, sizeof(T) * std::max<size_t>(n, 1)));
+    if (b.up)
+        HIPCHK(hipMemsetAsync(*out, 0, sizeof(T) * std::max<size_t>(n, 1), b.up));
+    else
+        HIPCHK(hipMemset(*out, 0, sizeof(T) * std::max<size_t>(n, 1)));
     return TW_OK;
 }
 
@@ -1071,6 +1154,11 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     ZERO(act_rcv, H * MAX_ACTIVE);
     UP(host_cc, host_cc);
     UP(wake, wake);
+    {
+        std::vector<int64_t> none(H, INT64_MAX);
+        UP(tmin_s, none);
+        UP(tmin_r, none);
+    }
     ZERO(due, H * MAX_DUE);
     UP(in_link, in_link);
     UP(sw_world, sw_world);
@@ -1098,6 +1186,21 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     UP(g_port_dn, g_pdn);
     UP(g_fail_t, g_fail);
     UP(sw_rng, rng);
+    if (b.for_server)
+    {
+        // a serving-loop request: the server's kernel brings the teams and the scratch
+        b.run = B.run;
+        b.mg_est = B.mg_est;
+        b.mg_done = B.mg_done;
+        b.run_host = run;
+        Batch *bd = nullptr;
+        if ((rc = dev_alloc(b, &bd, 1)))
+            return rc;
+        HIPCHK(hipMemcpyAsync(bd, &B, sizeof B, hipMemcpyHostToDevice, b.up));
+        HIPCHK(hipStreamSynchronize(b.up));
+        b.batch_dev = bd;
+        return TW_OK;
+    }
     // launch plan: teams of workgroups, each taking worlds from one queue
     int dev = 0, cus = 256, per_cu = 1;
     HIPCHK(hipGetDevice(&dev));
@@ -1157,16 +1260,12 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     }
     // the queues' worlds, in order
     std::vector<int32_t> worder, wq_off(nq), wq_len(nq, 0), q_teams(nq, 0);
-    std::vector<int64_t> q_largest(nq, 0);
     for (int q = 0; q < nq; q++)
     {
         wq_off[q] = (int32_t)worder.size();
         for (int i : order)
             if (world_q[i] == q)
-            {
                 worder.push_back(i);
-                q_largest[q] = std::max(q_largest[q], size_of(i));
-            }
         wq_len[q] = (int32_t)worder.size() - wq_off[q];
     }
     const int nteams = (int)team_T.size();
@@ -1185,14 +1284,9 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
         tp.first = blocks;
         tp.T = team_T[j];
         tp.queue = team_q[j];
-        if (cfg.hosts_per_wave > 0)
-            tp.hpw = std::min(64, cfg.hosts_per_wave);
-        else
-        {
-            // about one work item per wave of the team for its queue's largest world
-            const int64_t tw = (int64_t)tp.T * (sh.threads / 64);
-            tp.hpw = tp.T == 1 ? 64 : (int)std::max<int64_t>(1, std::min<int64_t>(64, (q_largest[tp.queue] + tw - 1) / tw));
-        }
+        // hosts per wave item: as asked, else chosen per world by the kernel (about one work item
+        // per wave of the team)
+        tp.hpw = cfg.hosts_per_wave > 0 ? std::min(64, cfg.hosts_per_wave) : 0;
         blocks += tp.T;
         team_of.insert(team_of.end(), tp.T, j);
         sh.teams |= tp.T > 1;
@@ -1425,5 +1519,335 @@ extern "C" int tw_counters_get(tw_batch *b, int world, uint64_t ev[32])
 extern "C" int tw_batch_destroy(tw_batch *b)
 {
     delete b;
+    return TW_OK;
+}
+
+// ------------------------------------------------------------------ the serving loop
+// Requests (one world each) are admitted in arrival order while their device footprint fits the
+// arena, packed there, and published to the persistent server kernel's queue for their size; the
+// kernel runs them side by side on its teams and flags each done in pinned host memory. The host
+// side (tw_server_poll) collects completions, admits waiting requests as memory frees up, and
+// relaunches the kernel when it has left (idle) while requests wait. No counterpart in the
+// reference, whose runs are one process per simulation.
+namespace
+{
+struct SrvReq
+{
+    std::unique_ptr<tw_world> w;  // until admitted
+    std::unique_ptr<tw_batch> b;  // once admitted, until its result is taken
+    int queue = -1;
+    uint32_t index = 0;
+    int state = TW_REQ_WAITING;
+};
+} // namespace
+
+struct tw_server
+{
+    int runs = 9, threads = 256, blocks = 0, nq = 3, cap = 0;
+    Arena arena;
+    hipStream_t ks = nullptr, up = nullptr;
+    ServerCtl *ctl_d = nullptr;
+    ServerReq *req = nullptr; // pinned
+    unsigned *tail = nullptr;
+    int *shutdown = nullptr, *done = nullptr;
+    unsigned *next_d = nullptr;
+    TeamState *teams_d = nullptr;
+    int nteams = 0;
+    int64_t small_max = 512, mid_max = 4096; // queue by entities: small, mid, large
+    std::vector<unsigned> published;
+    std::vector<SrvReq> reqs;
+    std::vector<int> waiting; // ids, in arrival order
+    size_t wait_head = 0;
+    std::vector<void *> dev, host;
+    bool launched = false;
+    uint64_t relaunches = 0;
+    ~tw_server()
+    {
+        if (shutdown)
+            __atomic_store_n(shutdown, 1, __ATOMIC_RELEASE);
+        if (ks)
+            (void)hipStreamSynchronize(ks);
+        reqs.clear();
+        for (void *p : dev)
+            (void)hipFree(p);
+        for (void *p : host)
+            (void)hipHostFree(p);
+        if (ks)
+            (void)hipStreamDestroy(ks);
+        if (up)
+            (void)hipStreamDestroy(up);
+    }
+};
+
+template <typename T>
+static int srv_dev(tw_server &s, T **out, size_t n)
+{
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1));
+    if (e != hipSuccess)
+        return fail(TW_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
+    s.dev.push_back(p);
+    *out = (T *)p;
+    return TW_OK;
+}
+template <typename T>
+static int srv_pinned(tw_server &s, T **out, size_t n)
+{
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocMapped);
+    if (e != hipSuccess)
+        return fail(TW_ENOMEM, "hipHostMalloc: %s", hipGetErrorString(e));
+    memset(p, 0, sizeof(T) * std::max<size_t>(n, 1));
+    s.host.push_back(p);
+    *out = (T *)p;
+    return TW_OK;
+}
+template <typename T>
+static T *mapped(T *p)
+{
+    void *d = nullptr;
+    return hipHostGetDevicePointer(&d, p, 0) == hipSuccess ? (T *)d : p;
+}
+
+extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
+{
+    if (!out)
+        return fail(TW_EINVAL, "null output");
+    tw_server_cfg c;
+    memset(&c, 0, sizeof c);
+    if (cfg)
+        c = *cfg;
+    int ndev = 0, dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(TW_EDEVICE, "no GPU: libtwins_core runs the model on the GPU only");
+    std::unique_ptr<tw_server> s(new tw_server);
+    s->runs = c.chain_runs == 25 ? 25 : 9;
+    s->threads = c.block_threads > 0 ? c.block_threads : 256;
+    if (s->threads % 64 || s->threads > 512)
+        return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 512");
+    s->cap = c.max_requests > 0 ? c.max_requests : 65536;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(tcp_max_blocks(s->runs, s->threads, &per_cu));
+    const int resident = std::max(1, per_cu) * cus;
+    // teams: large worlds on teams of 16 workgroups, mid-size on 4, small on one; a share of the
+    // GPU stays free for the uploads' fill kernels
+    const int nl = c.large_teams > 0 ? c.large_teams : 4, nm = c.mid_teams > 0 ? c.mid_teams : 16;
+    const int headroom = cus / 4;
+    const int ns = c.small_teams > 0 ? c.small_teams : resident - headroom - 16 * nl - 4 * nm;
+    if (ns < 1 || ns + 4 * nm + 16 * nl > resident)
+        return fail(TW_EINVAL, "server teams (%d + 4 x %d + 16 x %d workgroups) do not fit the %d the GPU holds", ns, nm, nl, resident);
+    std::vector<TeamPlan> plan;
+    std::vector<int32_t> team_of;
+    auto add = [&](int n, int T, int q) {
+        for (int i = 0; i < n; i++)
+        {
+            TeamPlan p{(int32_t)team_of.size(), T, 0, q};
+            team_of.insert(team_of.end(), T, (int32_t)plan.size());
+            plan.push_back(p);
+        }
+    };
+    add(ns, 1, 0);
+    add(nm, 4, 1);
+    add(nl, 16, 2);
+    s->blocks = (int)team_of.size();
+    s->nteams = (int)plan.size();
+    // the arena
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    const size_t want = c.arena_bytes ? c.arena_bytes : fr / 10 * 6;
+    void *ab = nullptr;
+    if (hipMalloc(&ab, want) != hipSuccess)
+        return fail(TW_ENOMEM, "arena of %zu bytes", want);
+    s->dev.push_back(ab);
+    s->arena.base = (char *)ab;
+    s->arena.size = want;
+    s->arena.free_[0] = want;
+    HIPCHK(hipStreamCreateWithFlags(&s->ks, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking));
+    int rc;
+    if ((rc = srv_pinned(*s, &s->req, (size_t)s->nq * s->cap)) || (rc = srv_pinned(*s, &s->tail, s->nq)) ||
+        (rc = srv_pinned(*s, &s->shutdown, 1)) || (rc = srv_pinned(*s, &s->done, s->cap)))
+        return rc;
+    ServerCtl h;
+    memset(&h, 0, sizeof h);
+    h.req = mapped(s->req);
+    h.tail = mapped(s->tail);
+    h.shutdown = mapped(s->shutdown);
+    h.done = mapped(s->done);
+    h.nq = s->nq;
+    h.cap = s->cap;
+    h.idle_ticks = (unsigned long long)((c.idle_s > 0 ? c.idle_s : 2.0) * 100e6);
+    const size_t waves = (size_t)s->blocks * (s->threads / 64);
+    if ((rc = srv_dev(*s, &s->next_d, s->nq)) || (rc = srv_dev(*s, &s->teams_d, s->nteams)) || (rc = srv_dev(*s, &h.team_of, team_of.size())) ||
+        (rc = srv_dev(*s, &h.plan, plan.size())) || (rc = srv_dev(*s, &h.scr_ts, waves * 64 * 2 * MAX_TX)) ||
+        (rc = srv_dev(*s, &h.scr_ref, waves * 64 * 2 * MAX_TX)) || (rc = srv_dev(*s, &h.scr_up, waves * 64 * 2 * MAX_TX)) ||
+        (rc = srv_dev(*s, &s->ctl_d, 1)))
+        return rc;
+    h.next = s->next_d;
+    h.teams = s->teams_d;
+    HIPCHK(hipMemcpy(h.team_of, team_of.data(), 4 * team_of.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h.plan, plan.data(), sizeof(TeamPlan) * plan.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(s->next_d, 0, 4 * s->nq));
+    HIPCHK(hipMemset(s->teams_d, 0, sizeof(TeamState) * s->nteams));
+    HIPCHK(hipMemcpy(s->ctl_d, &h, sizeof h, hipMemcpyHostToDevice));
+    s->published.assign(s->nq, 0);
+    *out = s.release();
+    return TW_OK;
+}
+
+// completions, admissions, and a relaunch of the kernel when it has left while requests wait
+static int srv_service(tw_server &s)
+{
+    for (auto &r : s.reqs)
+        if (r.state == TW_REQ_RESIDENT)
+        {
+            int d = __atomic_load_n(&s.done[&r - &s.reqs[0]], __ATOMIC_ACQUIRE);
+            if (d)
+                r.state = d == 1 ? TW_REQ_DONE : TW_REQ_FAILED;
+        }
+    while (s.wait_head < s.waiting.size())
+    {
+        const int id = s.waiting[s.wait_head];
+        SrvReq &r = s.reqs[id];
+        if (world_bytes(*r.w, s.runs) + (64u << 20) > s.arena.size - s.arena.used)
+            break; // admitted in order: the next waits for memory
+        std::unique_ptr<tw_batch> b(new tw_batch);
+        b->runs = s.runs;
+        b->arena = &s.arena;
+        b->up = s.up;
+        b->for_server = true;
+        b->worlds.emplace_back(new tw_world(*r.w));
+        tw_pack_cfg pc;
+        memset(&pc, 0, sizeof pc);
+        int rc = s.runs == 25 ? pack<25>(*b, pc) : pack<9>(*b, pc);
+        if (rc == TW_ENOMEM)
+            break; // the arena is fragmented: wait for more to free
+        if (rc)
+            return rc;
+        const int64_t n = (int64_t)r.w->H + r.w->S;
+        r.queue = n <= s.small_max ? 0 : n <= s.mid_max ? 1 : 2;
+        r.index = s.published[r.queue];
+        if ((int)r.index >= s.cap)
+            return fail(TW_ESTATE, "queue %d is full (%d requests)", r.queue, s.cap);
+        s.req[(size_t)r.queue * s.cap + r.index] = ServerReq{b->batch_dev, id, 0};
+        r.b = std::move(b);
+        r.w.reset();
+        r.state = TW_REQ_RESIDENT;
+        s.published[r.queue] = r.index + 1;
+        __atomic_store_n(&s.tail[r.queue], r.index + 1, __ATOMIC_RELEASE);
+        s.wait_head++;
+    }
+    bool unserved = false;
+    for (auto &r : s.reqs)
+        unserved |= r.state == TW_REQ_RESIDENT;
+    if (!unserved)
+        return TW_OK;
+    if (s.launched)
+    {
+        hipError_t q = hipStreamQuery(s.ks);
+        if (q == hipErrorNotReady)
+            return TW_OK;
+        HIPCHK(q);
+        // the kernel has left: look again at the flags it set before restarting the counters
+        for (auto &r : s.reqs)
+            if (r.state == TW_REQ_RESIDENT)
+            {
+                int d = __atomic_load_n(&s.done[&r - &s.reqs[0]], __ATOMIC_ACQUIRE);
+                if (d)
+                    r.state = d == 1 ? TW_REQ_DONE : TW_REQ_FAILED;
+            }
+        s.relaunches++;
+    }
+    // each queue restarts at its first request not yet run
+    std::vector<unsigned> first(s.published);
+    for (auto &r : s.reqs)
+        if (r.state == TW_REQ_RESIDENT)
+            first[r.queue] = std::min(first[r.queue], r.index);
+    std::vector<TeamState> zero(s.nteams);
+    memset(zero.data(), 0, sizeof(TeamState) * s.nteams);
+    HIPCHK(hipMemcpyAsync(s.next_d, first.data(), 4 * s.nq, hipMemcpyHostToDevice, s.up));
+    HIPCHK(hipMemcpyAsync(s.teams_d, zero.data(), sizeof(TeamState) * s.nteams, hipMemcpyHostToDevice, s.up));
+    HIPCHK(hipStreamSynchronize(s.up));
+    HIPCHK(tcp_server_launch(s.runs, s.ctl_d, s.blocks, s.threads, s.ks));
+    s.launched = true;
+    return TW_OK;
+}
+
+extern "C" int tw_server_submit(tw_server *s, const tw_world *w, int32_t *id)
+{
+    if (!s || !w || !id)
+        return fail(TW_EINVAL, "null argument");
+    if ((int)s->reqs.size() >= s->cap)
+        return fail(TW_ESTATE, "the server took its %d requests", s->cap);
+    if (world_bytes(*w, s->runs) + (64u << 20) > s->arena.size)
+        return fail(TW_ENOMEM, "the world needs more device memory than the arena holds");
+    SrvReq r;
+    r.w.reset(new tw_world(*w));
+    s->reqs.push_back(std::move(r));
+    *id = (int32_t)s->reqs.size() - 1;
+    s->waiting.push_back(*id);
+    return srv_service(*s);
+}
+
+extern "C" int tw_server_poll(tw_server *s, tw_server_stats *st)
+{
+    if (!s)
+        return fail(TW_EINVAL, "null server");
+    int rc = srv_service(*s);
+    if (rc)
+        return rc;
+    if (st)
+    {
+        memset(st, 0, sizeof *st);
+        for (auto &r : s->reqs)
+        {
+            st->waiting += r.state == TW_REQ_WAITING;
+            st->resident += r.state == TW_REQ_RESIDENT;
+            st->done += r.state == TW_REQ_DONE || r.state == TW_REQ_TAKEN;
+            st->failed += r.state == TW_REQ_FAILED;
+        }
+        st->arena_used = s->arena.used;
+        st->arena_bytes = s->arena.size;
+        st->relaunches = s->relaunches;
+    }
+    return TW_OK;
+}
+
+extern "C" int tw_server_state(tw_server *s, int32_t id, int32_t *state)
+{
+    if (!s || !state || id < 0 || id >= (int)s->reqs.size())
+        return fail(TW_EINVAL, "no request %d", id);
+    *state = s->reqs[id].state;
+    return TW_OK;
+}
+
+extern "C" int tw_server_result(tw_server *s, int32_t id, int64_t *est_ns, int64_t *done_ns, uint64_t ev[32])
+{
+    if (!s || id < 0 || id >= (int)s->reqs.size())
+        return fail(TW_EINVAL, "no request %d", id);
+    SrvReq &r = s->reqs[id];
+    if (r.state != TW_REQ_DONE && r.state != TW_REQ_FAILED)
+        return fail(TW_ESTATE, "request %d is not finished (state %d)", id, r.state);
+    tw_batch &b = *r.b;
+    HIPCHK(hipMemcpyAsync(b.run_host.data(), b.run, sizeof(WorldRun), hipMemcpyDeviceToHost, s->up));
+    HIPCHK(hipStreamSynchronize(s->up));
+    int rc = tw_fct_get(&b, 0, est_ns, done_ns);
+    if (rc)
+        return rc;
+    if (ev)
+        for (int i = 0; i < 32; i++)
+            ev[i] = b.run_host[0].ev[i];
+    if (ev)
+        ev[31] = b.run_host[0].slot;
+    const bool failed = r.state == TW_REQ_FAILED;
+    r.b.reset(); // its memory goes back to the arena
+    r.state = TW_REQ_TAKEN;
+    return failed ? fail(TW_EDEVICE, "request %d failed on the device (a team wait timed out)", id) : TW_OK;
+}
+
+extern "C" int tw_server_destroy(tw_server *s)
+{
+    delete s;
     return TW_OK;
 }
