@@ -77,7 +77,8 @@ def sweep_leg(a):
     wargs = workloads.config4_worlds(wdir, a.sweep_worlds, seed=4)
     worlds = [twins_amd.World(x) for x in wargs]
     log(f"sweep: {len(worlds)} worlds generated and built in {time.time() - t0:.1f} s")
-    tuner = autotune.Autotuner([w.info for w in worlds], twins_amd.launch_capacity())
+    # plans of one 512-thread workgroup per CU (co-resident workgroups of other worlds slow each other)
+    tuner = autotune.Autotuner([w.info for w in worlds], twins_amd.launch_capacity(9, 512), block_threads=512)
     for i in range(a.sweep_tune):
         plan = tuner.plan()
         t1 = time.time()
@@ -168,7 +169,7 @@ def serve_leg(a):
     with cf.ThreadPoolExecutor(a.cpu_cores) as ex:
         worlds = list(ex.map(lambda r: twins_amd.World(r[1]), reqs))
     log(f"serve: {len(reqs)} requests generated and built in {time.time() - t0:.1f} s")
-    srv = twins_amd.Server(idle_s=2.0, large_teams=8, mid_teams=24)
+    srv = twins_amd.Server(idle_s=2.0)
     n = len(reqs)
     arrival, finish, results = {}, {}, {}
     events = 0

@@ -67,9 +67,9 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nwaves = blockDim.x >> 6;
     const int32_t *team_of = SRV ? Sp->team_of : Bp->team_of;
     const TeamPlan *plans = SRV ? Sp->plan : Bp->plan;
-    const int team = team_of[blockIdx.x];
+    const int team = team_of[blockIdx.x] >> 8;
     const TeamPlan P = plans[team];
-    const int T = P.T, rank = blockIdx.x - P.first;
+    const int T = P.T, rank = team_of[blockIdx.x] & 255;
     const int TW = T * nwaves, gw = rank * nwaves + wv; // the team's waves; this wave's place
     TeamState *ts = (SRV ? Sp->teams : Bp->teams) + team;
     int *wl = lds + wv * LDS_PER_WAVE;

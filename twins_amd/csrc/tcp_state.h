@@ -162,8 +162,9 @@ struct WorldRun
     unsigned long long ev[EV_N];
 };
 
-// a team of the launch plan: its workgroups blockIdx first .. first + T - 1, the hosts per wave work
-// item it uses, and the queue of worlds it takes from
+// a team of the launch plan: its size T, the hosts per wave work item it uses (0: per world), and
+// the queue of worlds it takes from; first: its lowest workgroup. Its workgroups are those whose
+// team_of entry is team << 8 | rank.
 struct TeamPlan
 {
     int32_t first, T, hpw, queue;
@@ -258,7 +259,7 @@ struct BatchT
     TwMrgState *sw_rng; // per-packet ECMP streams
     // kernel work queues: queue q holds worlds world_order[wq_off[q] .. wq_off[q] + wq_len[q] - 1] in
     // the order its teams take them, queue_next[q] counts those taken; the plan's teams (team_of:
-    // workgroup -> team) and their barriers
+    // workgroup -> team << 8 | rank in the team) and their barriers
     int32_t *world_order;
     int32_t *wq_off, *wq_len;
     int32_t *queue_next;

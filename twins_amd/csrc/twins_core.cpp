@@ -993,6 +993,43 @@ static int dev_zero(tw_batch &b, T **out, size_t n)
     return TW_OK;
 }
 
+// Workgroups of a launch go to MI355X's 8 XCDs round-robin by workgroup id (XCD = id % 8), and
+// each XCD has its own L2. A team's workgroups share its world's state every slot, so they are
+// placed on one XCD: team j gets ids with one residue mod 8 (largest teams first, each on the XCD
+// with the most ids left); a team larger than an XCD's share spans several. Fills team_of
+// (workgroup -> team << 8 | rank) and each team's lowest workgroup.
+constexpr int XCDS = 8;
+static void place_teams(std::vector<TeamPlan> &plan, std::vector<int32_t> &team_of)
+{
+    int n = 0;
+    for (auto &p : plan)
+        n += p.T;
+    team_of.assign(n, -1);
+    std::vector<std::vector<int>> freeb(XCDS);
+    for (int b = n - 1; b >= 0; b--)
+        freeb[b % XCDS].push_back(b); // popped from the back: lowest ids first
+    std::vector<int> order(plan.size());
+    for (size_t j = 0; j < plan.size(); j++)
+        order[j] = (int)j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return plan[a].T > plan[c].T; });
+    for (int j : order)
+    {
+        int rank = 0;
+        plan[j].first = n;
+        while (rank < plan[j].T)
+        {
+            int x = 0;
+            for (int y = 1; y < XCDS; y++)
+                if (freeb[y].size() > freeb[x].size())
+                    x = y;
+            const int b = freeb[x].back();
+            freeb[x].pop_back();
+            team_of[b] = j << 8 | rank++;
+            plan[j].first = std::min(plan[j].first, b);
+        }
+    }
+}
+
 template <int RUNS>
 static int pack(tw_batch &b, const tw_pack_cfg &cfg)
 {
@@ -1206,7 +1243,10 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     LaunchShape &sh = b.shape;
-    sh.threads = cfg.block_threads > 0 ? cfg.block_threads : 256;
+    // workgroup size: 512 threads (a full CU) for sweeps and explicit plans, which put one workgroup
+    // per CU (co-resident workgroups of other worlds slow each other more than they hide each
+    // other's latency); 256 when a few large worlds are spread over teams (measured faster there)
+    sh.threads = cfg.block_threads > 0 ? cfg.block_threads : (cfg.plan || W >= cus ? 512 : 256);
     if (sh.threads % 64 || sh.threads > 512)
         return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 512");
     HIPCHK(tcp_max_blocks(RUNS, sh.threads, &per_cu));
@@ -1281,16 +1321,17 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     for (int j = 0; j < nteams; j++)
     {
         TeamPlan &tp = plan[j];
-        tp.first = blocks;
         tp.T = team_T[j];
         tp.queue = team_q[j];
+        if (tp.T > 255)
+            return fail(TW_EINVAL, "teams of at most 255 workgroups");
         // hosts per wave item: as asked, else chosen per world by the kernel (about one work item
         // per wave of the team)
         tp.hpw = cfg.hosts_per_wave > 0 ? std::min(64, cfg.hosts_per_wave) : 0;
         blocks += tp.T;
-        team_of.insert(team_of.end(), tp.T, j);
         sh.teams |= tp.T > 1;
     }
+    place_teams(plan, team_of);
     if (sh.teams && blocks > resident)
         return fail(TW_EINVAL, "the plan's %d workgroups exceed the %d the GPU holds at once (a team's workgroups must all be resident)",
                     blocks, resident);
@@ -1492,7 +1533,7 @@ extern "C" int tw_launch_capacity(int chain_runs, int block_threads, int32_t *re
 {
     if (!resident_blocks)
         return fail(TW_EINVAL, "null output");
-    const int threads = block_threads > 0 ? block_threads : 256;
+    const int threads = block_threads > 0 ? block_threads : 512;
     if (threads % 64 || threads > 512)
         return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 512");
     int ndev = 0, dev = 0, cus = 0, per_cu = 0;
@@ -1622,7 +1663,7 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
         return fail(TW_EDEVICE, "no GPU: libtwins_core runs the model on the GPU only");
     std::unique_ptr<tw_server> s(new tw_server);
     s->runs = c.chain_runs == 25 ? 25 : 9;
-    s->threads = c.block_threads > 0 ? c.block_threads : 256;
+    s->threads = c.block_threads > 0 ? c.block_threads : 512;
     if (s->threads % 64 || s->threads > 512)
         return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 512");
     s->cap = c.max_requests > 0 ? c.max_requests : 65536;
@@ -1632,8 +1673,8 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
     const int resident = std::max(1, per_cu) * cus;
     // teams: large worlds on teams of 16 workgroups, mid-size on 4, small on one; a share of the
     // GPU stays free for the uploads' fill kernels
-    const int nl = c.large_teams > 0 ? c.large_teams : 4, nm = c.mid_teams > 0 ? c.mid_teams : 16;
-    const int headroom = cus / 4;
+    const int nl = c.large_teams > 0 ? c.large_teams : 4, nm = c.mid_teams > 0 ? c.mid_teams : 8;
+    const int headroom = cus / 8;
     const int ns = c.small_teams > 0 ? c.small_teams : resident - headroom - 16 * nl - 4 * nm;
     if (ns < 1 || ns + 4 * nm + 16 * nl > resident)
         return fail(TW_EINVAL, "server teams (%d + 4 x %d + 16 x %d workgroups) do not fit the %d the GPU holds", ns, nm, nl, resident);
@@ -1641,15 +1682,12 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
     std::vector<int32_t> team_of;
     auto add = [&](int n, int T, int q) {
         for (int i = 0; i < n; i++)
-        {
-            TeamPlan p{(int32_t)team_of.size(), T, 0, q};
-            team_of.insert(team_of.end(), T, (int32_t)plan.size());
-            plan.push_back(p);
-        }
+            plan.push_back(TeamPlan{0, T, 0, q});
     };
     add(ns, 1, 0);
     add(nm, 4, 1);
     add(nl, 16, 2);
+    place_teams(plan, team_of);
     s->blocks = (int)team_of.size();
     s->nteams = (int)plan.size();
     // the arena
