@@ -77,8 +77,8 @@ def sweep_leg(a):
     wargs = workloads.config4_worlds(wdir, a.sweep_worlds, seed=4)
     worlds = [twins_amd.World(x) for x in wargs]
     log(f"sweep: {len(worlds)} worlds generated and built in {time.time() - t0:.1f} s")
-    # plans of one 512-thread workgroup per CU (co-resident workgroups of other worlds slow each other)
-    tuner = autotune.Autotuner([w.info for w in worlds], twins_amd.launch_capacity(9, 512), block_threads=512)
+    # plans of one workgroup per CU (the kernel's 256-thread workgroups fill a CU's registers)
+    tuner = autotune.Autotuner([w.info for w in worlds], twins_amd.launch_capacity(9, 256), block_threads=256)
     for i in range(a.sweep_tune):
         plan = tuner.plan()
         t1 = time.time()

@@ -96,7 +96,7 @@ typedef struct tw_plan
 typedef struct tw_pack_cfg
 {
     int32_t chain_runs;      /* recorded event-order runs: 9 (`twins`) or 25 (`twins128`); 0: 9 */
-    int32_t block_threads;   /* threads per workgroup (a multiple of 64, <= 512; 0: 512, one workgroup per CU) */
+    int32_t block_threads;   /* threads per workgroup (a multiple of 64, <= 256; 0: 256, one workgroup per CU) */
     int32_t blocks;          /* persistent workgroups (a multiple of team_blocks) */
     int32_t team_blocks;     /* workgroups that run one world together (1: a world per workgroup) */
     int32_t hosts_per_wave;  /* hosts a wavefront runs side by side (1..64) */
@@ -105,7 +105,7 @@ typedef struct tw_pack_cfg
     const tw_plan *plan;     /* teams and queues (NULL: one queue, teams from the fields above) */
 } tw_pack_cfg;
 
-/* Workgroups of block_threads (0: 512) that the GPU holds at once: the most a plan with teams of
+/* Workgroups of block_threads (0: 256) that the GPU holds at once: the most a plan with teams of
  * several workgroups may use. */
 int tw_launch_capacity(int chain_runs, int block_threads, int32_t *resident_blocks);
 
@@ -162,7 +162,7 @@ typedef struct tw_server tw_server;
 typedef struct tw_server_cfg
 {
     int32_t chain_runs;       /* 9 or 25 (0: 9) */
-    int32_t block_threads;    /* 0: 512 */
+    int32_t block_threads;    /* 0: 256 */
     int32_t small_teams;      /* teams of one workgroup (0: what the GPU holds after the others and a free share) */
     int32_t mid_teams;        /* teams of 4 workgroups, for worlds of 513-4096 hosts and switches (0: 8) */
     int32_t large_teams;      /* teams of 16, for larger worlds (0: 4) */

@@ -112,12 +112,14 @@ static void run_cpu(BatchT<RUNS> &B, int64_t budget)
     }
 }
 
-hipError_t tcp_launch(int runs, const void *batch_dev, const LaunchShape &s, hipStream_t)
+hipError_t tcp_launch_9(const void *batch_dev, const LaunchShape &s, hipStream_t)
 {
-    if (runs == 25)
-        run_cpu(*(BatchT<25> *)batch_dev, s.slot_budget);
-    else
-        run_cpu(*(BatchT<9> *)batch_dev, s.slot_budget);
+    run_cpu(*(BatchT<9> *)batch_dev, s.slot_budget);
+    return hipSuccess;
+}
+hipError_t tcp_launch_25(const void *batch_dev, const LaunchShape &s, hipStream_t)
+{
+    run_cpu(*(BatchT<25> *)batch_dev, s.slot_budget);
     return hipSuccess;
 }
 hipError_t tcp_reset(uint32_t *queue_next, int nq, void *teams, size_t team_bytes, hipStream_t)
@@ -126,10 +128,16 @@ hipError_t tcp_reset(uint32_t *queue_next, int nq, void *teams, size_t team_byte
     std::fill((char *)teams, (char *)teams + team_bytes, (char)0);
     return hipSuccess;
 }
-hipError_t tcp_server_launch(int, const void *, int, int, hipStream_t) { return hipErrorNotSupported; }
-hipError_t tcp_max_blocks(int, int, int *per_cu)
+hipError_t tcp_server_launch_9(const void *, int, int, hipStream_t) { return hipErrorNotSupported; }
+hipError_t tcp_server_launch_25(const void *, int, int, hipStream_t) { return hipErrorNotSupported; }
+hipError_t tcp_max_blocks_9(int, int *per_cu)
 {
-    *per_cu = 4;
+    *per_cu = 1;
+    return hipSuccess;
+}
+hipError_t tcp_max_blocks_25(int, int *per_cu)
+{
+    *per_cu = 1;
     return hipSuccess;
 }
 } // namespace tw

@@ -340,7 +340,7 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
 }
 
 template <int RUNS>
-__global__ void __launch_bounds__(512) tcp_worlds_kernel(const BatchT<RUNS> *__restrict__ Bp, int64_t slot_budget)
+__global__ void __launch_bounds__(256) tcp_worlds_kernel(const BatchT<RUNS> *__restrict__ Bp, int64_t slot_budget)
 {
     world_loop<RUNS, false>(Bp, nullptr, slot_budget);
 }
@@ -353,24 +353,46 @@ __global__ void __launch_bounds__(512) tcp_worlds_kernel(const BatchT<RUNS> *__r
 // ServerCtl::idle_ticks, the host's shutdown flag ends every wait, and the host relaunches the
 // kernel when it finds it gone.
 template <int RUNS>
-__global__ void __launch_bounds__(512) tcp_server_kernel(const ServerCtl *__restrict__ Sp)
+__global__ void __launch_bounds__(256) tcp_server_kernel(const ServerCtl *__restrict__ Sp)
 {
     world_loop<RUNS, true>(nullptr, Sp, INT64_MAX);
 }
 
-template <int RUNS>
-static hipError_t launch(const void *batch_dev, const LaunchShape &s, hipStream_t stream)
+// This file is compiled once per chain length (TW_RUNS = 9 or 25, two objects built in parallel:
+// fully inlined, each takes minutes); tcp_kernel.h dispatches on the run count.
+#ifndef TW_RUNS
+#define TW_RUNS 9
+#endif
+#define TW_CAT(a, b) a##b
+#define TW_NAME(a, b) TW_CAT(a, b)
+
+hipError_t TW_NAME(tcp_launch_, TW_RUNS)(const void *batch_dev, const LaunchShape &s, hipStream_t stream)
 {
     // an ordinary launch: the grid never exceeds the workgroups the GPU holds at once (the plan
     // checks it), so a team's workgroups are resident together, and a team barrier that still
     // waited too long ends the world with an error. No cooperative launch: it takes a device-wide
     // resource that one process can hold while another waits for it.
     const size_t lds = (size_t)(s.threads / 64) * LDS_PER_WAVE * sizeof(int);
-    const BatchT<RUNS> *b = reinterpret_cast<const BatchT<RUNS> *>(batch_dev);
-    hipLaunchKernelGGL(tcp_worlds_kernel<RUNS>, dim3(s.blocks), dim3(s.threads), lds, stream, b, s.slot_budget);
+    const BatchT<TW_RUNS> *b = reinterpret_cast<const BatchT<TW_RUNS> *>(batch_dev);
+    hipLaunchKernelGGL(tcp_worlds_kernel<TW_RUNS>, dim3(s.blocks), dim3(s.threads), lds, stream, b, s.slot_budget);
     return hipGetLastError();
 }
 
+hipError_t TW_NAME(tcp_server_launch_, TW_RUNS)(const void *ctl_dev, int blocks, int threads, hipStream_t stream)
+{
+    const size_t lds = (size_t)(threads / 64) * LDS_PER_WAVE * sizeof(int);
+    const ServerCtl *c = reinterpret_cast<const ServerCtl *>(ctl_dev);
+    hipLaunchKernelGGL(tcp_server_kernel<TW_RUNS>, dim3(blocks), dim3(threads), lds, stream, c);
+    return hipGetLastError();
+}
+
+hipError_t TW_NAME(tcp_max_blocks_, TW_RUNS)(int threads, int *per_cu)
+{
+    const size_t lds = (size_t)(threads / 64) * LDS_PER_WAVE * sizeof(int);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, tcp_worlds_kernel<TW_RUNS>, threads, lds);
+}
+
+#if TW_RUNS == 9
 // clears the world-queue counters and the teams' barrier words before a launch (a kernel, not a
 // memset, so that the pair can be captured in a graph)
 __global__ void tcp_reset_kernel(uint32_t *__restrict__ a, int na, uint32_t *__restrict__ b, int nb)
@@ -392,28 +414,6 @@ hipError_t tcp_reset(uint32_t *queue_next, int nq, void *teams, size_t team_byte
     hipLaunchKernelGGL(tcp_reset_kernel, dim3(blocks), dim3(256), 0, stream, queue_next, nq, (uint32_t *)teams, nb);
     return hipGetLastError();
 }
-
-hipError_t tcp_server_launch(int runs, const void *ctl_dev, int blocks, int threads, hipStream_t stream)
-{
-    const size_t lds = (size_t)(threads / 64) * LDS_PER_WAVE * sizeof(int);
-    const ServerCtl *c = reinterpret_cast<const ServerCtl *>(ctl_dev);
-    if (runs == 25)
-        hipLaunchKernelGGL(tcp_server_kernel<25>, dim3(blocks), dim3(threads), lds, stream, c);
-    else
-        hipLaunchKernelGGL(tcp_server_kernel<9>, dim3(blocks), dim3(threads), lds, stream, c);
-    return hipGetLastError();
-}
-
-hipError_t tcp_launch(int runs, const void *batch_dev, const LaunchShape &shape, hipStream_t stream)
-{
-    return runs == 25 ? launch<25>(batch_dev, shape, stream) : launch<9>(batch_dev, shape, stream);
-}
-
-hipError_t tcp_max_blocks(int runs, int threads, int *per_cu)
-{
-    const size_t lds = (size_t)(threads / 64) * LDS_PER_WAVE * sizeof(int);
-    return runs == 25 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, tcp_worlds_kernel<25>, threads, lds)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, tcp_worlds_kernel<9>, threads, lds);
-}
+#endif
 
 } // namespace tw

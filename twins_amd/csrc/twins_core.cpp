@@ -1243,12 +1243,13 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     LaunchShape &sh = b.shape;
-    // workgroup size: 512 threads (a full CU) for sweeps and explicit plans, which put one workgroup
-    // per CU (co-resident workgroups of other worlds slow each other more than they hide each
-    // other's latency); 256 when a few large worlds are spread over teams (measured faster there)
-    sh.threads = cfg.block_threads > 0 ? cfg.block_threads : (cfg.plan || W >= cus ? 512 : 256);
-    if (sh.threads % 64 || sh.threads > 512)
-        return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 512");
+    // workgroup size: 256 threads (4 waves), the kernel's bound: it may then use 512 VGPRs and
+    // runs fully inlined without spills; the GPU holds one such workgroup per CU, so worlds do not
+    // share a CU (co-resident workgroups of other worlds slowed each other more than they hid each
+    // other's latency)
+    sh.threads = cfg.block_threads > 0 ? cfg.block_threads : 256;
+    if (sh.threads % 64 || sh.threads > 256)
+        return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 256");
     HIPCHK(tcp_max_blocks(RUNS, sh.threads, &per_cu));
     const int resident = std::max(1, per_cu) * cus;
     auto size_of = [&](int i) { return (int64_t)b.worlds[i]->H + b.worlds[i]->S; };
@@ -1533,9 +1534,9 @@ extern "C" int tw_launch_capacity(int chain_runs, int block_threads, int32_t *re
 {
     if (!resident_blocks)
         return fail(TW_EINVAL, "null output");
-    const int threads = block_threads > 0 ? block_threads : 512;
-    if (threads % 64 || threads > 512)
-        return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 512");
+    const int threads = block_threads > 0 ? block_threads : 256;
+    if (threads % 64 || threads > 256)
+        return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 256");
     int ndev = 0, dev = 0, cus = 0, per_cu = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(TW_EDEVICE, "no GPU: libtwins_core runs the model on the GPU only");
@@ -1663,9 +1664,9 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
         return fail(TW_EDEVICE, "no GPU: libtwins_core runs the model on the GPU only");
     std::unique_ptr<tw_server> s(new tw_server);
     s->runs = c.chain_runs == 25 ? 25 : 9;
-    s->threads = c.block_threads > 0 ? c.block_threads : 512;
-    if (s->threads % 64 || s->threads > 512)
-        return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 512");
+    s->threads = c.block_threads > 0 ? c.block_threads : 256;
+    if (s->threads % 64 || s->threads > 256)
+        return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 256");
     s->cap = c.max_requests > 0 ? c.max_requests : 65536;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
