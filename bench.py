@@ -54,6 +54,15 @@ def oracle_run(flows, args, tcp, out):
     return sec, ev, rows
 
 
+def traffic_bytes():
+    """The measured HBM bytes per launch (profiles/tcp_worlds_kernel_config2_traffic.json), or None."""
+    p = os.path.join(ROOT, "profiles", "tcp_worlds_kernel_config2_traffic.json")
+    try:
+        return json.load(open(p))["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -314,7 +323,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6),
-            "traffic": None,
+            "traffic": traffic_bytes(),
+            "traffic_source": "HBM bytes per launch of this kernel on this workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                              "runs, scaled by the same counters' readings for a 1 GiB copy (profiles/tcp_worlds_kernel_config2.md)",
             "kernel": "tcp_worlds_kernel",
             "kernel_ms_per_step": round(1000 * kern / a.steps, 3),
             "bytes_per_event": BYTES_PER_EVENT,
