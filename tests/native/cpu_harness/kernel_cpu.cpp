@@ -81,10 +81,28 @@ static void run_cpu(BatchT<RUNS> &B, int64_t budget)
                     x.scr_ref = B.scr_ref;
                     x.scr_up = B.scr_up;
                 }
+                L.wave = lds.data();
                 for (auto &x : lanes)
                     Md::sw_window(B, W, x, t, L);
                 if (W.rng0 >= 0)
-                    Md::spray_draws(B, W, R, sw, t, L, B.scr_up);
+                {
+                    // the lockstep merge, phase by phase over the lanes
+                    std::vector<typename Md::Spray> sp(k);
+                    for (int q = 0; q < k; q++)
+                        Md::spray_init(B, W, lanes[q], t, L, sp[q]);
+                    for (;;)
+                    {
+                        for (int q = 0; q < k; q++)
+                            Md::spray_put(lanes[q], L, sp[q]);
+                        bool done = true;
+                        for (int q = 0; q < k; q++)
+                            done &= Md::spray_pick(B, W, R, lanes[q], t, L, sp[q]);
+                        if (done)
+                            break;
+                    }
+                    for (int q = 0; q < k; q++)
+                        Md::spray_end(B, W, lanes[q], L, sp[q]);
+                }
                 for (auto &x : lanes)
                     Md::sw_count(B, W, R, x, t, L);
                 for (auto &x : lanes)

@@ -260,6 +260,7 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
                 L.in_win = wl + 64 * 33 + base * 3;
                 L.base = wl + 64 * 33 + 64 * 3 + base;
                 L.cnt = wl + 64 * 33 + 64 * 3 + 64 + base;
+                L.wave = wl;
 #ifdef TW_PROF
                 const unsigned long long q0 = wall_clock64();
 #endif

@@ -2031,6 +2031,7 @@ struct Model
     struct SwLds
     {
         int *rows, *in_win, *base, *cnt;
+        int *wave; // the wavefront's whole LDS region (33 ints per lane at the start)
     };
     struct In
     {
@@ -2215,67 +2216,180 @@ struct Model
     // one slot of a switch on the GPU: the lanes of one wavefront, phases separated by wave_sync()
     static __device__ int switch_step(const Batch &B, const WorldDesc &W, WorldRun *R, const SwLane &x, int64_t t, const SwLds &L)
     {
+#ifdef TW_PROF
+        // profiler builds: the phases' wall-clock ticks, summed over switch items (counters 20-23)
+        unsigned long long p0 = wall_clock64();
+#define TW_PHASE(i)                                                                                                         \
+    {                                                                                                                       \
+        unsigned long long p1 = wall_clock64();                                                                             \
+        if ((threadIdx.x & 63) == 0)                                                                                        \
+            atomicAdd(&R->ev[20 + (i)], p1 - p0);                                                                           \
+        p0 = p1;                                                                                                            \
+    }
+#else
+#define TW_PHASE(i)
+#endif
         sw_window(B, W, x, t, L);
         if (W.rng0 >= 0)
         {
             wave_sync();
-            if (x.active && x.q == 0)
-                spray_draws(B, W, R, x.swl, t, L, x.scr_up);
+            Spray sp;
+            spray_init(B, W, x, t, L, sp);
+            wave_sync();
+            for (;;)
+            {
+                spray_put(x, L, sp);
+                wave_sync();
+                const bool done = spray_pick(B, W, R, x, t, L, sp);
+                wave_sync();
+                if (done)
+                    break;
+            }
+            spray_end(B, W, x, L, sp);
         }
         wave_sync();
+        TW_PHASE(0);
         sw_count(B, W, R, x, t, L);
         wave_sync();
         sw_prefix(W, x, L);
         wave_sync();
         sw_base(x, L);
         wave_sync();
+        TW_PHASE(1);
         sw_scatter(B, W, x, t, L);
         wave_sync();
-        return sw_egress(B, W, R, x, t, L);
+        TW_PHASE(2);
+        const int r = sw_egress(B, W, R, x, t, L);
+        TW_PHASE(3);
+#undef TW_PHASE
+        return r;
     }
-    // per-packet ECMP: one GetInteger draw per forwarded packet, in the node's arrival order
-    // (all ports merged; src/step.cuh:2706-2745)
-    static __device__ __forceinline__ void spray_draws(const Batch &B, const WorldDesc &W, WorldRun *R, int swl, int64_t t, const SwLds &L,
-                                                   uint8_t *scr_up)
+    // Per-packet ECMP (src/step.cuh:2706-2745): one GetInteger draw per forwarded packet from the
+    // switch's stream, in the switch's arrival order over all ports: at each step the earliest head
+    // of the ports' arrival lists, equal times decided in port order by frame_before. Packets
+    // arriving at a down port draw nothing; a packet with no route draws nothing and gets 0.
+    // The k lanes of the switch run the merge in lockstep, each holding its own port's list: a
+    // step exchanges the heads through LDS, every lane finds the same winner and makes the same
+    // draw (each lane carries the stream), the winner records it. Heads and route counts of the
+    // first 32 arrivals of a port sit in LDS (computed by the port's lane beforehand); frames are
+    // read only to break equal times. The phases (init, then put / pick until done, then end) run
+    // with wave_sync() between them; a CPU check may run each phase over the lanes in turn.
+    static constexpr int SPRAY_CACHE = 32;
+    static constexpr uint32_t SPRAY_NONE = 0xFFFFFFFFu;
+    struct Spray
     {
+        In in;
+        int c, p;      // this port's arrivals, the next one's place
+        uint32_t head; // its arrival time - slot start (SPRAY_NONE: none left)
+        TwMrg g;       // the switch's stream (every lane the same)
+        bool done;
+    };
+    // the packed entry of arrival j: time offset (20 bits), route count or 255 (a down port)
+    static __device__ uint32_t spray_code(const Batch &B, const WorldDesc &W, const SwLane &x, int64_t S0, const Frame &f)
+    {
+        const int ep = W.n_fail ? epoch_of(B, W, f.ts) : 0;
+        const uint32_t nr = port_down(B, W, x.swl, x.q, ep) ? 255u : (uint32_t)routes_to(B, W, x.swl, f.dst, ep);
+        return (uint32_t)(f.ts - S0) | nr << 20;
+    }
+    // this port's lane in the wavefront (its switch's first lane, base, plus the port)
+    static __device__ int spray_lane(const SwLane &x, const SwLds &L) { return (int)(L.cnt - L.wave - (64 * 33 + 64 * 3 + 64)) + x.q; }
+    static __device__ uint32_t spray_entry(const Batch &B, const WorldDesc &W, const SwLane &x, const SwLds &L, const Spray &sp, int64_t S0, int j)
+    {
+        if (j < SPRAY_CACHE)
+            return (uint32_t)L.wave[spray_lane(x, L) * 33 + j];
+        return spray_code(B, W, x, S0, sp.in.at(j));
+    }
+    static __device__ void spray_init(const Batch &B, const WorldDesc &W, const SwLane &x, int64_t t, const SwLds &L, Spray &sp)
+    {
+        sp.c = 0;
+        sp.p = 0;
+        sp.head = SPRAY_NONE;
+        sp.done = !x.active;
+        if (!x.active)
+            return;
+        const int64_t S0 = t * SLOT_PS;
+        sp.in = lane_input(B, W, x, t, L, x.q);
+        sp.c = sp.in.count();
+        const int lane = spray_lane(x, L);
+        const int n = sp.c < SPRAY_CACHE ? sp.c : SPRAY_CACHE;
+        for (int j = 0; j < n; j++)
+            L.wave[lane * 33 + j] = (int)spray_code(B, W, x, S0, sp.in.at(j));
+        if (sp.c > 0)
+            sp.head = (uint32_t)L.wave[lane * 33]; // (the lane's own write)
+        const TwMrgState &gs = B.sw_rng[W.rng0 + x.swl];
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+            sp.g.s[i] = gs.s[i];
+    }
+    static __device__ void spray_put(const SwLane &x, const SwLds &L, Spray &sp)
+    {
+        if (!x.active)
+            return;
+        L.cnt[x.q] = (int)sp.head;
+        L.base[x.q] = sp.p;
+    }
+    // one step of the merge; true when the switch has no arrivals left
+    static __device__ bool spray_pick(const Batch &B, const WorldDesc &W, WorldRun *R, const SwLane &x, int64_t t, const SwLds &L, Spray &sp)
+    {
+        if (!x.active || sp.done)
+            return true;
         const int k = W.k;
-        int pos[32];
+        const int64_t S0 = t * SLOT_PS;
+        // the earliest head (the first port holding it)
+        int best = -1;
+        uint32_t bt = SPRAY_NONE;
         for (int q = 0; q < k; q++)
-            pos[q] = 0;
-        TwMrg g;
-        TwMrgState &gs = B.sw_rng[W.rng0 + swl];
-#pragma unroll
-        for (int i = 0; i < 6; i++)
-            g.s[i] = gs.s[i];
-        for (;;)
         {
-            int best = -1;
-            const Frame *bf = nullptr;
-            for (int q = 0; q < k; q++)
+            const uint32_t h = (uint32_t)L.cnt[q] & 0xFFFFFu;
+            if (L.cnt[q] != (int)SPRAY_NONE && (best < 0 || h < bt))
             {
-                In in = input_of(B, B.in_link[W.link0 + W.H + swl * k + q], t, L.in_win + 3 * q);
-                if (pos[q] >= in.count())
-                    continue;
-                const Frame &f = in.at(pos[q]);
-                if (best < 0 || f.ts < bf->ts || (f.ts == bf->ts && frame_before(R, f, *bf)))
-                {
-                    best = q;
-                    bf = &f;
-                }
+                best = q;
+                bt = h;
             }
-            if (best < 0)
-                break;
-            int ep = W.n_fail ? epoch_of(B, W, bf->ts) : 0;
-            if (!port_down(B, W, swl, best, ep))
-            {
-                int nr = routes_to(B, W, swl, bf->dst, ep);
-                scr_up[best * 2 * MAX_TX + pos[best]] = (uint8_t)(nr ? tw_mrg_integer(g, (uint32_t)nr) : 0);
-            }
-            pos[best]++;
         }
+        if (best < 0)
+        {
+            sp.done = true;
+            return true;
+        }
+        // equal times: port order, frame_before decides (the reference's scan over the ports)
+        for (int q = best + 1; q < k; q++)
+        {
+            if (L.cnt[q] == (int)SPRAY_NONE || ((uint32_t)L.cnt[q] & 0xFFFFFu) != bt)
+                continue;
+            const Frame &f = lane_input(B, W, x, t, L, q).at(L.base[q]);
+            const Frame &b = lane_input(B, W, x, t, L, best).at(L.base[best]);
+            if (frame_before(R, f, b))
+                best = q;
+        }
+        const uint32_t nr = (uint32_t)L.cnt[best] >> 20;
+        uint8_t draw = 0;
+        if (nr != 255u && nr)
+            draw = (uint8_t)tw_mrg_integer(sp.g, nr);
+        if (best == x.q)
+        {
+            if (nr != 255u)
+                x.scr_up[x.q * 2 * MAX_TX + sp.p] = draw;
+            sp.p++;
+            sp.head = sp.p < sp.c ? spray_entry(B, W, x, L, sp, S0, sp.p) : SPRAY_NONE;
+        }
+        return false;
+    }
+    static __device__ void spray_end(const Batch &B, const WorldDesc &W, const SwLane &x, const SwLds &L, Spray &sp)
+    {
+        if (!x.active)
+            return;
+        if (x.q == 0)
+        {
+            TwMrgState &gs = B.sw_rng[W.rng0 + x.swl];
 #pragma unroll
-        for (int i = 0; i < 6; i++)
-            gs.s[i] = g.s[i];
+            for (int i = 0; i < 6; i++)
+                gs.s[i] = sp.g.s[i];
+        }
+        // the count matrix row this lane's cache overlapped starts from zero again
+        int *row = L.rows + x.q * (W.k + 1);
+        for (int i = 0; i <= W.k; i++)
+            row[i] = 0;
     }
 };
 

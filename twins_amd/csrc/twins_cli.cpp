@@ -206,6 +206,9 @@ int main(int argc, char **argv)
     if (ev[27]) // a TW_PROF build: wall-clock ticks (100 MHz) summed over slots
         printf("prof: per slot: period %.2f us, slowest workgroup: hosts %.2f us, switches %.2f us, work %.2f us\n", ev[27] / 100.0 / slots,
                ev[28] / 100.0 / slots, ev[29] / 100.0 / slots, ev[30] / 100.0 / slots);
+    if (ev[27]) // switch items' phases, summed over items (not per slot maxima)
+        printf("prof: switch phases (item-us per slot): window+spray %.2f, count %.2f, scatter %.2f, egress %.2f\n",
+               ev[20] / 100.0 / slots, ev[21] / 100.0 / slots, ev[22] / 100.0 / slots, ev[23] / 100.0 / slots);
     if (ev[5] || ev[6] || ev[7] || ev[14] || ev[19])
         fprintf(stderr, "warning: a capacity overflowed (active/ooo/due/rtt/ring-ovf); the results may differ from ns-3\n");
     if (ev[25])
