@@ -212,6 +212,12 @@ def serve_leg(a):
     lat = sorted(finish[r] - arrival[r] for r in finish)
     pct = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))]
     log(f"serve: {n} requests in {wall:.1f} s, p50 {pct(0.5):.3f} s, p99 {pct(0.99):.3f} s")
+    by_k = {}
+    for r in finish:
+        by_k.setdefault(worlds[r].info.hosts, []).append(finish[r] - arrival[r])
+    for h, xs in sorted(by_k.items()):
+        xs.sort()
+        log(f"  {h} hosts: {len(xs)} requests, p50 {xs[len(xs) // 2]:.3f} s, max {xs[-1]:.3f} s")
     # every serve_check-th request against the oracle (all host cores)
     def check(rid):
         out = os.path.join(a.sweep_dir, f"s{rid:04d}.csv")
@@ -252,9 +258,9 @@ def main():
     ap.add_argument("--sweep_dir", default="/tmp/twins_sweep")
     ap.add_argument("--cpu_cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--serve", type=int, default=1, help="also run the config-5 serving loop (the \"serve\" object)")
-    ap.add_argument("--serve_requests", type=int, default=300)
+    ap.add_argument("--serve_requests", type=int, default=1000)
     ap.add_argument("--serve_rate", type=float, default=100.0)
-    ap.add_argument("--serve_check", type=int, default=10, help="every n-th request is checked against the oracle")
+    ap.add_argument("--serve_check", type=int, default=20, help="every n-th request is checked against the oracle")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU (the worlds of a sweep are independent and would be "

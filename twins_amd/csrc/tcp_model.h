@@ -254,19 +254,14 @@ struct Model
         Due *due;
         int nd, id;
         int64_t slot_end;
-        // the earliest sender-side (RTO, SYN) and receiver-side (delayed ACK, SYN-ACK, LAST_ACK,
-        // FIN-ACK) timer armed during this slot
-        int64_t arm_s, arm_r;
     };
 
-    // a timer of one of the host's connections was armed for time v
-    static __device__ __forceinline__ void arm_snd(Host &x, int64_t v) { x.arm_s = v < x.arm_s ? v : x.arm_s; }
-    static __device__ __forceinline__ void arm_rcv(Host &x, int64_t v) { x.arm_r = v < x.arm_r ? v : x.arm_r; }
-
-    // Active lists: a host's sending and receiving connections. A connection leaves its list when
-    // it is finished (sender: closed; receiver: the sender closed and no receiver timer pending);
-    // compact() drops those, and returns the earliest timer of the rest.
-    static __device__ __noinline__ int64_t compact_snd(const Batch &B, int h)
+    // Active lists: a host's sending and receiving connections. A connection leaves its list at
+    // the end of a slot in which it is finished (sender: closed; receiver: the sender closed and
+    // no receiver timer pending), as in the reference (src/step.cuh:2470-2510; with a full list
+    // the add is refused and counted, so the removal's timing decides results in that regime);
+    // compact() drops those and returns the earliest timer of the rest.
+    static __device__ __forceinline__ int64_t compact_snd(const Batch &B, int h)
     {
         int64_t next = INT64_MAX;
         int32_t *l = B.act_snd + (size_t)h * MAX_ACTIVE;
@@ -286,7 +281,7 @@ struct Model
         B.n_snd[h] = k;
         return next;
     }
-    static __device__ __noinline__ int64_t compact_rcv(const Batch &B, int h)
+    static __device__ __forceinline__ int64_t compact_rcv(const Batch &B, int h)
     {
         int64_t next = INT64_MAX;
         int32_t *l = B.act_rcv + (size_t)h * MAX_ACTIVE;
@@ -1011,7 +1006,6 @@ struct Model
         if (s.rto_at < 0)
         {
             s.rto_at = now + s.rto;
-            arm_snd(x, s.rto_at);
             timer_chain(x, B.rchain, f, s.rto);
         }
         s.cwnd_limited = s.cwnd < inflight(s) + MSS;
@@ -1071,7 +1065,6 @@ struct Model
             int64_t r = est + (v > CLOCK_GRAN_PS ? v : CLOCK_GRAN_PS);
             s.rto = r > MIN_RTO_PS ? r : MIN_RTO_PS;
             s.rto_at = now + s.rto;
-            arm_snd(x, s.rto_at);
             timer_chain(x, x.B->rchain, f, s.rto);
         }
         if (ack >= s.tail && !(s.fin & FIN_SENT))
@@ -1305,7 +1298,6 @@ struct Model
         if (r.srv_syn_at < 0)
         {
             r.srv_syn_at = now + rto;
-            arm_rcv(x, r.srv_syn_at);
             timer_chain(x, x.B->dchain, f, rto);
         }
     }
@@ -1461,7 +1453,6 @@ struct Model
             int64_t rto = est + (v > CLOCK_GRAN_PS ? v : CLOCK_GRAN_PS);
             rto = rto > MIN_RTO_PS ? rto : MIN_RTO_PS;
             r.r_frx_at = now + rto;
-            arm_rcv(x, r.r_frx_at);
             timer_chain(x, x.B->fchain, f, rto);
         }
     }
@@ -1478,7 +1469,6 @@ struct Model
         r.r_last_n = DATA_RETRIES;
         int64_t lr = last_rto(r);
         r.r_last_at = now + lr;
-        arm_rcv(x, r.r_last_at);
         timer_chain(x, x.B->dchain, f, lr);
     }
     static __device__ __forceinline__ void rcv_closed(RcvState &r)
@@ -1522,7 +1512,6 @@ struct Model
             if (r.delack_at < 0)
             {
                 r.delack_at = now + DELACK_PS;
-                arm_rcv(x, r.delack_at);
                 timer_chain(x, x.B->dchain, f, DELACK_PS);
             }
         }
@@ -1572,17 +1561,9 @@ struct Model
     }
 
     // -------------------------------------------------------------- application and dispatch
-    // a full list first drops its finished connections (lists are compacted lazily: see host_step)
     static __device__ void add_active(Host &x, bool snd, int f)
     {
         int32_t *n = snd ? x.B->n_snd : x.B->n_rcv;
-        if (n[x.h] >= MAX_ACTIVE)
-        {
-            if (snd)
-                arm_snd(x, compact_snd(*x.B, x.h));
-            else
-                arm_rcv(x, compact_rcv(*x.B, x.h));
-        }
         int k = n[x.h];
         if (k >= MAX_ACTIVE)
         {
@@ -1614,7 +1595,6 @@ struct Model
         if (s.cwnd > pipe && s.syn_at < 0)
         {
             s.syn_at = now + 1;
-            arm_snd(x, s.syn_at);
             Chain ch;
             chain_child(ch, 2, x.echain);
             chain_store(B.schain + (size_t)f * CW, ch);
@@ -1753,11 +1733,10 @@ struct Model
         x.cubic = B.host_cc[h] == CC_CUBIC;
         x.due = B.due + (size_t)h * MAX_DUE;
         x.slot_end = E0;
-        x.arm_s = x.arm_r = INT64_MAX;
         Due *due = x.due;
-        // timers due in this slot, by time. The lists are scanned only when a timer of theirs may
-        // fall in the slot: tmin_s / tmin_r bound every armed timer from below (exact after a
-        // scan, lowered when a timer is armed)
+        // timers due in this slot, by time. A list is scanned only when one of its timers falls in
+        // the slot: tmin_s / tmin_r are its earliest, exact since the end of the host's last slot
+        // (timers change only while the host runs)
         int nd = 0;
         const int64_t bound_s = B.tmin_s[h], bound_r = B.tmin_r[h];
         const bool scan_s = bound_s < E0, scan_r = bound_r < E0;
@@ -1846,7 +1825,6 @@ struct Model
                     cub_hystart_reset(s, now);
                 s.rto = CONN_TIMEOUT_PS;
                 s.syn_at = now + CONN_TIMEOUT_PS;
-                arm_snd(x, s.syn_at);
                 timer_chain(x, B.rchain, f, CONN_TIMEOUT_PS);
                 s.timing_time = now;
                 emit(x, f, F_SYN, 0, 0, SYN_FRAME, false, now);
@@ -1874,7 +1852,6 @@ struct Model
                         int tries = ++s.syn_tries; // SYN retransmission, backed off
                         s.rto = CONN_TIMEOUT_PS << tries;
                         s.syn_at = now + s.rto;
-                        arm_snd(x, s.syn_at);
                         timer_chain(x, B.rchain, f, s.rto);
                         emit(x, f, F_SYN, 0, 0, SYN_FRAME, false, now);
                     }
@@ -1899,7 +1876,6 @@ struct Model
                             send_fin_ack(x, f, now);
                             int64_t lr = last_rto(r);
                             r.r_last_at = now + lr;
-                            arm_rcv(x, r.r_last_at);
                             timer_chain(x, B.dchain, f, lr);
                         }
                     }
@@ -1927,10 +1903,8 @@ struct Model
         x.nic.advance(E0 - 1);
         x.nic.store(t, rt);
         B.pend_next[h] = pn;
-        // the timer bounds: rescanned (dropping finished connections) after a slot whose timers
-        // were due, else lowered by the timers armed in it; the next wake-up
-        const int64_t ts_ = scan_s ? compact_snd(B, h) : bound_s, tr_ = scan_r ? compact_rcv(B, h) : bound_r;
-        const int64_t nb_s = ts_ < x.arm_s ? ts_ : x.arm_s, nb_r = tr_ < x.arm_r ? tr_ : x.arm_r;
+        // drop finished connections from the active lists; their earliest timers; the next wake-up
+        const int64_t nb_s = compact_snd(B, h), nb_r = compact_rcv(B, h);
         B.tmin_s[h] = nb_s;
         B.tmin_r[h] = nb_r;
         int64_t next = nb_s < nb_r ? nb_s : nb_r;
