@@ -1,0 +1,77 @@
+/* twins_rdma.h -- the C ABI of the RDMA model (`twins-rdma`) in libtwins_core.so: hosts with RDMA
+ * NICs (queue pairs paced by rate and window, go-back-N with NACKs), switches with a shared-buffer
+ * MMU (PFC with dynamic thresholds and headroom, ECN marking at dequeue), congestion control DCQCN,
+ * HPCC or DCTCP over RDMA; flow completion times equal to the HPCC simulator's.
+ *
+ * The reference has no library interface: `twins-rdma` (src/rdma/rmain.cu) reads an HPCC config
+ * file, a topology file and a flow file (or a job file of such worlds), runs them and writes
+ * "id start_ns fct_ns" per finished flow. Each entry point below replaces one stage of that main;
+ * the `twins-rdma` binary built here calls them.
+ *
+ * Conventions as twins.h: 0 on success, a negative TW_E* code on error with a message from
+ * trd_last_error() (per thread); handles own their memory; caller buffers are copied. Times are ns.
+ */
+#ifndef TWINS_RDMA_H
+#define TWINS_RDMA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct trd_world trd_world;
+typedef struct trd_batch trd_batch;
+
+/* One world: an HPCC config file (`KEY value ...` lines, rdma-ref's), overrides `KEY=VALUE` (or
+ * KEY=V1,V2,... for a key with several values), a TwiNS topology file (`hosts H`, `switches S`,
+ * `link A B` with A, B as hN or sN) and a flow file (a header line, then `src,dst,bytes,start_ns`).
+ * Every link has rate_bps and delay_ns (0: 100 Gb/s, 1000 ns).
+ * Replaces build_world (src/rdma/rmain.cu:364-1021) with read_config / set_override (111-143). */
+int trd_world_create(const char *config_path, const char *const *overrides, int32_t n_overrides, const char *topo_path,
+                     const char *flow_path, uint64_t rate_bps, int64_t delay_ns, trd_world **out);
+/* its size: hosts, switches, flows; and maxRtt / maxBdp (the reference prints them) */
+int trd_world_info(const trd_world *w, int32_t *hosts, int32_t *switches, int32_t *flows, uint64_t *max_rtt_ns, uint32_t *max_bdp);
+void trd_world_destroy(trd_world *w);
+
+typedef struct trd_pack_cfg
+{
+    int32_t q3cap;       /* frames a switch port's data queue holds (0: 4096; src/rdma/rmain.cu:1037) */
+    int32_t tail_log2;   /* the shared ring of chain tails holds 2^tail_log2 nodes (0: 24) */
+    int32_t block_threads; /* lanes of a world's workgroup (0: 256) */
+    int32_t pad;
+} trd_pack_cfg;
+
+/* Worlds side by side on the GPU (the reference's --batch): one slot (the shortest link delay of
+ * all), each world stepping its own slots. Replaces the device set-up of main
+ * (src/rdma/rmain.cu:1292-1990). */
+int trd_batch_create(trd_world *const *worlds, int32_t n, const trd_pack_cfg *cfg, trd_batch **out);
+
+typedef struct trd_run_stats
+{
+    int64_t slots;        /* the most slots a world ran */
+    int64_t sim_ns;       /* the latest world's end: its last slot's end (ns) */
+    double wall_s;        /* the run's wall time */
+    double kernel_s;      /* device time of the launches (HIP events) */
+    uint32_t flows_done;
+    int32_t launches;
+} trd_run_stats;
+
+/* Runs every world to its end: all flows done, the stop time (SIMULATOR_STOP_TIME), or
+ * quiescence. Replaces main's launch loop (src/rdma/rmain.cu:2020-2170). */
+int trd_batch_run(trd_batch *b, trd_run_stats *st);
+
+/* a world's flows, by id: start and completion time minus start (-1: not finished) */
+int trd_fct_get(const trd_batch *b, int32_t world, int64_t *start_ns, int64_t *fct_ns);
+/* the counters of src/rdma/rmain.cu:2257-2258, by the reference's indices (40 entries) */
+int trd_counters_get(const trd_batch *b, uint64_t *ev40);
+/* a world's slots run and device time (its team's constant-clock time, s) */
+int trd_world_stats_get(const trd_batch *b, int32_t world, int64_t *slots, double *device_s);
+void trd_batch_destroy(trd_batch *b);
+
+const char *trd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,47 @@
+// TEST INFRASTRUCTURE: the RDMA slot kernel's launch replaced by the same steps run on the CPU,
+// one world after the other and one entity after the other in each slot, so that the unchanged
+// host code (rdma_core.cpp, twins_rdma_cli.cpp) and model (rdma_model.h) run as a CPU program
+// (with AddressSanitizer: bounds checks of the kernel code) whose results are compared with the
+// reference's RDMA fixtures. Never linked into the product.
+#include "../../../twins_amd/csrc/rdma_model.h"
+#include "../../../twins_amd/csrc/rdma_kernel.h"
+
+namespace rd
+{
+int launch_worlds(const Sim *sp, int nworlds, int, hipStream_t)
+{
+    const Sim &s = *sp;
+    for (int w = 0; w < nworlds; w++)
+    {
+        const World &wd = s.wd[w];
+        WorldRun &r = s.run[w];
+        uint32_t *pend = s.pend + wd.pend_off;
+        while (!r.finished)
+        {
+            unsigned act = 0;
+            unsigned long long wake = ~0ull;
+            for (int e = 0; e < wd.nh + wd.ns; e++)
+            {
+                if (e < wd.nh)
+                {
+                    int64_t hw = INT64_MAX;
+                    act |= host_step(s, wd.h0 + e, r.slot, &r.flows_done, pend, wd.pend_n, &hw) ? 1u : 0u;
+                    if (hw != INT64_MAX && (unsigned long long)hw < wake)
+                        wake = (unsigned long long)hw;
+                }
+                else
+                {
+                    const int x = wd.s0 + (e - wd.nh);
+                    bool busy = false;
+                    if (s.sw_busy[x] || switch_has_input(s, x, r.slot))
+                        busy = switch_step(s, x, r.slot, pend, wd.pend_n);
+                    s.sw_busy[x] = busy ? 1 : 0;
+                    act |= busy ? 1u : 0u;
+                }
+            }
+            slot_end(s, wd, r, pend, act, wake);
+        }
+    }
+    return 0;
+}
+} // namespace rd

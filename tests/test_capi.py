@@ -1,5 +1,6 @@
-"""The C ABI library: it loads, exports every entry point include/twins.h declares, builds worlds
-on the host, and refuses to simulate without a GPU (it never falls back to the CPU)."""
+"""The C ABI library: it loads, exports every entry point include/twins.h and include/twins_rdma.h
+declare, builds worlds on the host, and refuses to simulate without a GPU (it never falls back to
+the CPU)."""
 import ctypes
 import os
 import re
@@ -8,19 +9,22 @@ import pytest
 
 import util
 
-HDR = os.path.join(util.ROOT, "include", "twins.h")
+HDRS = [os.path.join(util.ROOT, "include", h) for h in ("twins.h", "twins_rdma.h")]
 
 
 def declared():
-    text = open(HDR).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(tw_\w+)\s*\(", text, re.M)))
+    names = set()
+    for h in HDRS:
+        text = open(h).read()
+        names |= set(re.findall(r"^\s*(?:int|void|const char \*)\s*(trd_\w+|tw_\w+)\s*\(", text, re.M))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_entry_point():
     import twins_amd
     L = twins_amd.lib()
     names = declared()
-    assert len(names) >= 10
+    assert len(names) >= 20 and any(n.startswith("trd_") for n in names)
     for n in names:
         assert hasattr(L, n), n
 

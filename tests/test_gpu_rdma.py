@@ -1,0 +1,46 @@
+"""The RDMA model on the GPU against the reference's RDMA jobs (tests/golden/rdma): every world's
+flow completions identical to the reference's, and the batch's counters equal to the sum of the
+jobs' counters. Worlds run side by side in one batch (the reference's --batch), through the C ABI."""
+import os
+import subprocess
+import time
+
+import pytest
+
+import rdma_util
+
+pytestmark = pytest.mark.gpu
+
+
+def run_batch(sel, label):
+    from twins_amd import rdma
+    I = rdma_util.INPUTS
+    print(f"rdma batch {label}: {len(sel)} worlds", flush=True)
+    worlds = [rdma.World(rdma_util.CONFIG, f"{I}/{t}", f"{I}/{f}", kv) for _, t, f, kv, _, _ in sel]
+    b = rdma.Batch(worlds)
+    t = time.time()
+    st = b.run()
+    print(f"  {time.time() - t:.1f} s wall, {st.kernel_s:.1f} s kernel, {st.launches} launches", flush=True)
+    bad = [n for i, (n, *_) in enumerate(sel) if b.fct_text(i) != rdma_util.ref_fct(n)]
+    assert not bad, f"worlds differing from the reference: {bad}"
+    assert rdma_util.sum_counters([b.summary_line()]) == rdma_util.sum_counters([j[5] for j in sel])
+
+
+def test_gpu_reference_jobs_short():
+    """The 80 jobs the reference's CPU build runs in up to 3 s, in one batch."""
+    run_batch(rdma_util.jobs(max_ref_s=3.0), "short")
+
+
+def test_gpu_cli_one_job(tmp_path):
+    """The `twins-rdma` command line on the GPU: the reference's output file and summary lines."""
+    import util
+    I = rdma_util.INPUTS
+    out = tmp_path / "x.fct"
+    r = subprocess.run([os.path.join(util.ROOT, "twins_amd", "bin", "twins-rdma"), rdma_util.CONFIG, f"--topo={I}/fattree8.txt",
+                        f"--flows={I}/k8_incast16.csv", f"--out={out}", "CC_MODE=3", "BUFFER_SIZE=2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert out.read_text() == rdma_util.ref_fct("k8_incast16_b2_cc3")
+    job = [j for j in rdma_util.jobs() if j[0] == "k8_incast16_b2_cc3"][0]
+    assert [l for l in r.stdout.splitlines() if l.startswith("drops ")] == [job[5]]
+    assert "flows done 160 of 160" in r.stdout

@@ -1,0 +1,79 @@
+"""The RDMA model's code on the CPU (tests/native/rdma_cpu: the slot kernel's steps run one entity
+after the other, with the unchanged host code and command line) against the reference's RDMA jobs:
+flow completions and counters identical, job by job. Also the command line's and the library's
+error behaviour, and that the library refuses to run without a GPU."""
+import os
+import subprocess
+
+import pytest
+
+import rdma_util
+import util
+
+
+@pytest.fixture(scope="module")
+def cpu_bin():
+    subprocess.run(["make", "-C", os.path.dirname(rdma_util.CPU_BIN)], check=True, capture_output=True)
+    return rdma_util.CPU_BIN
+
+
+def run_cli(binary, topo, flows, kv, out, timeout=300):
+    I = rdma_util.INPUTS
+    return subprocess.run([binary, rdma_util.CONFIG, f"--topo={I}/{topo}", f"--flows={I}/{flows}", f"--out={out}"] + kv,
+                          capture_output=True, text=True, timeout=timeout)
+
+
+FAST = rdma_util.jobs(max_ref_s=1.0)
+
+
+@pytest.mark.parametrize("job", FAST, ids=[j[0] for j in FAST])
+def test_cpu_build_matches_reference_job(cpu_bin, job, tmp_path):
+    name, topo, flows, kv, _, counters = job
+    out = tmp_path / (name + ".fct")
+    r = run_cli(cpu_bin, topo, flows, kv, out)
+    assert r.returncode == 0, r.stderr
+    assert out.read_text() == rdma_util.ref_fct(name)
+    assert [l for l in r.stdout.splitlines() if l.startswith("drops ")] == [counters]
+
+
+def test_cpu_build_batch_equals_single_runs(cpu_bin, tmp_path):
+    """A job file of several worlds (the reference's --batch) gives each world's single-run results."""
+    sel = FAST[:6]
+    jf = tmp_path / "jobs.txt"
+    jf.write_text("".join(f"{n} {t} {f} {' '.join(kv)}\n" for n, t, f, kv, _, _ in sel))
+    r = subprocess.run([cpu_bin, rdma_util.CONFIG, f"--batch={jf}", f"--outdir={tmp_path}", f"--path={rdma_util.INPUTS}"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    for n, *_ in sel:
+        assert (tmp_path / (n + ".fct")).read_text() == rdma_util.ref_fct(n), n
+    got = [l for l in r.stdout.splitlines() if l.startswith("drops ")]
+    assert rdma_util.sum_counters(got) == rdma_util.sum_counters([j[5] for j in sel])
+
+
+@pytest.mark.parametrize("kv,msg", [(["CC_MODE=2"], "only 1 (DCQCN), 3 (HPCC) and 8 (DCTCP)"), (["CC_MODE=3", "MULTI_RATE=1"], "MULTI_RATE 1"),
+                                    (["L2_ACK_INTERVAL=0"], "L2_ACK_INTERVAL 0"), (["--bogus=1"], "unknown argument")])
+def test_cli_errors(cpu_bin, kv, msg, tmp_path):
+    r = run_cli(cpu_bin, "ft4.topo", "inc1.csv", kv, tmp_path / "x.fct")
+    assert r.returncode == 1 and msg in r.stderr
+
+
+def test_cli_missing_file(cpu_bin, tmp_path):
+    r = run_cli(cpu_bin, "nonexistent.topo", "inc1.csv", [], tmp_path / "x.fct")
+    assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+def test_world_build_on_host_and_no_cpu_fallback():
+    import twins_amd
+    from twins_amd import rdma
+    I = rdma_util.INPUTS
+    w = rdma.World(rdma_util.CONFIG, f"{I}/ft4.topo", f"{I}/inc1.csv", ["CC_MODE=1"])
+    assert (w.hosts, w.switches, w.flows) == (16, 20, 15)
+    # maxRtt: 6 hops of 1 us there and back plus 6 MTU frames (1000 B at 100 Gb/s, 80 ns each);
+    # maxBdp: its bytes at 100 Gb/s
+    assert w.max_rtt == 2 * 6 * 1000 + 6 * 80 and w.max_bdp == w.max_rtt * 100 // 8
+    with pytest.raises(twins_amd.TwError, match="CC_MODE 5"):
+        rdma.World(rdma_util.CONFIG, f"{I}/ft4.topo", f"{I}/inc1.csv", ["CC_MODE=5"])
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    with pytest.raises(twins_amd.TwError, match="no GPU"):
+        rdma.Batch([w])

@@ -1,0 +1,1940 @@
+// TwiNS-RDMA's packet model (the HPCC simulator's RDMA network: QbbNetDevice, RdmaHw with DCQCN,
+// HPCC and DCTCP, SwitchNode and SwitchMmu with PFC and ECN), restated for MI355X. Every function
+// here is a __host__ __device__ step of one entity in one slot; the slot kernel (rdma_kernel.hip)
+// deals the entities of a world over the lanes of its workgroup, and a CPU build of the same code
+// (tests/native/rdma_cpu) runs them one after the other. Reference: src/rdma/rdma.h and
+// src/rdma/rstep.cuh; file:line citations name the function each step follows.
+//
+// The execution model is the reference's (src/rdma/rdma.h:12-16): slots of the shortest link delay D;
+// a frame started in slot s arrives in slot s + 1 or later, filed by its link under its arrival slot,
+// so within one slot every host and switch runs its own events (by time, simultaneous events by
+// their event chains) from what earlier slots filed.
+#pragma once
+#include <cstdint>
+#include <climits>
+#include <cmath>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define RD_HD __host__ __device__ __forceinline__
+#else
+#define RD_HD inline
+#endif
+
+namespace rd
+{
+// ------------------------------------------------------------------ constants (src/rdma/rdma.h:101-106, 119-121)
+constexpr int RUNS = 6;                 // chain runs kept inline
+constexpr int NANC = 4;                 // ancestor lead hashes a chain carries (the memo's depths)
+constexpr int ORD_BITS = 5;
+constexpr uint32_t ORD_MAX = (1u << ORD_BITS) - 1;
+constexpr uint64_t LEAD_MAX = (1ull << (32 - ORD_BITS)) - 2;
+constexpr int MAXF = 256;               // frames one link delivers in one slot, at most
+constexpr int MAX_HOP = 5;              // IntHeader::maxHop
+constexpr int PG = 3;                   // the data priority group
+constexpr uint32_t RESERVE = 4 * 1024, RESUME_OFFSET = 3 * 1024;
+constexpr uint32_t PFC_SIZE = 9 + 20 + 14;
+constexpr int TXT_N = 9216;             // frame sizes with a tabulated tx time
+constexpr int ACKQ_CAP = 1024;          // a NIC's ACK queue
+constexpr int MAX_DUE = 64;             // DCQCN timers one host runs in one slot
+constexpr int NEV = 40;                 // counters
+
+// counters (the reference's ev[] indices, printed in its summary line, src/rdma/rmain.cu:2257-2258)
+enum : int
+{
+    EV_DROP = 0,
+    EV_MARK = 1,
+    EV_PAUSE = 2,
+    EV_RESUME = 3,
+    EV_TX_OVF = 4,
+    EV_QUEUE_OVF = 5,
+    EV_ACKQ_OVF = 6,
+    EV_QPS_OVF = 7,
+    EV_TIE = 8,
+    EV_LOST = 9,
+    EV_ACK_DONE = 10,
+    EV_NACK = 11,
+    EV_DUE_OVF = 12,
+    EV_DATA_DONE = 13,
+    EV_LATE = 32,
+};
+
+enum : uint8_t
+{
+    K_DATA = 0,
+    K_ACK,
+    K_NACK,
+    K_PAUSE,
+    K_RESUME
+};
+
+// ------------------------------------------------------------------ records
+// An event chain: run-length encoded leads back to the flow's StartApplication (src/rdma/rdma.h:108-131).
+struct Run
+{
+    uint32_t code, cnt;
+};
+struct Chain
+{
+    Run r[RUNS];
+    uint64_t tail; // older runs: a TailNode's ring position + 1 (0: none)
+    uint64_t hash; // of the leads pushed since the root's StartApplication
+    uint64_t anc[NANC];
+    uint32_t root; // 1 + the flow's index in start order (0: none)
+    uint32_t pad;
+};
+struct TailNode
+{
+    uint32_t code, cnt;
+    uint64_t next, pos;
+};
+
+// A frame's contents; INT hops are IntHop words (lineRate:3 time:24 bytes:20 qlen:17 from the low bits).
+struct Pkt
+{
+    uint64_t hop[MAX_HOP];
+    uint32_t flow;
+    uint32_t seq;
+    uint16_t size;
+    uint16_t nhop;
+    uint8_t kind, ecn, cnp, q, inport, ord;
+    uint16_t sport, src, dst;
+};
+static_assert(sizeof(Pkt) == 64, "Pkt");
+
+struct LFrame
+{
+    int64_t ts; // arrival time
+    Chain c;    // the chain of the event that started its transmission
+    Pkt p;
+};
+
+struct Params
+{
+    int cc; // CC_MODE: 1 DCQCN, 3 HPCC, 8 DCTCP
+    int int_size;
+    uint32_t mtu, ack_interval, chunk;
+    int64_t nack_interval;
+    int var_win, rate_bound, ack_high_prio, ecn;
+    double g, rate_first_cnp;
+    int clamp;
+    int64_t alpha_int, dec_int, rp_int;
+    uint32_t rpg_threshold;
+    uint64_t rai, rhai, min_rate, dctcp_rai;
+    double u_target;
+    uint32_t mi_thresh;
+    int fast_react, sample_feedback;
+    uint32_t int_multi;
+    uint32_t buffer_size;
+};
+
+struct FlowInfo
+{
+    int src, dst; // global host ids
+    uint64_t size;
+    int64_t start;
+    uint32_t sport, win;
+    uint64_t base_rtt;
+    Chain start_chain;
+};
+
+// a queue pair, sender side (RdmaQueuePair; src/rdma/rdma.h:174-229)
+struct SndQP
+{
+    uint64_t size, snd_nxt, snd_una;
+    uint64_t rate, max_rate;
+    int64_t next_avail;
+    uint32_t last_pkt, win;
+    uint64_t base_rtt;
+    uint8_t done, alpha_cnp, dec_cnp, first_cnp;
+    uint32_t rp_stage;
+    double alpha;
+    uint64_t target;
+    int64_t t_alpha, t_dec, t_rp; // pending DCQCN timers (-1: none)
+    uint32_t last_update_seq;
+    int32_t inc_stage;
+    uint64_t cur_rate;              // HPCC
+    uint32_t high_seq, ecn_cnt;     // DCTCP
+    double u;                       // HPCC
+    uint32_t batch, ca_state;       // DCTCP
+    uint64_t hop[MAX_HOP];
+    uint64_t tx_rate;               // the last tx time computed (qp_tx)
+    uint32_t tx_size;
+    int64_t tx_val;
+};
+struct QPChains
+{
+    Chain alpha, dec, rp;
+};
+struct RcvQP
+{
+    uint32_t expected, last_nack;
+    int64_t nack_timer;
+};
+
+struct Route
+{
+    uint32_t off, n;
+};
+
+// a NIC (QbbNetDevice + RdmaEgressQueue state) and the host's flow starts
+struct Host
+{
+    int64_t tc_at, ns_at; // pending TransmitComplete, m_nextSend (-1: none)
+    uint32_t paused;
+    int ackq_head, ackq_n;
+    int qp_n, rr;
+    int next_start, start_end;
+    int64_t wake;    // its earliest event without input
+    int64_t ob_last; // its out link's last bucket: slot << 16 | frames
+    int64_t due_min; // at most its queue pairs' earliest DCQCN timer
+};
+
+struct Port
+{
+    int64_t tc_at;
+    uint32_t paused;
+    int q0_head, q0_n, q3_head, q3_n;
+    uint32_t bytes_total;
+    uint64_t tx_bytes;
+    uint32_t ing, hdrm, egr, mmu_paused; // SwitchMmu, this port as ingress and as egress
+    int64_t ob_last;
+    double rng[6]; // its ECN marking stream (MRG32k3a)
+};
+
+struct Switch
+{
+    uint32_t shared_used, total_hdrm, total_rsrv, pad;
+};
+
+// a switch port's static set-up: its links, rate, headroom cap, PFC shift, ECN thresholds
+struct PortCfg
+{
+    uint64_t bps;
+    uint32_t hdrm_cap, shift, kmin, kmax;
+    double pmax;
+    int32_t link_in, link_out;
+};
+
+struct SwInfo
+{
+    uint32_t seed;  // its node id in its world (the ECMP hash seed)
+    int hbase;      // its world's first host
+    uint32_t rt_off;
+    int pbase, deg; // its ports
+};
+
+// a directed link: its frames filed by arrival slot in nb buckets of cap frames
+struct Link
+{
+    int64_t delay;
+    uint64_t bps;
+    uint64_t moff;
+    uint32_t coff, nb, cap, rc;
+};
+
+// a sender's place in its out link's ring (src/rdma/rstep.cuh:45-66)
+struct OutBucket
+{
+    int64_t lo, hi; // its slot's times (hi 0: none open)
+    uint64_t base;
+    uint32_t cnt, n;
+    int link;
+    int64_t last;
+};
+
+// a switch port's cursor over its slot's arrivals and its out bucket (per port, in HBM)
+struct PortRun
+{
+    int64_t ats; // next arrival's time (INT64_MAX: none)
+    uint32_t acode;
+    int ai, an;
+    int nout;
+    uint64_t abase;
+    OutBucket ob;
+};
+
+struct Due
+{
+    int64_t at;
+    int flow, kind;
+};
+
+// a world: its entities and flows (global ids), its pending-slot ring, its stop
+struct World
+{
+    int h0, nh, s0, ns, f0, nf;
+    int pend_off, pend_n;
+    int64_t max_slots;
+};
+// a world's run state
+struct WorldRun
+{
+    int64_t slot, slots_run;
+    uint64_t next_wake;
+    uint32_t flows_done, act;
+    int32_t finished, error;
+    int64_t ticks;
+};
+
+struct Sim
+{
+    int H, S, F, W, L, R;
+    int64_t D; // the slot (ns)
+    const Params *Pw;
+    const World *wd;
+    WorldRun *run;
+    const int *h_world, *h_link_in, *h_link_out;
+    const SwInfo *si;
+    const int *s_world;
+    const PortCfg *pc;
+    const Route *rt;
+    const uint8_t *rtp;
+    const FlowInfo *fi;
+    const uint32_t *root_rank;
+    const int *start_list;
+    SndQP *qp;
+    QPChains *qc;
+    RcvQP *rx;
+    Host *hs;
+    Chain *h_tc, *h_ns;
+    Pkt *ackq;
+    int *qps;
+    int qps_cap;
+    Due *due; // [H][MAX_DUE] a host's timers of the slot
+    Switch *sw;
+    int *sw_busy; // [S] it ended its last slot busy (it runs the next one)
+    Port *ps;
+    Chain *p_tc;
+    PortRun *pr;
+    Pkt *q0, *q3;
+    int q0_cap, q3_cap;
+    const Link *li;
+    LFrame *msg;
+    int64_t *msg_cnt;
+    const int32_t *txt; // [R][TXT_N]
+    uint32_t *pend;
+    TailNode *tail;
+    unsigned long long *tail_cnt;
+    uint64_t tail_mask;
+    unsigned long long *memo;
+    uint64_t memo_mask;
+    int64_t *fct;
+    unsigned long long *ev;
+};
+
+// ------------------------------------------------------------------ atomics (device) / plain (CPU build)
+RD_HD unsigned long long at_add64(unsigned long long *p, unsigned long long v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return atomicAdd(p, v);
+#else
+    unsigned long long o = *p;
+    *p = o + v;
+    return o;
+#endif
+}
+RD_HD unsigned at_add32(unsigned *p, unsigned v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return atomicAdd(p, v);
+#else
+    unsigned o = *p;
+    *p = o + v;
+    return o;
+#endif
+}
+RD_HD unsigned long long at_cas64(unsigned long long *p, unsigned long long cmp, unsigned long long v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return atomicCAS(p, cmp, v);
+#else
+    unsigned long long o = *p;
+    if (o == cmp)
+        *p = v;
+    return o;
+#endif
+}
+RD_HD void count(const Sim &s, int i) { at_add64(&s.ev[i], 1ull); }
+RD_HD uint64_t mulhi64(uint64_t a, uint64_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// ------------------------------------------------------------------ ns-3.17 time (src/rdma/rstep.cuh:9-29)
+// Seconds(x) in ns: int64x64_t(x) = floor(x) . floor(frac(x) 2^64), times 10^9, high word
+RD_HD int64_t ns_seconds(double x)
+{
+    double hi = floor(x);
+    uint64_t lo = (uint64_t)((x - hi) * 18446744073709551616.0);
+    return (int64_t)hi * 1000000000ll + (int64_t)mulhi64(lo, 1000000000ull);
+}
+// DataRate(bps).CalculateTxTime(bytes) in ns
+RD_HD int64_t tx_ns(uint64_t bps, uint32_t bytes) { return ns_seconds((double)bytes * 8 / (double)bps); }
+RD_HD int64_t link_tx(const Sim &s, int link, uint32_t bytes)
+{
+    const Link &l = s.li[link];
+    if (bytes < (uint32_t)TXT_N)
+        return s.txt[(size_t)l.rc * TXT_N + bytes];
+    return tx_ns(l.bps, bytes);
+}
+RD_HD uint64_t rate_of(double d) { return (uint64_t)d; }
+RD_HD int ring_add(int a, int b, int n)
+{
+    int x = a + b;
+    return x >= n ? x - n : x;
+}
+
+// ------------------------------------------------------------------ link rings (src/rdma/rstep.cuh:67-133; rdma.h:635-642)
+// frames of `link` arriving in `slot`: their count, first one at *base
+RD_HD int bucket(const Sim &s, int link, int64_t slot, uint64_t *base)
+{
+    const Link &l = s.li[link];
+    int b = (int)(slot % (int64_t)l.nb);
+    int64_t v = s.msg_cnt[l.coff + b];
+    *base = l.moff + (uint64_t)b * l.cap;
+    return (v >> 16) == slot ? (int)(v & 0xFFFF) : 0;
+}
+RD_HD void ob_open(OutBucket &o, int link, int64_t last)
+{
+    o.hi = 0;
+    o.link = link;
+    o.last = last;
+}
+RD_HD void ob_flush(const Sim &s, OutBucket &o)
+{
+    if (o.hi == 0)
+        return;
+    o.last = (o.lo / s.D) << 16 | o.n;
+    s.msg_cnt[o.cnt] = o.last;
+    o.hi = 0;
+}
+// a place in the bucket of the frame's arrival slot (null: full, counted)
+RD_HD LFrame *ob_push(const Sim &s, uint32_t *pend, int pend_n, int64_t ts, OutBucket &o)
+{
+    const Link &l = s.li[o.link];
+    if (ts >= o.hi)
+    {
+        ob_flush(s, o);
+        int64_t a = ts / s.D;
+        int b = (int)(a % (int64_t)l.nb);
+        o.cnt = l.coff + b;
+        o.lo = a * s.D;
+        o.hi = o.lo + s.D;
+        o.base = l.moff + (uint64_t)b * l.cap;
+        o.n = (o.last >> 16) == a ? (uint32_t)(o.last & 0xFFFF) : 0;
+        pend[a % pend_n] = 1;
+    }
+    if (o.n >= l.cap)
+    {
+        count(s, EV_TX_OVF);
+        return nullptr;
+    }
+    return s.msg + o.base + o.n++;
+}
+
+// ------------------------------------------------------------------ event order (src/rdma/rstep.cuh:155-660)
+RD_HD uint32_t ccode(int64_t lead, int ord)
+{
+    uint64_t l = (uint64_t)lead + 1;
+    if (l > LEAD_MAX)
+        l = LEAD_MAX;
+    return (uint32_t)(l << ORD_BITS | (ORD_MAX - (ord < (int)ORD_MAX ? (uint32_t)ord : ORD_MAX)));
+}
+// splitmix64's finalizer over the lead (ordinals do not enter the hash)
+RD_HD uint64_t chash(uint64_t h, uint32_t code)
+{
+    uint64_t z = h + 0x9e3779b97f4a7c15ull * ((code >> ORD_BITS) + 1ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+// the chain of an event scheduled with `code` by the event of chain par; the oldest run spills to
+// the shared tail ring when the inline runs are full
+RD_HD void cpush(const Sim &s, Chain *dst, uint32_t code, const Chain &par)
+{
+    Chain c;
+    if (par.r[0].code == code)
+    {
+        c = par;
+        if (c.r[0].cnt < 0xFFFFFFFFu)
+            c.r[0].cnt++;
+    }
+    else
+    {
+        c.r[0].code = code;
+        c.r[0].cnt = 1;
+        for (int i = 1; i < RUNS; i++)
+            c.r[i] = par.r[i - 1];
+        c.root = par.root;
+        c.tail = par.tail;
+        c.pad = 0;
+        if (par.r[RUNS - 1].code)
+        {
+            uint64_t pos = at_add64(s.tail_cnt, 1ull) + 1;
+            TailNode &n = s.tail[(pos - 1) & s.tail_mask];
+            n.code = par.r[RUNS - 1].code;
+            n.cnt = par.r[RUNS - 1].cnt;
+            n.next = par.tail;
+            n.pos = pos;
+            c.tail = pos;
+        }
+    }
+    c.hash = chash(par.hash, code);
+    c.anc[0] = par.hash;
+    for (int i = 1; i < NANC; i++)
+        c.anc[i] = par.anc[i - 1];
+    *dst = c;
+}
+
+// the runs of a chain (or of the chain cpush(code0, ch) would make), newest first
+struct Cursor
+{
+    const Chain *c;
+    int i;         // inline run, RUNS + 1: in the tail
+    uint64_t node; // the current tail node's position + 1
+    uint64_t nxt;
+    uint32_t code; // 0: the end
+    uint32_t left; // repetitions left of the current run
+    bool lost;
+};
+RD_HD void cur_load(const Sim &s, Cursor &k)
+{
+    if (k.i < RUNS)
+    {
+        k.code = k.c->r[k.i].code;
+        k.left = k.c->r[k.i].cnt;
+        if (k.code || k.i < RUNS - 1)
+            return;
+        k.i = RUNS;
+    }
+    if (k.i == RUNS && k.node == 0)
+    {
+        k.node = k.c->tail;
+        k.i = RUNS + 1;
+    }
+    if (k.node == 0)
+    {
+        k.code = 0;
+        return;
+    }
+    const TailNode n = s.tail[(k.node - 1) & s.tail_mask];
+    if (n.pos != k.node)
+    {
+        k.lost = true;
+        k.code = 0;
+        return;
+    }
+    k.code = n.code;
+    k.left = n.cnt;
+    k.nxt = n.next;
+}
+RD_HD void cur_init(const Sim &s, Cursor &k, const Chain &ch, uint32_t code0)
+{
+    k.c = &ch;
+    k.i = 0;
+    k.node = 0;
+    k.nxt = 0;
+    k.lost = false;
+    if (code0)
+    {
+        k.code = code0;
+        if (ch.r[0].code == code0)
+            k.left = ch.r[0].cnt + 1; // merged with the first run, consumed with it
+        else
+        {
+            k.left = 1;
+            k.i = -1;
+        }
+        return;
+    }
+    cur_load(s, k);
+}
+RD_HD void cur_next(const Sim &s, Cursor &k)
+{
+    if (k.i < RUNS)
+    {
+        k.i++;
+        if (k.i == RUNS)
+        {
+            k.node = k.c->tail;
+            k.i = RUNS + 1;
+        }
+    }
+    else
+        k.node = k.nxt;
+    cur_load(s, k);
+}
+
+// the memo of comparisons decided by leads, keyed by the two lead hashes (src/rdma/rstep.cuh:374-405)
+RD_HD uint64_t memo_key(uint64_t ha, uint64_t hb)
+{
+    uint64_t k = chash(ha * 0x9e3779b97f4a7c15ull ^ hb, 0x5bd1e995u) & ~3ull;
+    return k ? k : 4;
+}
+RD_HD int memo_get(const Sim &s, uint64_t ha, uint64_t hb)
+{
+    uint64_t k = memo_key(ha, hb);
+    for (int i = 0; i < 4; i++)
+    {
+        unsigned long long v = s.memo[(k + i) & s.memo_mask];
+        if (v == 0)
+            return 0;
+        if ((v & ~3ull) == k)
+            return (v & 3) == 1 ? -1 : 1;
+    }
+    return 0;
+}
+RD_HD void memo_put(const Sim &s, uint64_t ha, uint64_t hb, int r)
+{
+    uint64_t k = memo_key(ha, hb);
+    unsigned long long v = k | (r < 0 ? 1 : 2);
+    for (int i = 0; i < 4; i++)
+    {
+        unsigned long long old = at_cas64(&s.memo[(k + i) & s.memo_mask], 0ull, v);
+        if (old == 0 || (old & ~3ull) == k)
+            return;
+    }
+}
+
+// < 0: a first, > 0: b first, 0: undecided (src/rdma/rstep.cuh:406-503): the first differing lead
+// decides (larger: an earlier ancestor); a chain ending first runs first; with equal leads the
+// deepest differing ordinal decides; distinct roots decide by start order. *by_lead: decided by a
+// lead or an end (a function of the leads alone).
+RD_HD int ccmp_walk(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b, bool *inexact, bool *by_lead)
+{
+    *inexact = false;
+    *by_lead = false;
+    int ord = 0;
+    Cursor A, B;
+    cur_init(s, A, a, pa);
+    cur_init(s, B, b, pb);
+    for (;;)
+    {
+        uint32_t ca = A.code, cb = B.code;
+        if (ca == 0 || cb == 0)
+        {
+            *inexact = (ca == 0 && A.lost) || (cb == 0 && B.lost);
+            if (ca == 0 && cb == 0)
+            {
+                if (a.root && b.root && a.root != b.root)
+                    return a.root < b.root ? -1 : 1;
+                return ord;
+            }
+            if (ca == 0)
+            {
+                if (A.lost)
+                    return ord;
+                *by_lead = true;
+                return -1;
+            }
+            if (B.lost)
+                return ord;
+            *by_lead = true;
+            return 1;
+        }
+        if (ca >> ORD_BITS != cb >> ORD_BITS)
+        {
+            *by_lead = true;
+            return ca > cb ? -1 : 1;
+        }
+        if (ca != cb)
+            ord = (ca & ORD_MAX) > (cb & ORD_MAX) ? -1 : 1;
+        // the same place in the same tail node: one lineage from here on
+        if (A.i > RUNS && B.i > RUNS && A.node == B.node && A.left == B.left)
+            return ord;
+        uint32_t m = A.left < B.left ? A.left : B.left;
+        A.left -= m;
+        B.left -= m;
+        if (A.left == 0)
+            cur_next(s, A);
+        if (B.left == 0)
+            cur_next(s, B);
+    }
+}
+
+// ccmp_walk with the root rule for lockstep lineages and the memo (src/rdma/rstep.cuh:504-575)
+RD_HD int ccmp(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b, bool *inexact)
+{
+    uint64_t ha[NANC + 1], hb[NANC + 1];
+    ha[0] = pa ? chash(a.hash, pa) : a.hash;
+    hb[0] = pb ? chash(b.hash, pb) : b.hash;
+    for (int i = 1; i <= NANC; i++)
+    {
+        ha[i] = pa ? (i == 1 ? a.hash : a.anc[i - 2]) : a.anc[i - 1];
+        hb[i] = pb ? (i == 1 ? b.hash : b.anc[i - 2]) : b.anc[i - 1];
+    }
+    if (a.root && b.root && a.root != b.root && ha[0] == hb[0])
+    {
+        *inexact = false;
+        return s.root_rank[a.root - 1] < s.root_rank[b.root - 1] ? -1 : 1;
+    }
+    Cursor A, B;
+    cur_init(s, A, a, pa);
+    cur_init(s, B, b, pb);
+    int eq = 0;
+    for (int j = 0;; j++)
+    {
+        int m = memo_get(s, ha[j], hb[j]);
+        if (m)
+        {
+            *inexact = false;
+            return m;
+        }
+        if (j == NANC || A.code == 0 || B.code == 0 || (A.code >> ORD_BITS) != (B.code >> ORD_BITS))
+            break;
+        eq = j + 1;
+        if (--A.left == 0)
+            cur_next(s, A);
+        if (--B.left == 0)
+            cur_next(s, B);
+    }
+    bool by_lead;
+    int r = ccmp_walk(s, pa, a, pb, b, inexact, &by_lead);
+    if (by_lead && !*inexact)
+        for (int j = 0; j <= eq; j++)
+            memo_put(s, ha[j], hb[j], r);
+    return r;
+}
+
+// the first PRE runs of cpush(code0, ch) (src/rdma/rstep.cuh:577-600), and their order by leads
+// (src/rdma/rstep.cuh:602-624): -1, 1, or 0 when the prefixes do not decide
+constexpr int PRE = 3;
+RD_HD void cprefix(Run *out, uint32_t code0, const Chain &ch)
+{
+    if (!code0)
+    {
+        for (int i = 0; i < PRE; i++)
+            out[i] = ch.r[i];
+    }
+    else if (ch.r[0].code == code0)
+    {
+        out[0].code = code0;
+        out[0].cnt = ch.r[0].cnt + 1;
+        for (int i = 1; i < PRE; i++)
+            out[i] = ch.r[i];
+    }
+    else
+    {
+        out[0].code = code0;
+        out[0].cnt = 1;
+        for (int i = 1; i < PRE; i++)
+            out[i] = ch.r[i - 1];
+    }
+}
+RD_HD int pcmp(const Run *a, const Run *b)
+{
+    int ia = 0, ib = 0;
+    uint32_t ra = a[0].cnt, rb = b[0].cnt;
+    while (ia < PRE && ib < PRE)
+    {
+        uint32_t ca = a[ia].code, cb = b[ib].code;
+        if (!ca || !cb)
+            return 0;
+        if (ca >> ORD_BITS != cb >> ORD_BITS)
+            return ca > cb ? -1 : 1;
+        uint32_t m = ra < rb ? ra : rb;
+        ra -= m;
+        rb -= m;
+        if (ra == 0 && ++ia < PRE)
+            ra = a[ia].cnt;
+        if (rb == 0 && ++ib < PRE)
+            rb = b[ib].cnt;
+    }
+    return 0;
+}
+// does (ta, cpush(pa, a)) run before (tb, cpush(pb, b))? Undecided ties keep b (src/rdma/rstep.cuh:635-657)
+RD_HD bool before(const Sim &s, int64_t ta, uint32_t pa, const Chain &a, int64_t tb, uint32_t pb, const Chain &b)
+{
+    if (ta != tb)
+        return ta < tb;
+    bool inexact;
+    int c = ccmp(s, pa, a, pb, b, &inexact);
+    if (inexact)
+        count(s, EV_LOST);
+    if (c == 0)
+        count(s, EV_TIE);
+    return c < 0;
+}
+// a tie first by the prefixes (and the root rule), then by before(): the quick test of
+// src/rdma/rstep.cuh:626-633 that the reference applies to arrivals, TransmitCompletes and sends
+RD_HD bool before_q(const Sim &s, int64_t t, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b)
+{
+    Run xa[PRE], xb[PRE];
+    cprefix(xa, pa, a);
+    cprefix(xb, pb, b);
+    const uint64_t ha = pa ? chash(a.hash, pa) : a.hash, hb = pb ? chash(b.hash, pb) : b.hash;
+    int r;
+    if (a.root && b.root && a.root != b.root && ha == hb)
+        r = s.root_rank[a.root - 1] < s.root_rank[b.root - 1] ? -1 : 1;
+    else
+        r = pcmp(xa, xb);
+    if (r)
+        return r < 0;
+    return before(s, t, pa, a, t, pb, b);
+}
+
+// ------------------------------------------------------------------ frames (src/rdma/rstep.cuh:855-913)
+RD_HD uint32_t node_ip(int id) { return 0x0b000001u + (id / 256) * 0x00010000u + (id % 256) * 0x00000100u; }
+RD_HD uint32_t data_hdr(const Params &P) { return 48 + P.int_size; }
+RD_HD uint32_t ack_size(const Params &P)
+{
+    int base = 12 + P.int_size, pad = 60 - 14 - 20 - base;
+    return (pad > 0 ? pad : 0) + base + 20 + 14;
+}
+RD_HD uint64_t hop_time(uint64_t h) { return (h >> 3) & 0xFFFFFF; }
+RD_HD uint64_t hop_bytes(uint64_t h) { return (h >> 27) & 0xFFFFF; }
+RD_HD uint32_t hop_qlen(uint64_t h) { return (uint32_t)(h >> 47); }
+RD_HD uint64_t hop_rate(uint64_t h)
+{
+    switch (h & 7)
+    {
+    case 0: return 25000000000ull;
+    case 1: return 50000000000ull;
+    case 2: return 100000000000ull;
+    case 3: return 200000000000ull;
+    case 4: return 400000000000ull;
+    default: return 0;
+    }
+}
+RD_HD uint64_t hop_make(uint64_t time, uint64_t bytes, uint32_t qlen, uint64_t rate, uint32_t multi)
+{
+    uint64_t lr = rate == 25000000000ull ? 0 : rate == 50000000000ull ? 1 : rate == 100000000000ull ? 2 : rate == 200000000000ull ? 3 : 4;
+    uint64_t b = multi == 1 ? bytes >> 7 : bytes / (128ull * multi);
+    uint64_t q = multi == 1 ? qlen / 80u : qlen / (80u * multi);
+    return lr | (time & 0xFFFFFF) << 3 | (b & 0xFFFFF) << 27 | (q & 0x1FFFF) << 47;
+}
+RD_HD uint64_t hop_bytes_delta(uint64_t a, uint64_t b, uint32_t multi)
+{
+    uint64_t x = hop_bytes(a), y = hop_bytes(b);
+    return (x >= y ? x - y : x + (1u << 20) - y) * 128 * multi;
+}
+RD_HD uint64_t hop_time_delta(uint64_t a, uint64_t b)
+{
+    uint64_t x = hop_time(a), y = hop_time(b);
+    return x >= y ? x - y : x + (1u << 24) - y;
+}
+// SwitchNode::EcmpHash: MurmurHash3 x86_32 of three 32-bit words
+RD_HD uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+RD_HD uint32_t ecmp_hash(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t seed)
+{
+    const uint32_t w[3] = {w0, w1, w2};
+    uint32_t h = seed;
+    for (int i = 0; i < 3; i++)
+    {
+        uint32_t k = w[i] * 0xcc9e2d51u;
+        k = rotl32(k, 15) * 0x1b873593u;
+        h = rotl32(h ^ k, 13) * 5u + 0xe6546b64u;
+    }
+    h ^= 12u;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    return h ^ (h >> 16);
+}
+
+// MRG32k3a (RngStream::RandU01): every intermediate an integer below 2^53, so contraction-safe
+RD_HD double rand_u01(double *x)
+{
+    const double m1 = 4294967087.0, m2 = 4294944443.0, norm = 1.0 / (4294967087.0 + 1.0);
+    double p1 = 1403580.0 * x[1] - 810728.0 * x[0];
+    int32_t k = (int32_t)(p1 / m1);
+    p1 -= k * m1;
+    if (p1 < 0.0)
+        p1 += m1;
+    x[0] = x[1];
+    x[1] = x[2];
+    x[2] = p1;
+    double p2 = 527612.0 * x[5] - 1370589.0 * x[3];
+    k = (int32_t)(p2 / m2);
+    p2 -= k * m2;
+    if (p2 < 0.0)
+        p2 += m2;
+    x[3] = x[4];
+    x[4] = x[5];
+    x[5] = p2;
+    return (p1 > p2) ? (p1 - p2) * norm : (p1 - p2 + m1) * norm;
+}
+
+// ------------------------------------------------------------------ host: the NIC (src/rdma/rstep.cuh:986-1647)
+struct HCtx
+{
+    const Sim *s;
+    const Params *P;
+    uint32_t *pend;
+    int pend_n;
+    int h;
+    int64_t now, slot;
+    Chain cur; // the running event's chain
+    int ord;   // events it has scheduled
+    int nout;  // frames sent this slot
+    Host st;
+    int link;  // out link
+    int64_t ldelay;
+    OutBucket ob;
+};
+
+RD_HD uint64_t bytes_left(const SndQP &q) { return q.size >= q.snd_nxt ? q.size - q.snd_nxt : 0; }
+RD_HD int64_t qp_tx(SndQP &q, uint64_t rate, uint32_t bytes)
+{
+    if (rate != q.tx_rate || bytes != q.tx_size)
+    {
+        q.tx_rate = rate;
+        q.tx_size = bytes;
+        q.tx_val = tx_ns(rate, bytes);
+    }
+    return q.tx_val;
+}
+RD_HD bool win_bound(const Params &P, const SndQP &q)
+{
+    if (q.win == 0)
+        return false;
+    uint64_t w = q.win;
+    if (P.var_win)
+    {
+        w = (uint64_t)q.win * q.rate / q.max_rate; // (exact: q.win * rate stays below 2^64)
+        if (w == 0)
+            w = 1;
+    }
+    return q.snd_nxt - q.snd_una >= w;
+}
+
+// QbbNetDevice::TransmitStart (src/rdma/rstep.cuh:1056-1095)
+RD_HD void h_transmit(HCtx &c, Pkt p)
+{
+    const Sim &s = *c.s;
+    int64_t tx = link_tx(s, c.link, p.size);
+    c.st.tc_at = c.now + tx;
+    cpush(s, &s.h_tc[c.h], ccode(tx, c.ord++), c.cur);
+    p.ord = (uint8_t)(c.ord < 255 ? c.ord : 255);
+    c.ord++;
+    c.nout++;
+    int64_t ts = c.now + tx + c.ldelay;
+    LFrame *f = ob_push(s, c.pend, c.pend_n, ts, c.ob);
+    if (!f)
+        return;
+    f->ts = ts;
+    f->c = c.cur;
+    f->p = p;
+}
+RD_HD void h_schedule_ns(HCtx &c, int64_t at)
+{
+    c.st.ns_at = at;
+    cpush(*c.s, &c.s->h_ns[c.h], ccode(at - c.now, c.ord++), c.cur);
+}
+// RdmaEgressQueue::GetNextQindex: -1 the ACK queue, -1024 none, else the QP's place in the group;
+// finished queue pairs leave the group here (src/rdma/rstep.cuh:1106-1144)
+RD_HD int next_qindex(HCtx &c)
+{
+    const Sim &s = *c.s;
+    int ackq = c.P->ack_high_prio ? 0 : PG;
+    if (!(c.st.paused >> ackq & 1) && c.st.ackq_n > 0)
+        return -1;
+    int *list = s.qps + (size_t)c.h * s.qps_cap;
+    int res = -1024, fcount = c.st.qp_n, min_finish = INT_MAX;
+    int idx = fcount ? c.st.rr % fcount : 0;
+    for (int qi = 1; qi <= fcount; qi++)
+    {
+        idx = idx + 1 == fcount ? 0 : idx + 1;
+        const SndQP &q = s.qp[list[idx]];
+        if (!(c.st.paused >> PG & 1) && bytes_left(q) > 0 && !win_bound(*c.P, q))
+        {
+            if (q.next_avail > c.now)
+                continue;
+            res = idx;
+            break;
+        }
+        else if (q.snd_una >= q.size)
+            min_finish = idx < min_finish ? idx : min_finish;
+    }
+    if (min_finish < INT_MAX)
+    {
+        int nxt = min_finish;
+        for (int i = min_finish + 1; i < fcount; i++)
+            if (!(s.qp[list[i]].snd_una >= s.qp[list[i]].size))
+            {
+                if (i == res)
+                    res = nxt;
+                list[nxt++] = list[i];
+            }
+        c.st.qp_n = nxt;
+    }
+    return res;
+}
+// QbbNetDevice::DequeueAndTransmit on a host (src/rdma/rstep.cuh:1146-1196)
+RD_HD void h_dequeue(HCtx &c)
+{
+    const Sim &s = *c.s;
+    if (c.st.tc_at >= 0)
+        return;
+    int idx = next_qindex(c);
+    if (idx == -1)
+    {
+        Pkt p = s.ackq[(size_t)c.h * ACKQ_CAP + c.st.ackq_head];
+        c.st.ackq_head = ring_add(c.st.ackq_head, 1, ACKQ_CAP);
+        c.st.ackq_n--;
+        h_transmit(c, p);
+        return;
+    }
+    int *list = s.qps + (size_t)c.h * s.qps_cap;
+    if (idx >= 0)
+    {
+        int f = list[idx];
+        c.st.rr = idx;
+        SndQP &q = s.qp[f];
+        // RdmaHw::GetNxtPacket
+        uint64_t left = bytes_left(q);
+        uint32_t payload = left < c.P->mtu ? (uint32_t)left : c.P->mtu;
+        Pkt p = {};
+        p.kind = K_DATA;
+        p.flow = (uint32_t)f;
+        p.src = (uint16_t)s.fi[f].src;
+        p.dst = (uint16_t)s.fi[f].dst;
+        p.sport = (uint16_t)s.fi[f].sport;
+        p.seq = (uint32_t)q.snd_nxt;
+        p.size = (uint16_t)(payload + data_hdr(*c.P));
+        q.snd_nxt += payload;
+        h_transmit(c, p);
+        // RdmaHw::PktSent, UpdateNextAvail
+        q.last_pkt = p.size;
+        q.next_avail = c.now + qp_tx(q, c.P->rate_bound ? q.rate : q.max_rate, p.size);
+        return;
+    }
+    int64_t t = INT64_MAX;
+    for (int i = 0; i < c.st.qp_n; i++)
+    {
+        const SndQP &q = s.qp[list[i]];
+        if (bytes_left(q) == 0)
+            continue;
+        t = q.next_avail < t ? q.next_avail : t;
+    }
+    if (c.st.ns_at < 0 && t < INT64_MAX && t > c.now)
+        h_schedule_ns(c, t);
+}
+// RdmaHw::ChangeRate (src/rdma/rstep.cuh:1198-1207)
+RD_HD void change_rate(HCtx &c, SndQP &q, uint64_t new_rate)
+{
+    int64_t st = qp_tx(q, q.rate, q.last_pkt), nt = qp_tx(q, new_rate, q.last_pkt);
+    q.next_avail = q.next_avail + nt - st;
+    if (c.st.ns_at >= 0 && q.next_avail < c.st.ns_at)
+        h_schedule_ns(c, q.next_avail < c.now ? c.now : q.next_avail);
+    q.rate = new_rate;
+}
+// RdmaHw::UpdateRateHp with one rate for the path (src/rdma/rstep.cuh:1235-1312)
+RD_HD void hpcc_update(HCtx &c, SndQP &q, const Pkt &p, bool fast)
+{
+    const Params &P = *c.P;
+    uint32_t next_seq = (uint32_t)q.snd_nxt;
+    if (q.last_update_seq == 0)
+    {
+        q.last_update_seq = next_seq;
+        for (int i = 0; i < p.nhop && i < MAX_HOP; i++)
+            q.hop[i] = p.hop[i];
+        return;
+    }
+    if (p.nhop <= MAX_HOP)
+    {
+        double U = 0;
+        uint64_t dt = 0;
+        bool any = false;
+        for (int i = 0; i < p.nhop; i++)
+        {
+            if (P.sample_feedback && hop_qlen(p.hop[i]) == 0 && fast)
+                continue;
+            any = true;
+            uint64_t tau = hop_time_delta(p.hop[i], q.hop[i]);
+            double duration = (double)tau * 1e-9;
+            double tx_rate = (double)(hop_bytes_delta(p.hop[i], q.hop[i], P.int_multi) * 8) / duration;
+            uint32_t qa = hop_qlen(p.hop[i]) * 80 * P.int_multi, qb = hop_qlen(q.hop[i]) * 80 * P.int_multi;
+            double lr = (double)hop_rate(p.hop[i]);
+            double u = tx_rate / lr + (double)(qa < qb ? qa : qb) * (double)q.max_rate / lr / (double)q.win;
+            if (u > U)
+            {
+                U = u;
+                dt = tau;
+            }
+            q.hop[i] = p.hop[i];
+        }
+        if (any)
+        {
+            uint64_t new_rate;
+            int32_t new_inc;
+            if (dt > q.base_rtt)
+                dt = q.base_rtt;
+            // the fork's build contracts u (B - dt) + U dt into one fma
+            q.u = fma(q.u, (double)(q.base_rtt - dt), U * (double)dt) / (double)q.base_rtt;
+            double max_c = q.u / P.u_target;
+            if (max_c >= 1 || (uint32_t)q.inc_stage >= P.mi_thresh)
+            {
+                new_rate = rate_of((double)q.cur_rate / max_c) + P.rai;
+                new_inc = 0;
+            }
+            else
+            {
+                new_rate = q.cur_rate + P.rai;
+                new_inc = q.inc_stage + 1;
+            }
+            if (new_rate < P.min_rate)
+                new_rate = P.min_rate;
+            if (new_rate > q.max_rate)
+                new_rate = q.max_rate;
+            change_rate(c, q, new_rate);
+            if (!fast)
+            {
+                q.cur_rate = new_rate;
+                q.inc_stage = new_inc;
+            }
+        }
+    }
+    if (!fast && next_seq > q.last_update_seq)
+        q.last_update_seq = next_seq;
+}
+RD_HD void due_note(HCtx &c, int64_t t)
+{
+    if (t < c.st.due_min)
+        c.st.due_min = t;
+}
+// RdmaHw::cnp_received_mlx (src/rdma/rstep.cuh:1324-1345)
+RD_HD void dcqcn_cnp(HCtx &c, int f, SndQP &q)
+{
+    const Sim &s = *c.s;
+    q.alpha_cnp = 1;
+    q.dec_cnp = 1;
+    if (q.first_cnp)
+    {
+        q.alpha = 1;
+        q.alpha_cnp = 0;
+        q.t_alpha = c.now + c.P->alpha_int;
+        due_note(c, q.t_alpha);
+        cpush(s, &s.qc[f].alpha, ccode(c.P->alpha_int, c.ord++), c.cur);
+        q.t_dec = c.now + c.P->dec_int + 1;
+        due_note(c, q.t_dec);
+        cpush(s, &s.qc[f].dec, ccode(c.P->dec_int + 1, c.ord++), c.cur);
+        q.target = q.rate = rate_of((double)q.rate * c.P->rate_first_cnp);
+        q.first_cnp = 0;
+    }
+}
+// RdmaHw::HandleAckDctcp (src/rdma/rstep.cuh:1347-1385)
+RD_HD void dctcp_ack(HCtx &c, SndQP &q, const Pkt &p)
+{
+    const Params &P = *c.P;
+    bool new_batch = false;
+    q.ecn_cnt += p.cnp ? 1 : 0;
+    if (p.seq > q.last_update_seq)
+    {
+        new_batch = true;
+        if (q.last_update_seq == 0)
+        {
+            q.last_update_seq = (uint32_t)q.snd_nxt;
+            q.batch = (uint32_t)(q.snd_nxt / P.mtu + 1);
+        }
+        else
+        {
+            const double frac = (double)q.ecn_cnt / (double)q.batch;
+            q.alpha = fma(1 - P.g, q.alpha, P.g * (frac < 1.0 ? frac : 1.0));
+            q.last_update_seq = (uint32_t)q.snd_nxt;
+            q.ecn_cnt = 0;
+            q.batch = (uint32_t)((q.snd_nxt - (uint64_t)p.seq) / P.mtu + 1);
+        }
+    }
+    if (q.ca_state == 1 && p.seq > q.high_seq)
+        q.ca_state = 0;
+    if (p.cnp && q.ca_state == 0)
+    {
+        const uint64_t r = rate_of((double)q.rate * (1 - q.alpha / 2));
+        q.rate = r > P.min_rate ? r : P.min_rate;
+        q.ca_state = 1;
+        q.high_seq = (uint32_t)q.snd_nxt;
+    }
+    if (q.ca_state == 0 && new_batch)
+    {
+        const uint64_t r = q.rate + P.dctcp_rai;
+        q.rate = r < q.max_rate ? r : q.max_rate;
+    }
+}
+// the DCQCN timers: 0 UpdateAlphaMlx, 1 CheckRateDecreaseMlx, 2 RateIncEventTimerMlx
+// (src/rdma/rstep.cuh:1387-1437)
+RD_HD void dcqcn_timer(HCtx &c, int f, int kind)
+{
+    const Sim &s = *c.s;
+    const Params &P = *c.P;
+    SndQP &q = s.qp[f];
+    if (kind == 0)
+    {
+        if (q.alpha_cnp)
+            q.alpha = fma(1 - P.g, q.alpha, P.g);
+        else
+            q.alpha = (1 - P.g) * q.alpha;
+        q.alpha_cnp = 0;
+        q.t_alpha = c.now + P.alpha_int;
+        due_note(c, q.t_alpha);
+        cpush(s, &s.qc[f].alpha, ccode(P.alpha_int, c.ord++), c.cur);
+    }
+    else if (kind == 1)
+    {
+        q.t_dec = c.now + P.dec_int;
+        due_note(c, q.t_dec);
+        cpush(s, &s.qc[f].dec, ccode(P.dec_int, c.ord++), c.cur);
+        if (q.dec_cnp)
+        {
+            if (P.clamp || q.rp_stage != 0)
+                q.target = q.rate;
+            uint64_t r = rate_of((double)q.rate * (1 - q.alpha / 2));
+            q.rate = r > P.min_rate ? r : P.min_rate;
+            q.rp_stage = 0;
+            q.dec_cnp = 0;
+            q.t_rp = c.now + P.rp_int;
+            due_note(c, q.t_rp);
+            cpush(s, &s.qc[f].rp, ccode(P.rp_int, c.ord++), c.cur);
+        }
+    }
+    else
+    {
+        q.t_rp = c.now + P.rp_int;
+        due_note(c, q.t_rp);
+        cpush(s, &s.qc[f].rp, ccode(P.rp_int, c.ord++), c.cur);
+        if (q.rp_stage >= P.rpg_threshold)
+        {
+            q.target += q.rp_stage == P.rpg_threshold ? P.rai : P.rhai;
+            if (q.target > q.max_rate)
+                q.target = q.max_rate;
+        }
+        q.rate = rate_of((double)q.rate / 2) + rate_of((double)q.target / 2);
+        q.rp_stage++;
+    }
+}
+// RdmaHw::ReceiveAck, ACK and NACK (src/rdma/rstep.cuh:1440-1488)
+RD_HD void h_ack(HCtx &c, const Pkt &p, unsigned *flows_done)
+{
+    const Sim &s = *c.s;
+    int f = (int)p.flow;
+    SndQP &q = s.qp[f];
+    if (q.done)
+    {
+        count(s, EV_ACK_DONE);
+        return;
+    }
+    if (p.seq > q.snd_una)
+        q.snd_una = p.seq;
+    if (q.snd_una >= q.size)
+    {
+        // QpComplete
+        q.t_alpha = q.t_dec = q.t_rp = -1;
+        q.done = 1;
+        s.fct[f] = c.now - s.fi[f].start;
+        at_add32(flows_done, 1u);
+    }
+    if (p.kind == K_NACK)
+        q.snd_nxt = q.snd_una;
+    if (p.cnp && c.P->cc == 1 && !q.done)
+        dcqcn_cnp(c, f, q);
+    if (c.P->cc == 3)
+    {
+        if (p.seq > q.last_update_seq)
+            hpcc_update(c, q, p, false);
+        else if (c.P->fast_react)
+            hpcc_update(c, q, p, true);
+    }
+    if (c.P->cc == 8 && !q.done)
+        dctcp_ack(c, q, p);
+    h_dequeue(c);
+}
+// RdmaHw::ReceiveUdp and ReceiverCheckSeq (src/rdma/rstep.cuh:1490-1546)
+RD_HD void h_data(HCtx &c, const Pkt &p)
+{
+    const Sim &s = *c.s;
+    RcvQP &r = s.rx[p.flow];
+    uint32_t payload = p.size - data_hdr(*c.P);
+    if (r.expected >= s.fi[p.flow].size)
+        count(s, EV_DATA_DONE);
+    int x;
+    uint32_t expected = r.expected;
+    if (p.seq == expected)
+    {
+        r.expected = expected + payload;
+        if (r.expected >= c.P->ack_interval)
+            x = 1;
+        else if (c.P->chunk && r.expected % c.P->chunk == 0)
+            x = 1;
+        else
+            x = 5;
+    }
+    else if (p.seq > expected)
+    {
+        if (c.now >= r.nack_timer || r.last_nack != expected)
+        {
+            r.nack_timer = c.now + c.P->nack_interval;
+            r.last_nack = expected;
+            x = 2;
+        }
+        else
+            x = 4;
+    }
+    else
+        x = 3;
+    if (x != 1 && x != 2)
+        return;
+    if (x == 2)
+        count(s, EV_NACK);
+    Pkt a = p;
+    a.kind = x == 1 ? K_ACK : K_NACK;
+    a.seq = r.expected;
+    a.cnp = p.ecn ? 1 : 0;
+    a.ecn = 0;
+    a.size = (uint16_t)ack_size(*c.P);
+    if (c.st.ackq_n >= ACKQ_CAP)
+    {
+        count(s, EV_ACKQ_OVF);
+        return;
+    }
+    s.ackq[(size_t)c.h * ACKQ_CAP + ring_add(c.st.ackq_head, c.st.ackq_n, ACKQ_CAP)] = a;
+    c.st.ackq_n++;
+    h_dequeue(c);
+}
+// RdmaClient::StartApplication -> RdmaHw::AddQueuePair (src/rdma/rstep.cuh:1548-1587)
+RD_HD void h_start(HCtx &c, int f)
+{
+    const Sim &s = *c.s;
+    const FlowInfo &fi = s.fi[f];
+    SndQP &q = s.qp[f];
+    q = SndQP{};
+    q.size = fi.size;
+    const uint64_t nic = s.li[c.link].bps;
+    q.rate = q.max_rate = nic;
+    q.win = fi.win;
+    q.base_rtt = fi.base_rtt;
+    q.alpha = 1;
+    q.first_cnp = 1;
+    q.target = nic;
+    q.t_alpha = q.t_dec = q.t_rp = -1;
+    if (c.P->cc == 3)
+    {
+        q.cur_rate = nic;
+        q.u = 1;
+    }
+    q.next_avail = c.now;
+    if (c.st.qp_n >= s.qps_cap)
+    {
+        count(s, EV_QPS_OVF);
+        return;
+    }
+    s.qps[(size_t)c.h * s.qps_cap + c.st.qp_n++] = f;
+    h_dequeue(c);
+}
+
+RD_HD const Chain &due_chain(const Sim &s, const Due &d)
+{
+    const QPChains &qc = s.qc[d.flow];
+    return d.kind == 0 ? qc.alpha : d.kind == 1 ? qc.dec : qc.rp;
+}
+RD_HD int64_t due_now(const Sim &s, const Due &d)
+{
+    const SndQP &q = s.qp[d.flow];
+    return d.kind == 0 ? q.t_alpha : d.kind == 1 ? q.t_dec : q.t_rp;
+}
+RD_HD int64_t qp_timer_min(const Sim &s, int h, int n)
+{
+    const int *list = s.qps + (size_t)h * s.qps_cap;
+    int64_t mn = INT64_MAX;
+    for (int i = 0; i < n; i++)
+    {
+        const SndQP &q = s.qp[list[i]];
+        const int64_t t[3] = {q.t_alpha, q.t_dec, q.t_rp};
+        for (int k = 0; k < 3; k++)
+            mn = t[k] >= 0 && t[k] < mn ? t[k] : mn;
+    }
+    return mn;
+}
+
+// One host's slot (src/rdma/rstep.cuh:1649-2112 for a host with one NIC): its events in ns-3's
+// order - the candidates are its next arrival, its TransmitComplete, its m_nextSend, its next DCQCN
+// timer and its next flow start; the earliest runs, simultaneous ones by their chains. Returns
+// whether the host is active (sent, or holds frames); *wake: its earliest event without input.
+RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, uint32_t *pend, int pend_n, int64_t *wake)
+{
+    const int64_t D = s.D, t0 = slot * D, t1 = t0 + D;
+    const Params &P = s.Pw[s.h_world[h]];
+    const int lin = s.h_link_in[h];
+    uint64_t abase;
+    const int an = bucket(s, lin, slot, &abase);
+    if (an == 0 && s.hs[h].wake >= t1)
+    {
+        *wake = s.hs[h].wake;
+        return false;
+    }
+    HCtx c;
+    c.s = &s;
+    c.P = &P;
+    c.pend = pend;
+    c.pend_n = pend_n;
+    c.h = h;
+    c.slot = slot;
+    c.now = t0;
+    c.nout = 0;
+    c.ord = 0;
+    c.st = s.hs[h];
+    c.link = s.h_link_out[h];
+    c.ldelay = s.li[c.link].delay;
+    ob_open(c.ob, c.link, c.st.ob_last);
+    const int64_t in_delay = s.li[lin].delay;
+    // the DCQCN timers due in this slot, in event order (their intervals are at least a slot: none
+    // set in it falls due in it)
+    Due *due = s.due + (size_t)h * MAX_DUE;
+    int ndue = 0;
+    if (P.cc == 1 && c.st.due_min < t1)
+    {
+        const int *list = s.qps + (size_t)h * s.qps_cap;
+        int64_t mn = INT64_MAX;
+        for (int i = 0; i < c.st.qp_n; i++)
+        {
+            const SndQP &q = s.qp[list[i]];
+            const int64_t t[3] = {q.t_alpha, q.t_dec, q.t_rp};
+            for (int k = 0; k < 3; k++)
+                mn = t[k] >= 0 && t[k] < mn ? t[k] : mn;
+            for (int k = 0; k < 3; k++)
+                if (t[k] >= 0 && t[k] < t1)
+                {
+                    if (ndue == MAX_DUE)
+                    {
+                        count(s, EV_DUE_OVF);
+                        continue;
+                    }
+                    Due x = {t[k], list[i], k};
+                    int j = ndue++;
+                    while (j > 0 && before(s, x.at, 0, due_chain(s, x), due[j - 1].at, 0, due_chain(s, due[j - 1])))
+                    {
+                        due[j] = due[j - 1];
+                        j--;
+                    }
+                    due[j] = x;
+                }
+        }
+        c.st.due_min = mn;
+    }
+    int di = 0, ai = 0;
+    int sf = -1, sf_at = -1;
+    int64_t sf_t = INT64_MAX;
+    for (;;)
+    {
+        const LFrame *af = ai < an ? &s.msg[abase + ai] : nullptr;
+        const uint32_t acode = af ? ccode(link_tx(s, lin, af->p.size) + in_delay, af->p.ord) : 0;
+        while (di < ndue && due_now(s, due[di]) != due[di].at)
+            di++; // cancelled or moved
+        if (sf_at != c.st.next_start)
+        {
+            sf_at = c.st.next_start;
+            sf = sf_at < c.st.start_end ? s.start_list[sf_at] : -1;
+            sf_t = sf >= 0 ? s.fi[sf].start : INT64_MAX;
+        }
+        // candidate times: 0 arrival, 1 TransmitComplete, 2 m_nextSend, 3 DCQCN timer, 4 flow start
+        int64_t tt[5];
+        tt[0] = af ? af->ts : INT64_MAX;
+        tt[1] = c.st.tc_at >= 0 && c.st.tc_at < t1 ? c.st.tc_at : INT64_MAX;
+        tt[2] = c.st.ns_at >= 0 && c.st.ns_at < t1 ? c.st.ns_at : INT64_MAX;
+        tt[3] = di < ndue ? due[di].at : INT64_MAX;
+        tt[4] = sf_t < t1 ? sf_t : INT64_MAX;
+        // only the candidates at the earliest time are compared (in candidate order)
+        int64_t bt = INT64_MAX;
+        for (int k = 0; k < 5; k++)
+            bt = tt[k] < bt ? tt[k] : bt;
+        int bk = -1;
+        for (int k = 0; k < 5 && bt != INT64_MAX; k++)
+        {
+            const int64_t t = tt[k];
+            if (t != bt)
+                continue;
+            if (bk < 0)
+            {
+                bk = k;
+                continue;
+            }
+            // a tie: the quick prefixes where both are arrivals, TransmitCompletes or sends, else the chains
+            const uint32_t pk = k == 0 ? acode : 0, pb = bk == 0 ? acode : 0;
+            const Chain &ck = k == 0 ? af->c : k == 1 ? s.h_tc[h] : k == 2 ? s.h_ns[h] : k == 3 ? due_chain(s, due[di]) : s.fi[sf].start_chain;
+            const Chain &cb = bk == 0 ? af->c : bk == 1 ? s.h_tc[h] : bk == 2 ? s.h_ns[h] : bk == 3 ? due_chain(s, due[di]) : s.fi[sf].start_chain;
+            const bool b4 = (k <= 2 && bk <= 2) ? before_q(s, t, pk, ck, pb, cb) : before(s, t, pk, ck, bt, pb, cb);
+            if (b4)
+                bk = k;
+        }
+        if (bk < 0)
+            break;
+        if (bt < t0)
+            count(s, EV_LATE);
+        c.now = bt;
+        if (bk == 0)
+            cpush(s, &c.cur, acode, af->c);
+        else
+            c.cur = bk == 1 ? s.h_tc[h] : bk == 2 ? s.h_ns[h] : bk == 3 ? due_chain(s, due[di]) : s.fi[sf].start_chain;
+        c.ord = 0;
+        if (bk == 0)
+        {
+            const Pkt p = af->p;
+            ai++;
+            if (p.kind == K_DATA)
+                h_data(c, p);
+            else if (p.kind == K_ACK || p.kind == K_NACK)
+                h_ack(c, p, flows_done);
+            else if (p.kind == K_PAUSE)
+                c.st.paused |= 1u << p.q;
+            else
+            {
+                c.st.paused &= ~(1u << p.q);
+                h_dequeue(c);
+            }
+        }
+        else if (bk == 1)
+        {
+            c.st.tc_at = -1;
+            h_dequeue(c);
+        }
+        else if (bk == 2)
+        {
+            c.st.ns_at = -1;
+            h_dequeue(c);
+        }
+        else if (bk == 3)
+        {
+            const Due x = due[di++];
+            dcqcn_timer(c, x.flow, x.kind);
+        }
+        else
+        {
+            c.st.next_start++;
+            h_start(c, sf);
+        }
+    }
+    bool act = c.nout > 0;
+    int64_t w = INT64_MAX;
+    if (c.st.next_start < c.st.start_end)
+        w = s.fi[s.start_list[c.st.next_start]].start;
+    ob_flush(s, c.ob);
+    c.st.ob_last = c.ob.last;
+    if (c.st.ackq_n > 0 || c.st.tc_at >= 0)
+        act = true;
+    if (c.st.ns_at >= 0 && c.st.ns_at < w)
+        w = c.st.ns_at;
+    if (P.cc == 1)
+    {
+        if (c.st.due_min < t1)
+            c.st.due_min = qp_timer_min(s, h, c.st.qp_n);
+        if (c.st.due_min < w)
+            w = c.st.due_min;
+    }
+    if (c.st.tc_at >= 0 && c.st.tc_at < w)
+        w = c.st.tc_at;
+    c.st.wake = w;
+    s.hs[h] = c.st;
+    *wake = w;
+    return act;
+}
+
+// ------------------------------------------------------------------ switch (src/rdma/rstep.cuh:2117-2467)
+struct SCtx
+{
+    const Sim *s;
+    const Params *P;
+    uint32_t *pend;
+    int pend_n;
+    int x, node;
+    Port *pt;      // its ports (in HBM)
+    PortRun *w;
+    Chain *tc;     // the pending TransmitCompletes' chains
+    const PortCfg *pc;
+    Switch sw;
+    int64_t now;
+    Chain cur;     // made on first use: cpush(cur_code, *cur_src)
+    const Chain *cur_src;
+    uint32_t cur_code;
+    bool cur_ok;
+    int ord;
+};
+RD_HD const Chain &sc_cur(SCtx &c)
+{
+    if (!c.cur_ok)
+    {
+        if (c.cur_code)
+            cpush(*c.s, &c.cur, c.cur_code, *c.cur_src);
+        else
+            c.cur = *c.cur_src;
+        c.cur_ok = true;
+    }
+    return c.cur;
+}
+RD_HD uint32_t pfc_thresh(const SCtx &c, int port)
+{
+    return (c.P->buffer_size - c.sw.total_hdrm - c.sw.total_rsrv - c.sw.shared_used) >> c.pc[port].shift;
+}
+RD_HD uint32_t shared_used(const Port &p) { return p.ing > RESERVE ? p.ing - RESERVE : 0; }
+
+RD_HD void s_transmit(SCtx &c, int port, Pkt p)
+{
+    const Sim &s = *c.s;
+    const int lo = c.pc[port].link_out;
+    int64_t tx = link_tx(s, lo, p.size);
+    c.pt[port].tc_at = c.now + tx;
+    const Chain &cur = sc_cur(c);
+    cpush(s, &c.tc[port], ccode(tx, c.ord++), cur);
+    p.ord = (uint8_t)(c.ord < 255 ? c.ord : 255);
+    c.ord++;
+    c.w[port].nout++;
+    int64_t ts = c.now + tx + s.li[lo].delay;
+    LFrame *f = ob_push(s, c.pend, c.pend_n, ts, c.w[port].ob);
+    if (!f)
+        return;
+    f->ts = ts;
+    f->c = cur;
+    f->p = p;
+}
+RD_HD void s_enqueue(SCtx &c, int port, int q, const Pkt &p)
+{
+    const Sim &s = *c.s;
+    Port &pt = c.pt[port];
+    const size_t pi = (size_t)s.si[c.x].pbase + port;
+    if (q == 0)
+    {
+        if (pt.q0_n >= s.q0_cap)
+        {
+            count(s, EV_QUEUE_OVF);
+            return;
+        }
+        s.q0[pi * s.q0_cap + ring_add(pt.q0_head, pt.q0_n, s.q0_cap)] = p;
+        pt.q0_n++;
+    }
+    else
+    {
+        if (pt.q3_n >= s.q3_cap)
+        {
+            count(s, EV_QUEUE_OVF);
+            return;
+        }
+        s.q3[pi * s.q3_cap + ring_add(pt.q3_head, pt.q3_n, s.q3_cap)] = p;
+        pt.q3_n++;
+    }
+    pt.bytes_total += p.size;
+}
+RD_HD void s_start_q0(SCtx &c, int port)
+{
+    const Sim &s = *c.s;
+    Port &pt = c.pt[port];
+    if (pt.tc_at >= 0)
+        return;
+    const size_t pi = (size_t)s.si[c.x].pbase + port;
+    Pkt p = s.q0[pi * s.q0_cap + pt.q0_head];
+    pt.q0_head = ring_add(pt.q0_head, 1, s.q0_cap);
+    pt.q0_n--;
+    pt.bytes_total -= p.size;
+    pt.tx_bytes += p.size;
+    s_transmit(c, port, p);
+}
+// QbbNetDevice::SendPfc
+RD_HD void s_send_pfc(SCtx &c, int port, int type)
+{
+    Pkt p = {};
+    p.kind = type == 0 ? K_PAUSE : K_RESUME;
+    p.q = PG;
+    p.size = PFC_SIZE;
+    count(*c.s, type == 0 ? EV_PAUSE : EV_RESUME);
+    s_enqueue(c, port, 0, p);
+    if (c.pt[port].q0_n > 0)
+        s_start_q0(c, port);
+}
+// SwitchNotifyDequeue, ECN marking, CheckAndSendResume, INT, TransmitStart for a data-queue frame
+// (src/rdma/rstep.cuh:2322-2395)
+RD_HD void s_start_q3(SCtx &c, int port, Pkt p)
+{
+    const Sim &s = *c.s;
+    const Params &P = *c.P;
+    Port &pt = c.pt[port];
+    Port &in = c.pt[p.inport];
+    uint32_t from_hdrm = in.hdrm < p.size ? in.hdrm : p.size;
+    uint32_t sh = in.ing > RESERVE ? in.ing - RESERVE : 0;
+    uint32_t from_shared = p.size - from_hdrm < sh ? p.size - from_hdrm : sh;
+    in.hdrm -= from_hdrm;
+    in.ing -= p.size - from_hdrm;
+    c.sw.shared_used -= from_shared;
+    pt.egr -= p.size;
+    if (P.ecn)
+    {
+        // SwitchMmu::ShouldSendCN
+        bool mark = false;
+        const uint32_t kmin = c.pc[port].kmin, kmax = c.pc[port].kmax;
+        if (pt.egr > kmax)
+            mark = true;
+        else if (pt.egr > kmin)
+        {
+            double pr = c.pc[port].pmax * (double)(pt.egr - kmin) / (double)(kmax - kmin);
+            mark = rand_u01(pt.rng) < pr;
+        }
+        if (mark)
+        {
+            p.ecn = 3;
+            count(s, EV_MARK);
+        }
+    }
+    if (in.mmu_paused)
+    {
+        uint32_t su = shared_used(in);
+        if (in.hdrm == 0 && (su == 0 || su + RESUME_OFFSET <= pfc_thresh(c, p.inport)))
+        {
+            s_send_pfc(c, p.inport, 1);
+            in.mmu_paused = 0;
+        }
+    }
+    if (p.kind == K_DATA && P.cc == 3 && p.nhop < 0xFFFF)
+    {
+        const uint64_t hv = hop_make((uint64_t)c.now, pt.tx_bytes, pt.bytes_total, c.pc[port].bps, P.int_multi);
+        p.hop[p.nhop % MAX_HOP] = hv;
+        p.nhop++;
+    }
+    pt.tx_bytes += p.size;
+    s_transmit(c, port, p);
+}
+// QbbNetDevice::DequeueAndTransmit on a switch (src/rdma/rstep.cuh:2299-2320)
+RD_HD void s_dequeue(SCtx &c, int port)
+{
+    const Sim &s = *c.s;
+    Port &pt = c.pt[port];
+    if (pt.tc_at >= 0)
+        return;
+    if (pt.q0_n > 0)
+    {
+        s_start_q0(c, port);
+        return;
+    }
+    if ((pt.paused >> PG & 1) || pt.q3_n == 0)
+        return;
+    Pkt p = s.q3[((size_t)s.si[c.x].pbase + port) * s.q3_cap + pt.q3_head];
+    pt.q3_head = ring_add(pt.q3_head, 1, s.q3_cap);
+    pt.q3_n--;
+    pt.bytes_total -= p.size;
+    s_start_q3(c, port, p);
+}
+// SwitchNode's route: the egress port of a data frame or ACK (-1: none) and its queue
+// (src/rdma/rstep.cuh:2397-2409)
+RD_HD int s_route(const Sim &s, const Params &P, int x, const Pkt &p, int *q)
+{
+    const SwInfo &si = s.si[x];
+    const bool data = p.kind == K_DATA;
+    const int src = (data ? p.src : p.dst) - si.hbase, dst = (data ? p.dst : p.src) - si.hbase;
+    const Route rt = s.rt[si.rt_off + dst];
+    *q = (P.ack_high_prio && !data) ? 0 : PG;
+    if (rt.n == 0)
+        return -1;
+    const uint32_t w2 = data ? (p.sport | 100u << 16) : (100u | (uint32_t)p.sport << 16);
+    const uint32_t hsh = ecmp_hash(node_ip(src), node_ip(dst), w2, si.seed);
+    return s.rtp[rt.off + hsh % rt.n];
+}
+// ingress admission, PFC PAUSE, enqueue or cut-through (src/rdma/rstep.cuh:2411-2467)
+RD_HD void s_forward(SCtx &c, int inport, Pkt p, int out, int q)
+{
+    const Sim &s = *c.s;
+    if (out < 0)
+        return;
+    if (q != 0)
+    {
+        Port &in = c.pt[inport];
+        const uint32_t th = pfc_thresh(c, inport);
+        // CheckIngressAdmission
+        if (p.size + in.hdrm > c.pc[inport].hdrm_cap && p.size + shared_used(in) > th)
+        {
+            count(s, EV_DROP);
+            return;
+        }
+        // UpdateIngressAdmission, UpdateEgressAdmission
+        const uint32_t nb = in.ing + p.size;
+        if (nb <= RESERVE)
+            in.ing += p.size;
+        else if (nb - RESERVE > th)
+            in.hdrm += p.size;
+        else
+        {
+            in.ing += p.size;
+            c.sw.shared_used += p.size < nb - RESERVE ? p.size : nb - RESERVE;
+        }
+        c.pt[out].egr += p.size;
+        // CheckAndSendPfc
+        if (!in.mmu_paused && (in.hdrm > 0 || shared_used(in) >= pfc_thresh(c, inport)))
+        {
+            s_send_pfc(c, inport, 0);
+            in.mmu_paused = 1;
+        }
+    }
+    p.inport = (uint8_t)inport;
+    Port &o = c.pt[out];
+    if (q != 0 && o.tc_at < 0 && o.q0_n == 0 && o.q3_n == 0 && !(o.paused >> PG & 1))
+    {
+        s_start_q3(c, out, p); // an idle port with empty queues hands it straight on
+        return;
+    }
+    s_enqueue(c, out, q, p);
+    s_dequeue(c, out);
+}
+
+// a port's next arrival of the slot
+RD_HD void sw_peek(const Sim &s, const SCtx &c, int p)
+{
+    PortRun &w = c.w[p];
+    w.ats = INT64_MAX;
+    if (w.ai < w.an)
+    {
+        const LFrame &f = s.msg[w.abase + w.ai];
+        const int lin = c.pc[p].link_in;
+        w.ats = f.ts;
+        w.acode = ccode(link_tx(s, lin, f.p.size) + s.li[lin].delay, f.p.ord);
+    }
+}
+
+// One switch's slot, every event in order on one lane (the reference's serial path,
+// src/rdma/rstep.cuh:3032-3189 and 3436-3446): candidates are each port's next arrival and its
+// TransmitComplete, in port order, arrival first; ties by the chains, undecided ties keep the
+// earlier candidate. Returns whether the switch is busy (sent, or holds frames).
+RD_HD bool switch_step(const Sim &s, int x, int64_t slot, uint32_t *pend, int pend_n)
+{
+    const SwInfo &si = s.si[x];
+    const int64_t t1 = (slot + 1) * s.D;
+    const int deg = si.deg;
+    const int w = s.s_world[x];
+    SCtx c;
+    c.s = &s;
+    c.P = &s.Pw[w];
+    c.pend = pend;
+    c.pend_n = pend_n;
+    c.x = x;
+    c.node = s.H + x;
+    c.pt = s.ps + si.pbase;
+    c.w = s.pr + si.pbase;
+    c.tc = s.p_tc + si.pbase;
+    c.pc = s.pc + si.pbase;
+    c.sw = s.sw[x];
+    c.now = slot * s.D;
+    for (int p = 0; p < deg; p++)
+    {
+        PortRun &r = c.w[p];
+        r.ai = 0;
+        r.nout = 0;
+        r.an = bucket(s, c.pc[p].link_in, slot, &r.abase);
+        ob_open(r.ob, c.pc[p].link_out, c.pt[p].ob_last);
+        sw_peek(s, c, p);
+    }
+    for (;;)
+    {
+        int best = -1, bk = 0;
+        int64_t bt = INT64_MAX;
+        for (int p = 0; p < deg; p++)
+        {
+            const int64_t ta = c.w[p].ats, tcat = c.pt[p].tc_at;
+            for (int k = 0; k < 2; k++)
+            {
+                const int64_t t = k ? tcat : ta;
+                if (k ? (tcat < 0 || tcat >= t1 || tcat > bt) : (ta == INT64_MAX || ta > bt))
+                    continue;
+                if (t < bt || best < 0)
+                {
+                    best = p, bk = k, bt = t;
+                    continue;
+                }
+                const uint32_t ca = k ? 0 : c.w[p].acode, cb = bk ? 0 : c.w[best].acode;
+                const Chain &xa = k ? c.tc[p] : s.msg[c.w[p].abase + c.w[p].ai].c;
+                const Chain &xb = bk ? c.tc[best] : s.msg[c.w[best].abase + c.w[best].ai].c;
+                if (before_q(s, t, ca, xa, cb, xb))
+                    best = p, bk = k;
+            }
+        }
+        if (best < 0)
+            break;
+        const int p = best;
+        c.now = bt;
+        c.ord = 0;
+        c.cur_ok = false;
+        if (bk == 1)
+        {
+            c.cur_src = &c.tc[p];
+            c.cur_code = 0;
+            sc_cur(c); // (the TransmitComplete's chain is overwritten by the next transmission)
+            c.pt[p].tc_at = -1;
+            s_dequeue(c, p);
+            continue;
+        }
+        const LFrame &f = s.msg[c.w[p].abase + c.w[p].ai];
+        const Pkt pk = f.p;
+        c.cur_code = c.w[p].acode;
+        c.cur_src = &f.c;
+        c.w[p].ai++;
+        sw_peek(s, c, p);
+        if (pk.kind == K_PAUSE || pk.kind == K_RESUME)
+        {
+            if (pk.kind == K_PAUSE)
+                c.pt[p].paused |= 1u << pk.q;
+            else
+            {
+                c.pt[p].paused &= ~(1u << pk.q);
+                s_dequeue(c, p);
+            }
+        }
+        else
+        {
+            int q = PG;
+            const int out = s_route(s, *c.P, x, pk, &q);
+            s_forward(c, p, pk, out, q);
+        }
+    }
+    s.sw[x] = c.sw;
+    bool act = false;
+    for (int p = 0; p < deg; p++)
+    {
+        ob_flush(s, c.w[p].ob);
+        c.pt[p].ob_last = c.w[p].ob.last;
+        if (c.w[p].nout > 0 || c.pt[p].q0_n > 0 || c.pt[p].q3_n > 0 || c.pt[p].tc_at >= 0)
+            act = true;
+    }
+    return act;
+}
+
+// does the switch have events in this slot? (busy last slot, or arrivals; src/rdma/rstep.cuh:3859-3881)
+RD_HD bool switch_has_input(const Sim &s, int x, int64_t slot)
+{
+    const SwInfo &si = s.si[x];
+    for (int p = 0; p < si.deg; p++)
+    {
+        uint64_t base;
+        if (bucket(s, s.pc[si.pbase + p].link_in, slot, &base) > 0)
+            return true;
+    }
+    return false;
+}
+
+// The end of a world's slot, after all its entities ran it (src/rdma/rstep.cuh:3451-3585 without
+// windows): finish when every flow is done or at the stop; with nothing sent or queued anywhere,
+// skip to the slot of the earliest host event or pending arrival; with none, the run is quiescent.
+RD_HD void slot_end(const Sim &s, const World &wd, WorldRun &r, uint32_t *pend, unsigned act, unsigned long long wake)
+{
+    const int64_t slot = r.slot;
+    // (an atomic read: the lanes' completions were counted by atomics, in L2)
+    if (at_add32(&r.flows_done, 0u) >= (uint32_t)wd.nf || slot + 1 >= wd.max_slots)
+        r.finished = 1;
+    pend[slot % wd.pend_n] = 0;
+    int64_t next = slot + 1;
+    if (act == 0)
+    {
+        int64_t ws = wake != ~0ull ? (int64_t)(wake / (unsigned long long)s.D) : INT64_MAX;
+        for (int i = 1; i < wd.pend_n; i++)
+            if (pend[(slot + i) % wd.pend_n])
+            {
+                ws = slot + i < ws ? slot + i : ws;
+                break;
+            }
+        if (ws != INT64_MAX && ws > next)
+            next = ws < wd.max_slots ? ws : wd.max_slots;
+        if (ws == INT64_MAX)
+            r.finished = 1;
+    }
+    r.slots_run++;
+    r.slot = next;
+}
+
+} // namespace rd

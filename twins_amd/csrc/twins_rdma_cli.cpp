@@ -1,0 +1,217 @@
+// twins-rdma: the RDMA model's command line over libtwins_core (include/twins_rdma.h), with the
+// reference's arguments, output files and summary lines (src/rdma/rmain.cu:1-42, 1023-1254,
+// 2178-2258, 2316-2373):
+//
+//   twins-rdma CONFIG --topo=F --flows=F --out=F [--rate=100Gbps --delay=1000] [--q3cap=N]
+//              [--tail=log2] [KEY=VALUE ...]
+//   twins-rdma CONFIG --batch=JOBFILE --outdir=DIR [--path=DIR:DIR...] [options] [KEY=VALUE ...]
+//
+// out: "id start_ns fct_ns" per finished flow, by id. Exit status: 0 every flow done, 2 some
+// unfinished at the stop time, 3 an overflow or a late event (the results are not exact), 1 an
+// argument or input error. (The reference's launch-shape flags --serial, --spread, --win, --gwin,
+// --persist and --rank* are accepted and change nothing here.)
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/twins.h"
+#include "../../include/twins_rdma.h"
+
+using std::string;
+using std::vector;
+
+static uint64_t parse_rate(const string &s)
+{
+    size_t i = 0;
+    while (i < s.size() && (isdigit((unsigned char)s[i]) || s[i] == '.'))
+        i++;
+    double v = atof(s.substr(0, i).c_str());
+    const char u = i < s.size() ? s[i] : 'b';
+    double m = (u == 'K' || u == 'k') ? 1e3 : u == 'M' ? 1e6 : u == 'G' ? 1e9 : 1;
+    return (uint64_t)(v * m);
+}
+
+struct Job
+{
+    string name, topo, flows;
+    vector<string> kv;
+};
+
+int main(int argc, char **argv)
+{
+    if (argc < 2)
+    {
+        fprintf(stderr, "usage: twins-rdma CONFIG --topo=F --flows=F --out=F [KEY=VALUE ...]\n"
+                        "       twins-rdma CONFIG --batch=JOBFILE --outdir=DIR [KEY=VALUE ...]\n");
+        return 1;
+    }
+    const string config = argv[1];
+    string topo, flows, out = "fct.txt", batch, outdir = ".", dirs, rate_s = "100Gbps";
+    int64_t delay = 1000;
+    trd_pack_cfg pc = {};
+    vector<string> kv;
+    for (int i = 2; i < argc; i++)
+    {
+        const string a = argv[i];
+        const size_t eq = a.find('=');
+        const string k = a.substr(0, eq), v = eq == string::npos ? "" : a.substr(eq + 1);
+        if (k == "--topo")
+            topo = v;
+        else if (k == "--flows")
+            flows = v;
+        else if (k == "--out")
+            out = v;
+        else if (k == "--batch")
+            batch = v;
+        else if (k == "--outdir")
+            outdir = v;
+        else if (k == "--path")
+            dirs = v;
+        else if (k == "--rate")
+            rate_s = v;
+        else if (k == "--delay")
+            delay = atoll(v.c_str());
+        else if (k == "--q3cap")
+            pc.q3cap = atoi(v.c_str());
+        else if (k == "--tail")
+            pc.tail_log2 = atoi(v.c_str());
+        else if (k == "--serial" || k == "--spread" || k == "--win" || k == "--gwin" || k == "--persist" || k.compare(0, 6, "--rank") == 0)
+            continue;
+        else if (k.compare(0, 2, "--") != 0 && eq != string::npos)
+            kv.push_back(a);
+        else
+        {
+            fprintf(stderr, "unknown argument %s\n", argv[i]);
+            return 1;
+        }
+    }
+    const uint64_t bps = parse_rate(rate_s);
+    auto find_file = [&](const string &f) {
+        size_t b = 0;
+        while (!dirs.empty() && b <= dirs.size())
+        {
+            size_t c = dirs.find(':', b);
+            const string p = dirs.substr(b, c == string::npos ? string::npos : c - b) + "/" + f;
+            if (std::ifstream(p))
+                return p;
+            if (c == string::npos)
+                break;
+            b = c + 1;
+        }
+        return f;
+    };
+    vector<Job> jobs;
+    if (batch.empty())
+        jobs.push_back({"run", topo, flows, kv});
+    else
+    {
+        std::ifstream f(batch);
+        if (!f)
+        {
+            fprintf(stderr, "cannot open %s\n", batch.c_str());
+            return 1;
+        }
+        string line;
+        while (std::getline(f, line))
+        {
+            std::istringstream is(line);
+            Job j;
+            if (!(is >> j.name) || j.name[0] == '#' || !(is >> j.topo >> j.flows))
+                continue;
+            j.kv = kv;
+            string x;
+            while (is >> x)
+            {
+                if (x.find('=') == string::npos)
+                {
+                    fprintf(stderr, "%s: bad override %s\n", j.name.c_str(), x.c_str());
+                    return 1;
+                }
+                j.kv.push_back(x);
+            }
+            j.topo = find_file(j.topo);
+            j.flows = find_file(j.flows);
+            jobs.push_back(j);
+        }
+        if (jobs.empty())
+        {
+            fprintf(stderr, "%s: no jobs\n", batch.c_str());
+            return 1;
+        }
+    }
+    vector<trd_world *> worlds;
+    int F = 0, H = 0, S = 0;
+    for (auto &j : jobs)
+    {
+        vector<const char *> ov;
+        for (auto &x : j.kv)
+            ov.push_back(x.c_str());
+        trd_world *w = nullptr;
+        if (trd_world_create(config.c_str(), ov.data(), (int)ov.size(), j.topo.c_str(), j.flows.c_str(), bps, delay, &w))
+        {
+            fprintf(stderr, "%s: %s\n", j.name.c_str(), trd_last_error());
+            return 1;
+        }
+        int32_t h, s, f;
+        uint64_t rtt;
+        uint32_t bdp;
+        trd_world_info(w, &h, &s, &f, &rtt, &bdp);
+        if (jobs.size() == 1)
+            printf("maxRtt=%llu maxBdp=%u\n", (unsigned long long)rtt, bdp);
+        F += f, H += h, S += s;
+        worlds.push_back(w);
+    }
+    if (jobs.size() > 1)
+        printf("%d worlds: %d hosts, %d switches, %d flows\n", (int)jobs.size(), H, S, F);
+    fflush(stdout);
+    trd_batch *b = nullptr;
+    if (trd_batch_create(worlds.data(), (int)worlds.size(), &pc, &b))
+    {
+        fprintf(stderr, "%s\n", trd_last_error());
+        return 1;
+    }
+    trd_run_stats st;
+    if (trd_batch_run(b, &st))
+    {
+        fprintf(stderr, "%s\n", trd_last_error());
+        return 1;
+    }
+    uint64_t ev[40];
+    trd_counters_get(b, ev);
+    for (size_t wi = 0; wi < jobs.size(); wi++)
+    {
+        int32_t f = 0;
+        trd_world_info(worlds[wi], nullptr, nullptr, &f, nullptr, nullptr);
+        vector<int64_t> start(f), fct(f);
+        trd_fct_get(b, (int)wi, start.data(), fct.data());
+        const string path = batch.empty() ? out : outdir + "/" + jobs[wi].name + ".fct";
+        FILE *fo = fopen(path.c_str(), "w");
+        if (!fo)
+        {
+            fprintf(stderr, "cannot write %s\n", path.c_str());
+            return 1;
+        }
+        for (int i = 0; i < f; i++)
+            if (fct[i] >= 0)
+                fprintf(fo, "%d %lld %lld\n", i, (long long)start[i], (long long)fct[i]);
+        fclose(fo);
+    }
+    printf("flows done %u of %d; simulated %.6f s in %.3f s wall (%d launches)\n", st.flows_done, F, st.sim_ns * 1e-9, st.wall_s,
+           st.launches);
+    printf("drops %llu marks %llu pauses %llu resumes %llu nacks %llu | undecided ties %llu, on lost tails %llu | overflows: tx %llu queue "
+           "%llu ackq %llu qps %llu due %llu | acks for done QPs %llu\n",
+           (unsigned long long)ev[0], (unsigned long long)ev[1], (unsigned long long)ev[2], (unsigned long long)ev[3],
+           (unsigned long long)ev[11], (unsigned long long)ev[8], (unsigned long long)ev[9], (unsigned long long)ev[4],
+           (unsigned long long)ev[5], (unsigned long long)ev[6], (unsigned long long)ev[7], (unsigned long long)ev[12],
+           (unsigned long long)ev[10]);
+    const bool bad = ev[4] || ev[5] || ev[6] || ev[7] || ev[12] || ev[32];
+    if (bad)
+        fprintf(stderr, "warning: overflows or late events: the results are not exact\n");
+    trd_batch_destroy(b);
+    for (auto *w : worlds)
+        trd_world_destroy(w);
+    return bad ? 3 : st.flows_done == (uint32_t)F ? 0 : 2;
+}
