@@ -8,11 +8,15 @@
 
 namespace rd
 {
-int launch_worlds(const Sim *sp, int nworlds, int, hipStream_t)
+static SwWork g_work; // a switch's slot workspace (LDS on the GPU)
+int launch_capacity(int) { return 64; }
+// every world of the queue to its end, one after the other (device memory is host memory here)
+int launch_worlds(const Sim *sp, const LaunchCfg *cfg, int, int, hipStream_t)
 {
     const Sim &s = *sp;
-    for (int w = 0; w < nworlds; w++)
+    for (int qi = 0; qi < cfg->nq; qi++)
     {
+        const int w = cfg->queue[qi];
         const World &wd = s.wd[w];
         WorldRun &r = s.run[w];
         uint32_t *pend = s.pend + wd.pend_off;
@@ -34,7 +38,8 @@ int launch_worlds(const Sim *sp, int nworlds, int, hipStream_t)
                     const int x = wd.s0 + (e - wd.nh);
                     bool busy = false;
                     if (s.sw_busy[x] || switch_has_input(s, x, r.slot))
-                        busy = switch_step(s, x, r.slot, pend, wd.pend_n);
+                        busy = s.si[x].deg <= KS ? switch_slot(s, x, r.slot, pend, wd.pend_n, g_work, 0, 1)
+                                                 : switch_step(s, x, r.slot, pend, wd.pend_n);
                     s.sw_busy[x] = busy ? 1 : 0;
                     act |= busy ? 1u : 0u;
                 }

@@ -473,6 +473,13 @@ struct trd_batch
     vector<int> fbase, nf;
     vector<int64_t> start; // [F] flows' starts, by global id
     int threads = 256;
+    int team_blocks = 0; // 0: from the worlds' sizes
+    vector<int> items;   // per world: work items per slot (host items of 64 hosts, switches)
+    int *queue_d = nullptr;
+    unsigned *qnext_d = nullptr;
+    rd::TeamState *teams_d = nullptr;
+    rd::LaunchCfg *cfg_d = nullptr;
+    int max_teams = 0;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool ran = false;
@@ -745,7 +752,10 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
             const rd::SwInfo &q = si[x];
             sw[x] = rd::Switch{};
             for (int p = 0; p < q.deg; p++)
+            {
                 sw[x].total_hdrm += pc[q.pbase + p].hdrm_cap;
+                sw[x].max_shift = std::max(sw[x].max_shift, pc[q.pbase + p].shift);
+            }
             sw[x].total_rsrv = rd::RESERVE * (uint32_t)q.deg;
             for (int p = 0; p < q.deg; p++)
             {
@@ -796,8 +806,21 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
     for (int f = 0; f < F; f++)
         b.start[f] = fi[f].start;
     b.nf.resize(W);
+    b.items.resize(W);
     for (int wi = 0; wi < W; wi++)
+    {
         b.nf[wi] = (int)worlds[wi]->fi.size();
+        b.items[wi] = (worlds[wi]->H + 63) / 64 + worlds[wi]->S;
+    }
+    // launches: the queue of worlds, the teams' barrier state
+    const int cap = rd::launch_capacity(b.threads);
+    if (cap <= 0)
+        return fail(TW_EDEVICE, "the rdma kernel fits no workgroup on the GPU");
+    b.max_teams = cap;
+    TRY(b.alloc(&b.queue_d, W));
+    TRY(b.alloc(&b.qnext_d, 1));
+    TRY(b.alloc(&b.teams_d, cap));
+    TRY(b.alloc(&b.cfg_d, 1));
     return TW_OK;
 }
 
@@ -813,7 +836,8 @@ extern "C" int trd_batch_create(trd_world *const *worlds, int32_t n, const trd_p
         return fail(TW_EDEVICE, "no GPU: the RDMA model runs on the GPU only");
     trd_pack_cfg c = cfg ? *cfg : trd_pack_cfg{};
     std::unique_ptr<trd_batch> b(new trd_batch);
-    b->threads = c.block_threads > 0 ? std::min(c.block_threads, 256) : 256;
+    b->threads = 256;
+    b->team_blocks = c.team_blocks;
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
         hipEventCreate(&b->e1) != hipSuccess)
         return fail(TW_EDEVICE, "hipStreamCreate failed");
@@ -834,10 +858,48 @@ extern "C" int trd_batch_run(trd_batch *b, trd_run_stats *st)
     int launches = 0;
     double kernel_s = 0;
     int64_t prev = -1;
+    // teams: one size for the launch, from the largest world's work items (about one per wave, at
+    // most 16 workgroups), as many teams as the GPU holds at once
+    const int cap = b->max_teams;
+    int T = b->team_blocks;
+    if (T <= 0)
+    {
+        int mx = 1;
+        for (int v : b->items)
+            mx = std::max(mx, v);
+        T = std::min(16, (mx + 3) / 4);
+    }
+    T = std::max(1, std::min(T, cap));
+    // worlds by work items, largest first
+    vector<int> order(b->W);
+    for (int i = 0; i < b->W; i++)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->items[a] > b->items[c]; });
     for (;;)
     {
+        vector<int> queue;
+        for (int w : order)
+            if (!b->runs[w].finished)
+                queue.push_back(w);
+        int nteams = std::min<int>((int)queue.size(), cap / T);
+        rd::LaunchCfg L = {};
+        L.T = T;
+        L.nq = (int)queue.size();
+        if (nteams >= 8)
+        {
+            nteams -= nteams % 8;
+            L.xcd_local = 1;
+        }
+        L.queue = b->queue_d;
+        L.qnext = b->qnext_d;
+        L.teams = b->teams_d;
+        if (hipMemcpyAsync(b->queue_d, queue.data(), sizeof(int) * queue.size(), hipMemcpyHostToDevice, b->st) != hipSuccess ||
+            hipMemcpyAsync(b->cfg_d, &L, sizeof L, hipMemcpyHostToDevice, b->st) != hipSuccess ||
+            hipMemsetAsync(b->qnext_d, 0, sizeof(unsigned), b->st) != hipSuccess ||
+            hipMemsetAsync(b->teams_d, 0, sizeof(rd::TeamState) * nteams, b->st) != hipSuccess)
+            return fail(TW_EDEVICE, "launch set-up failed");
         (void)hipEventRecord(b->e0, b->st);
-        int e = rd::launch_worlds(b->sim_d, b->W, b->threads, b->st);
+        int e = rd::launch_worlds(b->sim_d, b->cfg_d, nteams * T, b->threads, b->st);
         if (e)
             return fail(TW_EDEVICE, "rdma kernel launch failed (%d)", e);
         (void)hipEventRecord(b->e1, b->st);
@@ -854,8 +916,14 @@ extern "C" int trd_batch_run(trd_batch *b, trd_run_stats *st)
         {
             all &= r.finished != 0;
             total += r.slots_run;
-            if (r.error)
-                return fail(TW_EDEVICE, "a world's run failed (error %d)", r.error);
+        }
+        {
+            vector<rd::TeamState> ts(nteams);
+            if (nteams && hipMemcpy(ts.data(), b->teams_d, sizeof(rd::TeamState) * nteams, hipMemcpyDeviceToHost) != hipSuccess)
+                return fail(TW_EDEVICE, "hipMemcpy failed");
+            for (auto &t : ts)
+                if (t.err)
+                    return fail(TW_EDEVICE, "a team barrier timed out");
         }
         if (all)
             break;

@@ -6,7 +6,25 @@
 
 namespace rd
 {
-// one launch: each world's workgroup runs its slots until it finishes or the launch's time is up
-// (sim: the device copy of the Sim); 0 or a hipError_t
-int launch_worlds(const Sim *sim, int nworlds, int threads, hipStream_t st);
+// a team's barrier and its world's slot decisions (device memory, zeroed before a launch)
+struct TeamState
+{
+    unsigned arrive, gen;
+    int world, err;
+    unsigned act;
+    int stop;
+    unsigned long long wake;
+};
+// one launch: teams of T workgroups take worlds queue[0 .. nq) in order (qnext: the next index)
+struct LaunchCfg
+{
+    int T, nq, xcd_local, pad;
+    const int *queue;
+    unsigned *qnext;
+    TeamState *teams;
+};
+// the workgroups of 256 threads the GPU holds at once (-1: unknown)
+int launch_capacity(int threads);
+// one launch of nblocks workgroups (teams of cfg->T); cfg_dev and sim are device copies; 0 or a hipError_t
+int launch_worlds(const Sim *sim, const LaunchCfg *cfg_dev, int nblocks, int threads, hipStream_t st);
 } // namespace rd

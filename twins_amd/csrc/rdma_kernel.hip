@@ -1,81 +1,207 @@
-// The RDMA slot kernel for gfx950: one workgroup per world, persistent over the world's slots.
-// In a slot the workgroup's lanes take the world's hosts and switches in turn (a lane runs one
-// entity's events; entities only read what earlier slots filed, src/rdma/rdma.h:12-16), the
-// workgroup's barrier closes the slot, and lane 0 decides the next one (slot_end: finish, idle skip).
-// A launch ends when every world has finished or its time (about a second) is up; the host then
-// launches again.
+// The RDMA slot kernel for gfx950. A world runs on a team of T workgroups (256 threads, 4 waves
+// each) for all of its slots; teams take worlds from a queue, longest first. In a slot the world's
+// work items are dealt over the team's waves: a host item is up to 64 hosts side by side (a lane
+// runs one host's events, src/rdma/rstep.cuh:1649-2112), a switch item is one switch on one wave
+// (its ports in LDS; a safe slot runs each egress port's events on a lane of its own, otherwise lane
+// 0 runs them all; rdma_model.h switch_slot). Entities only read what earlier slots filed
+// (src/rdma/rdma.h:12-16), so a slot needs one team barrier; the last workgroup to arrive decides the
+// next slot (slot_end: finish, idle skip). A launch ends when the queue is empty or its time (about
+// a second) is up; the host launches again for the worlds left.
 #include <hip/hip_runtime.h>
 #include "rdma_model.h"
 #include "rdma_kernel.h"
 
 namespace rd
 {
-constexpr long long LAUNCH_TICKS = 100000000; // 1 s of the 100 MHz constant clock
+constexpr long long LAUNCH_TICKS = 100000000;          // a launch's time: 1 s of the 100 MHz constant clock
+constexpr unsigned long long WAIT_TICKS = 2000000000ull; // a barrier wait gives up after 20 s
+constexpr int WAVES = 4;                               // waves of a 256-thread workgroup
 
-__global__ void __launch_bounds__(256) rdma_worlds_kernel(const Sim *sp)
+__device__ __forceinline__ unsigned ld_acquire(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
+
+__global__ void __launch_bounds__(256) rdma_team_kernel(const Sim *__restrict__ sp, const LaunchCfg *__restrict__ lp)
 {
     const Sim &s = *sp;
-    const int w = blockIdx.x;
-    const World wd = s.wd[w];
-    WorldRun &r = s.run[w];
-    uint32_t *pend = s.pend + wd.pend_off;
-    __shared__ int64_t slot;
-    __shared__ int stop;
-    __shared__ unsigned act;
-    __shared__ unsigned long long wake;
-    const long long t0 = wall_clock64();
-    const int ne = wd.nh + wd.ns;
-    for (;;)
+    const LaunchCfg L = *lp;
+    __shared__ SwWork work[WAVES];
+    __shared__ int64_t s_slot;
+    __shared__ int s_w, s_stop, s_err;
+    __shared__ unsigned s_act;
+    __shared__ unsigned long long s_wake;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int T = L.T;
+    // a team's workgroups share an XCD (workgroups reach the 8 XCDs round-robin by id) when the
+    // teams fill the XCDs evenly
+    int team, rank;
+    if (L.xcd_local)
     {
-        if (threadIdx.x == 0)
-        {
-            slot = r.slot;
-            stop = r.finished || wall_clock64() - t0 > LAUNCH_TICKS;
-            act = 0;
-            wake = ~0ull;
-        }
+        const int g = blockIdx.x & 7, i = blockIdx.x >> 3;
+        team = (i / T) * 8 + g;
+        rank = i % T;
+    }
+    else
+    {
+        team = blockIdx.x / T;
+        rank = blockIdx.x % T;
+    }
+    TeamState *ts = L.teams + team;
+    const long long t_launch = wall_clock64();
+    unsigned gen = 0;
+
+    // the team barrier; the last workgroup to arrive runs `last` first
+    auto barrier = [&](auto last) -> bool {
         __syncthreads();
-        if (stop)
-            break;
-        const int64_t sl = slot;
-        unsigned a = 0;
-        unsigned long long wk = ~0ull;
-        for (int e = threadIdx.x; e < ne; e += blockDim.x)
+        if (tid == 0)
         {
-            if (e < wd.nh)
-            {
-                int64_t hw = INT64_MAX;
-                a |= host_step(s, wd.h0 + e, sl, &r.flows_done, pend, wd.pend_n, &hw) ? 1u : 0u;
-                if (hw != INT64_MAX && (unsigned long long)hw < wk)
-                    wk = (unsigned long long)hw;
-            }
+            if (T == 1)
+                last();
             else
             {
-                const int x = wd.s0 + (e - wd.nh);
-                bool busy = false;
-                if (s.sw_busy[x] || switch_has_input(s, x, sl))
-                    busy = switch_step(s, x, sl, pend, wd.pend_n);
-                s.sw_busy[x] = busy ? 1 : 0;
-                a |= busy ? 1u : 0u;
+                __threadfence();
+                const unsigned g = gen;
+                if (atomicAdd(&ts->arrive, 1u) == (unsigned)T - 1)
+                {
+                    __threadfence(); // (what the others wrote before arriving, e.g. pending-slot flags)
+                    last();
+                    atomicExch(&ts->arrive, 0u);
+                    __threadfence();
+                    atomicAdd(&ts->gen, 1u);
+                }
+                else
+                {
+                    const unsigned long long w0 = wall_clock64();
+                    while (ld_acquire(&ts->gen) == g)
+                    {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (wall_clock64() - w0 > WAIT_TICKS)
+                        {
+                            atomicExch(&ts->err, 1);
+                            break;
+                        }
+                    }
+                }
+                __threadfence();
+                gen = g + 1;
             }
+            s_err = T > 1 && __hip_atomic_load(&ts->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (a)
-            atomicOr(&act, a);
-        if (wk != ~0ull)
-            atomicMin(&wake, wk);
-        __threadfence_block();
         __syncthreads();
-        if (threadIdx.x == 0)
-            slot_end(s, wd, r, pend, act, wake);
+        return s_err;
+    };
+
+    for (;;)
+    {
+        // the team's next world (none once the launch's time is up: the host launches again)
+        if (barrier([&] {
+                ts->world = wall_clock64() - t_launch > LAUNCH_TICKS ? L.nq : (int)atomicAdd(L.qnext, 1u);
+                ts->act = 0;
+                ts->wake = ~0ull;
+                ts->stop = 0;
+            }))
+            return;
+        if (tid == 0)
+            s_w = __hip_atomic_load(&ts->world, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
+        if (s_w >= L.nq)
+            return;
+        const int w = L.queue[s_w];
+        const World wd = s.wd[w];
+        WorldRun &r = s.run[w];
+        uint32_t *pend = s.pend + wd.pend_off;
+        const int nhw = (wd.nh + 63) / 64, items = nhw + wd.ns, TW = T * WAVES, gw = rank * WAVES + wv;
+        const long long t_world = wall_clock64();
+        for (;;)
+        {
+            if (tid == 0)
+            {
+                s_slot = __hip_atomic_load(&r.slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_stop = __hip_atomic_load(&r.finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                         __hip_atomic_load(&ts->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_act = 0;
+                s_wake = ~0ull;
+            }
+            __syncthreads();
+            if (s_stop)
+                break;
+            const int64_t sl = s_slot;
+            unsigned a = 0;
+            unsigned long long wk = ~0ull;
+            for (int it = gw; it < items; it += TW)
+            {
+                if (it < nhw)
+                {
+                    const int e = it * 64 + lane;
+                    if (e < wd.nh)
+                    {
+                        int64_t hw = INT64_MAX;
+                        a |= host_step(s, wd.h0 + e, sl, &r.flows_done, pend, wd.pend_n, &hw) ? 1u : 0u;
+                        if (hw != INT64_MAX && (unsigned long long)hw < wk)
+                            wk = (unsigned long long)hw;
+                    }
+                }
+                else
+                {
+                    const int x = wd.s0 + (it - nhw);
+                    const int deg = s.si[x].deg;
+                    bool busy = false;
+                    if (s.sw_busy[x] || switch_has_input(s, x, sl))
+                    {
+                        if (deg <= KS)
+                            busy = switch_slot(s, x, sl, pend, wd.pend_n, work[wv], lane, 64);
+                        else
+                        {
+                            int b = 0;
+                            if (lane == 0)
+                                b = switch_step(s, x, sl, pend, wd.pend_n) ? 1 : 0;
+                            busy = __shfl(b, 0) != 0;
+                        }
+                    }
+                    if (lane == 0)
+                    {
+                        s.sw_busy[x] = busy ? 1 : 0;
+                        a |= busy ? 1u : 0u;
+                    }
+                }
+            }
+            if (a)
+                atomicOr(&s_act, a);
+            if (wk != ~0ull)
+                atomicMin(&s_wake, wk);
+            __syncthreads();
+            // this workgroup's share joins the team's before it arrives at the barrier
+            if (tid == 0)
+            {
+                if (s_act)
+                    atomicOr(&ts->act, s_act);
+                if (s_wake != ~0ull)
+                    atomicMin(&ts->wake, s_wake);
+            }
+            if (barrier([&] {
+                    slot_end(s, wd, r, pend, atomicOr(&ts->act, 0u), atomicMin(&ts->wake, ~0ull));
+                    ts->act = 0;
+                    ts->wake = ~0ull;
+                    if (wall_clock64() - t_launch > LAUNCH_TICKS)
+                        ts->stop = 1;
+                }))
+                return;
+        }
+        if (tid == 0 && rank == 0)
+            r.ticks += wall_clock64() - t_world;
     }
-    if (threadIdx.x == 0)
-        r.ticks += wall_clock64() - t0;
 }
 
-int launch_worlds(const Sim *sim, int nworlds, int threads, hipStream_t st)
+int launch_capacity(int threads)
 {
-    hipLaunchKernelGGL(rdma_worlds_kernel, dim3(nworlds), dim3(threads), 0, st, sim);
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rdma_team_kernel, threads, 0) != hipSuccess)
+        return -1;
+    return per_cu * cus;
+}
+
+int launch_worlds(const Sim *sim, const LaunchCfg *cfg_dev, int nblocks, int threads, hipStream_t st)
+{
+    hipLaunchKernelGGL(rdma_team_kernel, dim3(nblocks), dim3(threads), 0, st, sim, cfg_dev);
     return (int)hipGetLastError();
 }
 } // namespace rd

@@ -57,6 +57,7 @@ enum : int
     EV_DUE_OVF = 12,
     EV_DATA_DONE = 13,
     EV_LATE = 32,
+    EV_PKT = 38, // packet-events: frames sent (this build's; the reference's nout, src/rdma/rstep.cuh:1086, 2227)
 };
 
 enum : uint8_t
@@ -204,7 +205,8 @@ struct Port
 
 struct Switch
 {
-    uint32_t shared_used, total_hdrm, total_rsrv, pad;
+    uint32_t shared_used, total_hdrm, total_rsrv;
+    uint32_t max_shift; // the largest pfc shift of its ports (the lowest PFC threshold)
 };
 
 // a switch port's static set-up: its links, rate, headroom cap, PFC shift, ECN thresholds
@@ -341,6 +343,16 @@ RD_HD unsigned at_add32(unsigned *p, unsigned v)
 #else
     unsigned o = *p;
     *p = o + v;
+    return o;
+#endif
+}
+RD_HD unsigned at_sub32(unsigned *p, unsigned v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return atomicSub(p, v);
+#else
+    unsigned o = *p;
+    *p = o - v;
     return o;
 #endif
 }
@@ -1529,6 +1541,8 @@ RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, ui
         w = c.st.tc_at;
     c.st.wake = w;
     s.hs[h] = c.st;
+    if (c.nout)
+        at_add64(&s.ev[EV_PKT], (unsigned long long)c.nout);
     *wake = w;
     return act;
 }
@@ -1551,6 +1565,7 @@ struct SCtx
     const Chain *cur_src;
     uint32_t cur_code;
     bool cur_ok;
+    bool safe; // a safe slot: this context runs one egress port's events (see switch_slot)
     int ord;
 };
 RD_HD const Chain &sc_cur(SCtx &c)
@@ -1651,13 +1666,23 @@ RD_HD void s_start_q3(SCtx &c, int port, Pkt p)
     const Params &P = *c.P;
     Port &pt = c.pt[port];
     Port &in = c.pt[p.inport];
-    uint32_t from_hdrm = in.hdrm < p.size ? in.hdrm : p.size;
-    uint32_t sh = in.ing > RESERVE ? in.ing - RESERVE : 0;
-    uint32_t from_shared = p.size - from_hdrm < sh ? p.size - from_hdrm : sh;
-    in.hdrm -= from_hdrm;
-    in.ing -= p.size - from_hdrm;
-    c.sw.shared_used -= from_shared;
-    pt.egr -= p.size;
+    if (c.safe)
+    {
+        // no headroom anywhere, no port paused: RemoveFromIngressAdmission takes it from ing
+        at_sub32(&in.ing, (uint32_t)p.size);
+        pt.egr -= p.size;
+    }
+    else
+    {
+        // RemoveFromIngressAdmission, RemoveFromEgressAdmission
+        uint32_t from_hdrm = in.hdrm < p.size ? in.hdrm : p.size;
+        uint32_t sh = in.ing > RESERVE ? in.ing - RESERVE : 0;
+        uint32_t from_shared = p.size - from_hdrm < sh ? p.size - from_hdrm : sh;
+        in.hdrm -= from_hdrm;
+        in.ing -= p.size - from_hdrm;
+        c.sw.shared_used -= from_shared;
+        pt.egr -= p.size;
+    }
     if (P.ecn)
     {
         // SwitchMmu::ShouldSendCN
@@ -1676,7 +1701,7 @@ RD_HD void s_start_q3(SCtx &c, int port, Pkt p)
             count(s, EV_MARK);
         }
     }
-    if (in.mmu_paused)
+    if (!c.safe && in.mmu_paused)
     {
         uint32_t su = shared_used(in);
         if (in.hdrm == 0 && (su == 0 || su + RESUME_OFFSET <= pfc_thresh(c, p.inport)))
@@ -1735,7 +1760,14 @@ RD_HD void s_forward(SCtx &c, int inport, Pkt p, int out, int q)
     const Sim &s = *c.s;
     if (out < 0)
         return;
-    if (q != 0)
+    if (q != 0 && c.safe)
+    {
+        // below every threshold for the whole slot: no drop, no headroom, no PAUSE; only ing
+        // changes (shared_used follows from it at the slot's end)
+        at_add32(&c.pt[inport].ing, (uint32_t)p.size);
+        c.pt[out].egr += p.size;
+    }
+    else if (q != 0)
     {
         Port &in = c.pt[inport];
         const uint32_t th = pfc_thresh(c, inport);
@@ -1789,38 +1821,50 @@ RD_HD void sw_peek(const Sim &s, const SCtx &c, int p)
     }
 }
 
-// One switch's slot, every event in order on one lane (the reference's serial path,
-// src/rdma/rstep.cuh:3032-3189 and 3436-3446): candidates are each port's next arrival and its
-// TransmitComplete, in port order, arrival first; ties by the chains, undecided ties keep the
-// earlier candidate. Returns whether the switch is busy (sent, or holds frames).
-RD_HD bool switch_step(const Sim &s, int x, int64_t slot, uint32_t *pend, int pend_n)
+RD_HD void sw_ctx(SCtx &c, const Sim &s, int x, int64_t slot, uint32_t *pend, int pend_n, Port *pt, PortRun *w)
 {
     const SwInfo &si = s.si[x];
-    const int64_t t1 = (slot + 1) * s.D;
-    const int deg = si.deg;
-    const int w = s.s_world[x];
-    SCtx c;
     c.s = &s;
-    c.P = &s.Pw[w];
+    c.P = &s.Pw[s.s_world[x]];
     c.pend = pend;
     c.pend_n = pend_n;
     c.x = x;
     c.node = s.H + x;
-    c.pt = s.ps + si.pbase;
-    c.w = s.pr + si.pbase;
+    c.pt = pt;
+    c.w = w;
     c.tc = s.p_tc + si.pbase;
     c.pc = s.pc + si.pbase;
     c.sw = s.sw[x];
+    c.safe = false;
+    c.cur_ok = false;
+    c.ord = 0;
     c.now = slot * s.D;
+}
+// a port's slot set-up: its arrivals' cursor and its out bucket
+RD_HD void sw_port_open(const Sim &s, SCtx &c, int p, int64_t slot)
+{
+    PortRun &r = c.w[p];
+    r.ai = 0;
+    r.nout = 0;
+    r.an = bucket(s, c.pc[p].link_in, slot, &r.abase);
+    ob_open(r.ob, c.pc[p].link_out, c.pt[p].ob_last);
+}
+// a port's slot end: its bucket's count out; busy (sent, or holds frames)?
+RD_HD bool sw_port_close(const Sim &s, SCtx &c, int p, unsigned long long *nout)
+{
+    ob_flush(s, c.w[p].ob);
+    c.pt[p].ob_last = c.w[p].ob.last;
+    *nout += (unsigned long long)c.w[p].nout;
+    return c.w[p].nout > 0 || c.pt[p].q0_n > 0 || c.pt[p].q3_n > 0 || c.pt[p].tc_at >= 0;
+}
+
+// All of a switch's events of the slot in order on one lane (the reference's serial path,
+// src/rdma/rstep.cuh:3032-3189): candidates are each port's next arrival and its TransmitComplete,
+// in port order, arrival first; ties by the chains; undecided ties keep the earlier candidate.
+RD_HD void sw_serial_events(const Sim &s, SCtx &c, int deg, int64_t t1)
+{
     for (int p = 0; p < deg; p++)
-    {
-        PortRun &r = c.w[p];
-        r.ai = 0;
-        r.nout = 0;
-        r.an = bucket(s, c.pc[p].link_in, slot, &r.abase);
-        ob_open(r.ob, c.pc[p].link_out, c.pt[p].ob_last);
         sw_peek(s, c, p);
-    }
     for (;;)
     {
         int best = -1, bk = 0;
@@ -1879,20 +1923,288 @@ RD_HD bool switch_step(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
         else
         {
             int q = PG;
-            const int out = s_route(s, *c.P, x, pk, &q);
+            const int out = s_route(s, *c.P, c.x, pk, &q);
             s_forward(c, p, pk, out, q);
         }
     }
+}
+
+// One switch's slot on one lane, its ports in HBM: the serial path for any number of ports.
+// Returns whether the switch is busy.
+RD_HD bool switch_step(const Sim &s, int x, int64_t slot, uint32_t *pend, int pend_n)
+{
+    const SwInfo &si = s.si[x];
+    SCtx c;
+    sw_ctx(c, s, x, slot, pend, pend_n, s.ps + si.pbase, s.pr + si.pbase);
+    for (int p = 0; p < si.deg; p++)
+        sw_port_open(s, c, p, slot);
+    sw_serial_events(s, c, si.deg, (slot + 1) * s.D);
     s.sw[x] = c.sw;
     bool act = false;
-    for (int p = 0; p < deg; p++)
-    {
-        ob_flush(s, c.w[p].ob);
-        c.pt[p].ob_last = c.w[p].ob.last;
-        if (c.w[p].nout > 0 || c.pt[p].q0_n > 0 || c.pt[p].q3_n > 0 || c.pt[p].tc_at >= 0)
-            act = true;
-    }
+    unsigned long long nout = 0;
+    for (int p = 0; p < si.deg; p++)
+        act |= sw_port_close(s, c, p, &nout);
+    if (nout)
+        at_add64(&s.ev[EV_PKT], nout);
     return act;
+}
+
+// ------------------------------------------------------------------ a switch's slot by the lanes of a wave
+// A switch of at most KS ports runs its slot from a workspace (LDS on the GPU): its ports, its
+// arrivals' routes and times. The lanes load the ports and route the arrivals; then the slot is
+// either safe (src/rdma/rstep.cuh:2760-3030) - no port paused by the MMU, none with headroom bytes,
+// and every port below the lowest PFC threshold the slot can reach even if all its arrivals stay,
+// so that no MMU decision (drop, headroom, PAUSE, RESUME) can occur and egress ports interact
+// only through sums in their ingress counters - and each egress port's events (the arrivals routed
+// to it, merged from the ingress ports in event order, and its TransmitCompletes) run on a lane of
+// their own; or lane 0 runs all events in order. The same code runs on the CPU with one lane, port
+// after port (egress ports of a safe slot are independent, so the order of their runs is free).
+constexpr int KS = 32;  // ports of a switch whose slot runs from the workspace
+constexpr int LK = 512; // arrivals of a slot a safe slot takes
+struct SwWork
+{
+    Port pt[KS];
+    PortRun w[KS];
+    int cnt[KS], first[KS];
+    uint32_t abytes[KS], ing0[KS];
+    int16_t mc[KS][KS]; // [egress][ingress]: the ingress's next arrival for the egress (-1: none)
+    int16_t onext[LK];  // an arrival's next one of the same ingress and egress (-1: none)
+    int64_t ats[LK];
+    int8_t own[LK];     // the egress port it belongs to in a safe slot (-1: none)
+    uint8_t q[LK];
+    int total, safe;
+    unsigned act;
+    unsigned nout;
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RD_WAVE_SYNC()                                                                                                            \
+    do                                                                                                                            \
+    {                                                                                                                             \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");                                                                    \
+        __builtin_amdgcn_wave_barrier();                                                                                          \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");                                                                    \
+    } while (0)
+#else
+#define RD_WAVE_SYNC() \
+    do                 \
+    {                  \
+    } while (0)
+#endif
+
+// the arrival code and chain of arrival j (of ingress p) in the workspace
+RD_HD const LFrame &sw_arr(const Sim &s, const SwWork &k, int p, int j) { return s.msg[k.w[p].abase + (uint64_t)(j - k.first[p])]; }
+RD_HD uint32_t sw_acode(const Sim &s, const SCtx &c, int p, const LFrame &f)
+{
+    const int lin = c.pc[p].link_in;
+    return ccode(link_tx(s, lin, f.p.size) + s.li[lin].delay, f.p.ord);
+}
+
+// egress port o's events of a safe slot (src/rdma/rstep.cuh:2806-3005)
+RD_HD void sw_egress_events(const Sim &s, SCtx &c, SwWork &k, int o, int deg, int64_t t1)
+{
+    for (;;)
+    {
+        // the next arrival for o: the ingress heads in port order, the earlier (time, chains) kept
+        int bp = -1, kb = -1;
+        for (int p = 0; p < deg; p++)
+        {
+            const int j = k.mc[o][p];
+            if (j < 0)
+                continue;
+            if (bp < 0)
+            {
+                bp = p, kb = j;
+                continue;
+            }
+            bool b4;
+            if (k.ats[j] != k.ats[kb])
+                b4 = k.ats[j] < k.ats[kb];
+            else
+            {
+                const LFrame &fa = sw_arr(s, k, p, j), &fb = sw_arr(s, k, bp, kb);
+                b4 = before_q(s, k.ats[j], sw_acode(s, c, p, fa), fa.c, sw_acode(s, c, bp, fb), fb.c);
+            }
+            if (b4)
+                bp = p, kb = j;
+        }
+        const int64_t ta = kb >= 0 ? k.ats[kb] : INT64_MAX, tc = k.pt[o].tc_at;
+        const bool htc = tc >= 0 && tc < t1;
+        if (ta == INT64_MAX && !htc)
+            break;
+        bool arr;
+        const LFrame *fa = kb >= 0 ? &sw_arr(s, k, bp, kb) : nullptr;
+        const uint32_t acode = fa ? sw_acode(s, c, bp, *fa) : 0;
+        if (!htc || ta == INT64_MAX)
+            arr = !htc;
+        else if (ta != tc)
+            arr = ta < tc;
+        else
+            arr = before_q(s, ta, acode, fa->c, 0, c.tc[o]);
+        c.ord = 0;
+        c.cur_ok = false;
+        if (!arr)
+        {
+            c.now = tc;
+            c.cur_src = &c.tc[o];
+            c.cur_code = 0;
+            sc_cur(c);
+            k.pt[o].tc_at = -1;
+            s_dequeue(c, o);
+            continue;
+        }
+        k.mc[o][bp] = k.onext[kb];
+        c.now = ta;
+        c.cur_src = &fa->c;
+        c.cur_code = acode;
+        const Pkt pk = fa->p;
+        if (pk.kind == K_PAUSE || pk.kind == K_RESUME)
+        {
+            if (pk.kind == K_PAUSE)
+                k.pt[o].paused |= 1u << pk.q;
+            else
+            {
+                k.pt[o].paused &= ~(1u << pk.q);
+                s_dequeue(c, o);
+            }
+        }
+        else
+            s_forward(c, bp, pk, o, k.q[kb]);
+    }
+}
+
+// One switch's slot by lane `lane` of `nlanes` (a wave on the GPU, one lane on the CPU); ports
+// p = lane, lane + nlanes, ... are this lane's. Returns (on every lane) whether the switch is busy.
+RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pend_n, SwWork &k, int lane, int nlanes)
+{
+    const SwInfo &si = s.si[x];
+    const int deg = si.deg;
+    const int64_t t1 = (slot + 1) * s.D;
+    SCtx c;
+    sw_ctx(c, s, x, slot, pend, pend_n, k.pt, k.w);
+    // ports in, arrivals counted
+    for (int p = lane; p < deg; p += nlanes)
+    {
+        k.pt[p] = s.ps[si.pbase + p];
+        k.ing0[p] = k.pt[p].ing;
+        k.abytes[p] = 0;
+        sw_port_open(s, c, p, slot);
+        k.cnt[p] = k.w[p].an;
+    }
+    if (lane == 0)
+    {
+        k.act = 0;
+        k.nout = 0;
+    }
+    RD_WAVE_SYNC();
+    int total = 0;
+    for (int p = 0; p < deg; p++)
+        total += k.cnt[p];
+    // routes and times of the arrivals (a safe slot's candidates), per ingress port
+    if (total <= LK)
+        for (int p = lane; p < deg; p += nlanes)
+        {
+            int f0 = 0;
+            for (int r = 0; r < p; r++)
+                f0 += k.cnt[r];
+            k.first[p] = f0;
+            for (int i = 0; i < k.cnt[p]; i++)
+            {
+                const int j = f0 + i;
+                const LFrame &f = s.msg[k.w[p].abase + i];
+                k.ats[j] = f.ts;
+                int q = 0;
+                if (f.p.kind == K_PAUSE || f.p.kind == K_RESUME)
+                    k.own[j] = (int8_t)p;
+                else
+                {
+                    const int out = s_route(s, *c.P, x, f.p, &q);
+                    k.own[j] = (int8_t)out;
+                    if (out >= 0 && q != 0)
+                        k.abytes[p] += f.p.size;
+                }
+                k.q[j] = (uint8_t)q;
+            }
+            for (int o = 0; o < deg; o++)
+                k.mc[o][p] = -1;
+            for (int j = f0 + k.cnt[p] - 1; j >= f0; j--)
+            {
+                const int o = k.own[j];
+                if (o < 0)
+                    continue;
+                k.onext[j] = k.mc[o][p];
+                k.mc[o][p] = (int16_t)j;
+            }
+        }
+    RD_WAVE_SYNC();
+    // is the slot safe? (every lane decides alike)
+    bool safe = total <= LK;
+    if (safe)
+    {
+        uint64_t sum = 0;
+        for (int p = 0; p < deg && safe; p++)
+        {
+            sum += k.abytes[p];
+            safe = !k.pt[p].hdrm && !k.pt[p].mmu_paused;
+        }
+        // (GetPfcThreshold is uint32_t: with less buffer than headroom and reserves the difference
+        // wraps to about 2^32 and stays there, so such a switch never pauses)
+        const int64_t amax = (int64_t)c.sw.shared_used + (int64_t)sum;
+        const int64_t room = (int64_t)(uint32_t)(c.P->buffer_size - c.sw.total_hdrm - c.sw.total_rsrv) - amax;
+        const int64_t th_min = room > 0 ? room >> c.sw.max_shift : -1;
+        for (int p = 0; p < deg && safe; p++)
+        {
+            const int64_t su = (int64_t)k.pt[p].ing + k.abytes[p] - RESERVE;
+            safe = (su > 0 ? su : 0) < th_min;
+        }
+    }
+    if (safe)
+    {
+        c.safe = true;
+        for (int o = lane; o < deg; o += nlanes)
+            sw_egress_events(s, c, k, o, deg, t1);
+    }
+    else if (lane == 0)
+    {
+        sw_serial_events(s, c, deg, t1);
+        s.sw[x] = c.sw;
+    }
+    RD_WAVE_SYNC();
+    // ports out
+    unsigned long long nout = 0;
+    bool act = false;
+    for (int p = lane; p < deg; p += nlanes)
+    {
+        act |= sw_port_close(s, c, p, &nout);
+        s.ps[si.pbase + p] = k.pt[p];
+    }
+    if (act)
+        at_add32(&k.act, 1u);
+    if (nout)
+        at_add32(&k.nout, (unsigned)nout);
+    RD_WAVE_SYNC();
+    if (lane == 0)
+    {
+        if (safe)
+        {
+            // shared_used = sum over ports of max(0, ing - reserve), which changed with ing only
+            // (modulo 2^32, as the counter)
+            uint32_t d = 0;
+            for (int p = 0; p < deg; p++)
+            {
+                const uint32_t a = k.ing0[p] > RESERVE ? k.ing0[p] - RESERVE : 0, b = k.pt[p].ing > RESERVE ? k.pt[p].ing - RESERVE : 0;
+                d += b - a;
+            }
+            Switch sw = s.sw[x];
+            sw.shared_used += d;
+            s.sw[x] = sw;
+        }
+        if (k.nout)
+            at_add64(&s.ev[EV_PKT], (unsigned long long)k.nout);
+    }
+    const bool busy = k.act != 0;
+    RD_WAVE_SYNC(); // (the workspace is reused by the next switch)
+    return busy;
 }
 
 // does the switch have events in this slot? (busy last slot, or arrivals; src/rdma/rstep.cuh:3859-3881)

@@ -290,6 +290,48 @@ def config5_requests(outdir, n_requests=1000, seed=5, rate_per_s=100.0, ks=(4, 8
 CONFIG2_ARGS = ["--topo=graph", "--spray=1", "--RngRun=1", "--fin=1", "--fail=300000:s256:s128,700000:s256:s128:up"]
 
 
+def config3_flows(seed=3, leaves=32, hpl=16, load=0.4, duration_ns=1_000_000, median=50_000, sigma=1.5, incast_n=160,
+                  incast_bytes=64 * 1024, incast_ns=500_000):
+    """Config 3's flows on the 512-host leaf-spine: per-host Poisson all-to-all at `load` over
+    `duration_ns` with LogNormal(median, sigma) sizes clipped to [1 KB, 10 MB], then a 160-to-1
+    incast of 64 KB messages at 500 us from senders outside the receiver's leaf (the receiver drawn
+    by the seed). Rows `src,dst,bytes,start_ns` after a header, background first (by start)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    H = leaves * hpl
+    mean = median * np.exp(sigma ** 2 / 2)
+    rate = load * LINK_BPS / 8 / mean / 1e9  # flows per ns per host
+    rows = []
+    for h in range(H):
+        t = 0.0
+        while True:
+            t += rng.exponential(1 / rate)
+            if t >= duration_ns:
+                break
+            d = int(rng.integers(0, H - 1))
+            d += d >= h
+            size = int(np.clip(round(rng.lognormal(np.log(median), sigma)), 1000, 10_000_000))
+            rows.append((int(t), h, d, size))
+    rows.sort()
+    recv = int(rng.integers(0, H))
+    others = [h for h in range(H) if h // hpl != recv // hpl]
+    senders = rng.choice(others, size=incast_n, replace=False)
+    lines = ["src,dst,flow_size,start_ns"] + [f"{a},{b},{n},{t}" for t, a, b, n in rows]
+    lines += [f"{int(a)},{recv},{incast_bytes},{incast_ns}" for a in senders]
+    return "\n".join(lines) + "\n"
+
+
+def write_config3(outdir):
+    """Writes config 3's topology and flows; returns (topology path, flows path, [(name, overrides)])."""
+    os.makedirs(outdir, exist_ok=True)
+    files = {"ls32.txt": leafspine_topology(32, 16, 16), "config3_flows.csv": config3_flows()}
+    for name, text in files.items():
+        p = os.path.join(outdir, name)
+        if not os.path.exists(p) or open(p).read() != text:
+            with open(p, "w") as f:
+                f.write(text)
+    return os.path.join(outdir, "ls32.txt"), os.path.join(outdir, "config3_flows.csv"), [("hpcc", ["CC_MODE=3"]), ("dcqcn", ["CC_MODE=1"])]
+
+
 def write_configs(outdir):
     """Writes the config-1 and config-2 inputs into outdir; returns {name: (flows, args)}."""
     os.makedirs(outdir, exist_ok=True)
