@@ -23,7 +23,17 @@ with the oracle's; the plain per-world launch rate (each world alone, one after 
 measured on every 8th world; the CPU baseline is the oracle on every world, on all the host cores
 given, worlds in parallel.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--sweep 0|1]
+RDMA (config 3, the "rdma" object): `twins-rdma` on the 512-host leaf-spine (32 leaves x 16
+spines, 16 hosts per leaf, 100 Gb/s, 1 us), the reference's HPCC configuration, HPCC (CC_MODE 3)
+and DCQCN (CC_MODE 1) as two worlds of one batch; background all-to-all LogNormal(50 KB, 1.5) at
+40% load for 1 ms and a 160-to-1 incast of 64 KB at 500 us; port queues of 16,384 frames (the
+default 4,096 overflows under the incast, which the reference reports as inexact). A step is one
+complete run of both worlds. Parity: every flow's completion time against the reference's own RDMA
+model built for the CPU (oracle/_ref, src/rdma compiled with TWINS_CPU), which is also the CPU
+baseline (the two worlds on two cores at once, its event loop's wall time). roofline: 480 B per
+packet-event (a 176-byte link frame written and read, a 64-byte queue entry written and read).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--sweep 0|1] [--serve 0|1] [--rdma 0|1]
 """
 import argparse
 import concurrent.futures as cf
@@ -244,6 +254,79 @@ def serve_leg(a):
     }
 
 
+RDMA_BYTES_PER_EVENT = 480
+RDMA_Q3CAP = 16384
+
+
+def rdma_leg(a):
+    """Config 3: the RDMA model on the 512-host leaf-spine; returns the "rdma" object of the JSON line."""
+    from twins_amd import rdma, workloads
+    topo, flows, runs = workloads.write_config3(os.path.join(ROOT, "tests", "golden", "configs"))
+    cfg = os.path.join(ROOT, "tests", "golden", "rdma", "inputs", "hpcc.cfg")
+
+    def step():
+        ws = [rdma.World(cfg, topo, flows, kv) for _, kv in runs]
+        b = rdma.Batch(ws, q3cap=RDMA_Q3CAP)
+        st = b.run()
+        return ws, b, st
+
+    log(f"rdma: config 3, {len(runs)} worlds, {a.warmup} warm-up + {a.steps} steps")
+    for _ in range(a.warmup):
+        step()
+    wall, kern, events = 0.0, 0.0, 0
+    for _ in range(a.steps):
+        ws, b, st = step()
+        wall += st.wall_s
+        kern += st.kernel_s
+        events += b.counters()[38]
+    ev = b.counters()
+    inexact = [rdma.EV_NAMES[i] for i in rdma.BAD if ev[i]]
+    per_step = events // a.steps
+    # the reference's RDMA model built for the CPU: parity target and CPU baseline (both worlds at once)
+    ref = os.path.join(ROOT, "oracle", "_ref", "twins-rdma-cpu")
+    same, total, cpu = 0, 0, None
+    if os.path.exists(ref):
+        procs = []
+        for name, kv in runs:
+            out = f"/tmp/bench_rdma_ref_{name}.fct"
+            procs.append((out, subprocess.Popen([ref, cfg, f"--topo={topo}", f"--flows={flows}", f"--out={out}", "--gwin=0",
+                                                 f"--q3cap={RDMA_Q3CAP}"] + kv, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+        walls = []
+        for i, (out, p) in enumerate(procs):
+            so, se = p.communicate(timeout=600)
+            m = re.search(r"in ([0-9.]+) s wall", so)
+            walls.append(float(m.group(1)) if m else float("nan"))
+            want = open(out).read().splitlines()
+            got = b.fct_text(i).splitlines()
+            same += sum(x == y for x, y in zip(want, got)) if len(want) == len(got) else 0
+            total += ws[i].flows
+            log(f"rdma: reference CPU build, {runs[i][0]}: {walls[-1]:.2f} s; {len(want)} flows done")
+        cpu = {"value": round(per_step / max(walls), 1), "unit": "packet-events/s", "cores": len(runs), "kind": "reference",
+               "sample": "the reference's RDMA model built for the CPU (oracle/_ref), the two worlds of one step as two processes "
+                         "at once, its event loop's wall time (the longer of the two)"}
+    else:
+        log("rdma: oracle/_ref/twins-rdma-cpu is not built: no parity check or CPU baseline")
+    achieved = RDMA_BYTES_PER_EVENT * events / kern / 1e9
+    return {
+        "metric": "simulated packet-events/s at bit-exact FCT",
+        "value": round(events / wall, 1),
+        "unit": "packet-events/s",
+        "steps": a.steps,
+        "ms_per_step": round(1000 * wall / a.steps, 3),
+        "config": {"workload": "config 3: twins-rdma, leaf-spine 32 x 16 (512 hosts, 48 switches of 32 ports), 100 Gb/s 1 us; "
+                               "16,768 LogNormal(50 KB, 1.5) all-to-all flows at 40% load over 1 ms + 160-to-1 incast of 64 KB at "
+                               "500 us; HPCC (CC_MODE 3) and DCQCN (CC_MODE 1) as 2 worlds of one batch; q3cap 16384",
+                   "worlds": len(runs), "packet_events_per_step": per_step, "slots_max": st.slots, "flows_done": st.flows_done,
+                   "inexact_counters": inexact},
+        "parity": {"flows_identical_to_reference": same, "flows": total,
+                   "reference": "src/rdma (the reference's model) compiled for the CPU: oracle/_ref/twins-rdma-cpu"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "rdma_team_kernel",
+                     "kernel_ms_per_step": round(1000 * kern / a.steps, 3), "bytes_per_event": RDMA_BYTES_PER_EVENT},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +344,7 @@ def main():
     ap.add_argument("--serve_requests", type=int, default=1000)
     ap.add_argument("--serve_rate", type=float, default=100.0)
     ap.add_argument("--serve_check", type=int, default=20, help="every n-th request is checked against the oracle")
+    ap.add_argument("--rdma", type=int, default=1, help="also run config 3, the RDMA model (the \"rdma\" object)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU (the worlds of a sweep are independent and would be "
@@ -348,8 +432,14 @@ def main():
         line["sweep"] = sweep_leg(a)
     if a.serve:
         line["serve"] = serve_leg(a)
+    if a.rdma:
+        line["rdma"] = rdma_leg(a)
     print(json.dumps(line), flush=True)
     bad = [k for k in ("sweep", "serve") if k in line and line[k]["parity"]["flows_identical_to_oracle"] != line[k]["parity"]["flows"]]
+    if "rdma" in line:
+        r = line["rdma"]
+        if r["parity"]["flows_identical_to_reference"] != r["parity"]["flows"] or r["config"]["inexact_counters"]:
+            bad.append("rdma")
     if same != total or bad:
         sys.exit(1)
 

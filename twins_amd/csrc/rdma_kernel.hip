@@ -126,8 +126,14 @@ __global__ void __launch_bounds__(256) rdma_team_kernel(const Sim *__restrict__ 
             const int64_t sl = s_slot;
             unsigned a = 0;
             unsigned long long wk = ~0ull;
+#ifdef RD_PROF
+            long long p0 = wall_clock64(), ph = 0, psw = 0;
+#endif
             for (int it = gw; it < items; it += TW)
             {
+#ifdef RD_PROF
+                const long long q0 = wall_clock64();
+#endif
                 if (it < nhw)
                 {
                     const int e = it * 64 + lane;
@@ -162,7 +168,22 @@ __global__ void __launch_bounds__(256) rdma_team_kernel(const Sim *__restrict__ 
                         a |= busy ? 1u : 0u;
                     }
                 }
+#ifdef RD_PROF
+                (it < nhw ? ph : psw) += wall_clock64() - q0;
+#endif
             }
+#ifdef RD_PROF
+            // (profiling builds: the waves' host and switch item time, and the time to the slot's end)
+            if (lane == 0)
+            {
+                if (ph)
+                    atomicAdd(&s.ev[33], (unsigned long long)ph);
+                if (psw)
+                    atomicAdd(&s.ev[34], (unsigned long long)psw);
+            }
+            const long long pb = wall_clock64();
+            (void)p0;
+#endif
             if (a)
                 atomicOr(&s_act, a);
             if (wk != ~0ull)
@@ -184,6 +205,10 @@ __global__ void __launch_bounds__(256) rdma_team_kernel(const Sim *__restrict__ 
                         ts->stop = 1;
                 }))
                 return;
+#ifdef RD_PROF
+            if (tid == 0 && rank == 0)
+                atomicAdd(&s.ev[35], (unsigned long long)(wall_clock64() - pb));
+#endif
         }
         if (tid == 0 && rank == 0)
             r.ticks += wall_clock64() - t_world;

@@ -2158,6 +2158,10 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
             safe = (su > 0 ? su : 0) < th_min;
         }
     }
+#ifdef RD_PROF
+    if (lane == 0)
+        at_add64(&s.ev[safe ? 36 : 37], 1ull); // (profiling builds: safe and serial switch slots)
+#endif
     if (safe)
     {
         c.safe = true;
