@@ -474,7 +474,8 @@ struct trd_batch
     vector<int64_t> start; // [F] flows' starts, by global id
     int threads = 256;
     int team_blocks = 0; // 0: from the worlds' sizes
-    vector<int> items;   // per world: work items per slot (host items of 64 hosts, switches)
+    int hpw = 0;         // hosts per host item (0: from the worlds' sizes)
+    vector<int> hs, ss;  // per world: hosts, switches
     int *queue_d = nullptr;
     unsigned *qnext_d = nullptr;
     rd::TeamState *teams_d = nullptr;
@@ -806,11 +807,13 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
     for (int f = 0; f < F; f++)
         b.start[f] = fi[f].start;
     b.nf.resize(W);
-    b.items.resize(W);
+    b.hs.resize(W);
+    b.ss.resize(W);
     for (int wi = 0; wi < W; wi++)
     {
         b.nf[wi] = (int)worlds[wi]->fi.size();
-        b.items[wi] = (worlds[wi]->H + 63) / 64 + worlds[wi]->S;
+        b.hs[wi] = worlds[wi]->H;
+        b.ss[wi] = worlds[wi]->S;
     }
     // launches: the queue of worlds, the teams' barrier state
     const int cap = rd::launch_capacity(b.threads);
@@ -838,6 +841,7 @@ extern "C" int trd_batch_create(trd_world *const *worlds, int32_t n, const trd_p
     std::unique_ptr<trd_batch> b(new trd_batch);
     b->threads = 256;
     b->team_blocks = c.team_blocks;
+    b->hpw = c.hosts_per_wave > 0 ? std::min(c.hosts_per_wave, 64) : 0;
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
         hipEventCreate(&b->e1) != hipSuccess)
         return fail(TW_EDEVICE, "hipStreamCreate failed");
@@ -858,23 +862,28 @@ extern "C" int trd_batch_run(trd_batch *b, trd_run_stats *st)
     int launches = 0;
     double kernel_s = 0;
     int64_t prev = -1;
-    // teams: one size for the launch, from the largest world's work items (about one per wave, at
-    // most 16 workgroups), as many teams as the GPU holds at once
+    // the launch shape: hosts per host item (fewer: less divergence among a wave's hosts, more
+    // waves), and one team size, from the largest world's work items (about one per wave, at most 32
+    // workgroups); as many teams as the GPU holds at once
     const int cap = b->max_teams;
+    const int hpw = b->hpw > 0 ? b->hpw : (b->W <= 8 ? 16 : 64);
+    vector<int> items(b->W);
+    for (int i = 0; i < b->W; i++)
+        items[i] = (b->hs[i] + hpw - 1) / hpw + b->ss[i];
     int T = b->team_blocks;
     if (T <= 0)
     {
         int mx = 1;
-        for (int v : b->items)
+        for (int v : items)
             mx = std::max(mx, v);
-        T = std::min(16, (mx + 3) / 4);
+        T = std::min(32, (mx + 3) / 4);
     }
     T = std::max(1, std::min(T, cap));
     // worlds by work items, largest first
     vector<int> order(b->W);
     for (int i = 0; i < b->W; i++)
         order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->items[a] > b->items[c]; });
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return items[a] > items[c]; });
     for (;;)
     {
         vector<int> queue;
@@ -884,6 +893,7 @@ extern "C" int trd_batch_run(trd_batch *b, trd_run_stats *st)
         int nteams = std::min<int>((int)queue.size(), cap / T);
         rd::LaunchCfg L = {};
         L.T = T;
+        L.hpw = hpw;
         L.nq = (int)queue.size();
         if (nteams >= 8)
         {

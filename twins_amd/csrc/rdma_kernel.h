@@ -18,7 +18,7 @@ struct TeamState
 // one launch: teams of T workgroups take worlds queue[0 .. nq) in order (qnext: the next index)
 struct LaunchCfg
 {
-    int T, nq, xcd_local, pad;
+    int T, nq, xcd_local, hpw; // hpw: hosts of a host item (side by side on a wave's lanes, at most 64)
     const int *queue;
     unsigned *qnext;
     TeamState *teams;

@@ -1,6 +1,6 @@
 // The RDMA slot kernel for gfx950. A world runs on a team of T workgroups (256 threads, 4 waves
 // each) for all of its slots; teams take worlds from a queue, longest first. In a slot the world's
-// work items are dealt over the team's waves: a host item is up to 64 hosts side by side (a lane
+// work items are dealt over the team's waves: a host item is up to hpw hosts side by side (a lane
 // runs one host's events, src/rdma/rstep.cuh:1649-2112), a switch item is one switch on one wave
 // (its ports in LDS; a safe slot runs each egress port's events on a lane of its own, otherwise lane
 // 0 runs them all; rdma_model.h switch_slot). Entities only read what earlier slots filed
@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(256) rdma_team_kernel(const Sim *__restrict__ 
         const World wd = s.wd[w];
         WorldRun &r = s.run[w];
         uint32_t *pend = s.pend + wd.pend_off;
-        const int nhw = (wd.nh + 63) / 64, items = nhw + wd.ns, TW = T * WAVES, gw = rank * WAVES + wv;
+        const int hpw = L.hpw, nhw = (wd.nh + hpw - 1) / hpw, items = nhw + wd.ns, TW = T * WAVES, gw = rank * WAVES + wv;
         const long long t_world = wall_clock64();
         for (;;)
         {
@@ -136,8 +136,8 @@ __global__ void __launch_bounds__(256) rdma_team_kernel(const Sim *__restrict__ 
 #endif
                 if (it < nhw)
                 {
-                    const int e = it * 64 + lane;
-                    if (e < wd.nh)
+                    const int e = it * hpw + lane;
+                    if (lane < hpw && e < wd.nh)
                     {
                         int64_t hw = INT64_MAX;
                         a |= host_step(s, wd.h0 + e, sl, &r.flows_done, pend, wd.pend_n, &hw) ? 1u : 0u;

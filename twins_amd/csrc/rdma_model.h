@@ -178,6 +178,14 @@ struct Route
     uint32_t off, n;
 };
 
+// an entity's share of the tail ring: positions next .. end - 1 reserved for it (taken from the
+// shared counter TAIL_CHUNK at a time, so that a chain's spill waits for no global atomic)
+struct TailAlloc
+{
+    uint64_t next, end;
+};
+constexpr uint64_t TAIL_CHUNK = 16;
+
 // a NIC (QbbNetDevice + RdmaEgressQueue state) and the host's flow starts
 struct Host
 {
@@ -189,6 +197,7 @@ struct Host
     int64_t wake;    // its earliest event without input
     int64_t ob_last; // its out link's last bucket: slot << 16 | frames
     int64_t due_min; // at most its queue pairs' earliest DCQCN timer
+    TailAlloc ta;
 };
 
 struct Port
@@ -201,6 +210,7 @@ struct Port
     uint32_t ing, hdrm, egr, mmu_paused; // SwitchMmu, this port as ingress and as egress
     int64_t ob_last;
     double rng[6]; // its ECN marking stream (MRG32k3a)
+    TailAlloc ta;
 };
 
 struct Switch
@@ -243,6 +253,8 @@ struct OutBucket
     uint32_t cnt, n;
     int link;
     int64_t last;
+    uint32_t txs[2]; // its last two frame sizes and their tx times (a table lookup saved)
+    int32_t txv[2];
 };
 
 // a switch port's cursor over its slot's arrivals and its out bucket (per port, in HBM)
@@ -394,6 +406,8 @@ RD_HD int64_t link_tx(const Sim &s, int link, uint32_t bytes)
         return s.txt[(size_t)l.rc * TXT_N + bytes];
     return tx_ns(l.bps, bytes);
 }
+// link_tx for a sender's out link, remembering its last two sizes (a sender sends few sizes)
+RD_HD int64_t link_txo(const Sim &s, struct OutBucket &o, uint32_t bytes);
 RD_HD uint64_t rate_of(double d) { return (uint64_t)d; }
 RD_HD int ring_add(int a, int b, int n)
 {
@@ -416,6 +430,18 @@ RD_HD void ob_open(OutBucket &o, int link, int64_t last)
     o.hi = 0;
     o.link = link;
     o.last = last;
+    o.txs[0] = o.txs[1] = 0;
+}
+RD_HD int64_t link_txo(const Sim &s, OutBucket &o, uint32_t bytes)
+{
+    if (bytes == o.txs[0])
+        return o.txv[0];
+    if (bytes == o.txs[1])
+        return o.txv[1];
+    const int32_t t = (int32_t)link_tx(s, o.link, bytes);
+    o.txs[1] = o.txs[0], o.txv[1] = o.txv[0];
+    o.txs[0] = bytes, o.txv[0] = t;
+    return t;
 }
 RD_HD void ob_flush(const Sim &s, OutBucket &o)
 {
@@ -467,7 +493,7 @@ RD_HD uint64_t chash(uint64_t h, uint32_t code)
 }
 // the chain of an event scheduled with `code` by the event of chain par; the oldest run spills to
 // the shared tail ring when the inline runs are full
-RD_HD void cpush(const Sim &s, Chain *dst, uint32_t code, const Chain &par)
+RD_HD void cpush(const Sim &s, TailAlloc &ta, Chain *dst, uint32_t code, const Chain &par)
 {
     Chain c;
     if (par.r[0].code == code)
@@ -487,7 +513,12 @@ RD_HD void cpush(const Sim &s, Chain *dst, uint32_t code, const Chain &par)
         c.pad = 0;
         if (par.r[RUNS - 1].code)
         {
-            uint64_t pos = at_add64(s.tail_cnt, 1ull) + 1;
+            if (ta.next == ta.end)
+            {
+                ta.next = at_add64(s.tail_cnt, TAIL_CHUNK) + 1;
+                ta.end = ta.next + TAIL_CHUNK;
+            }
+            const uint64_t pos = ta.next++;
             TailNode &n = s.tail[(pos - 1) & s.tail_mask];
             n.code = par.r[RUNS - 1].code;
             n.cnt = par.r[RUNS - 1].cnt;
@@ -920,9 +951,9 @@ RD_HD bool win_bound(const Params &P, const SndQP &q)
 RD_HD void h_transmit(HCtx &c, Pkt p)
 {
     const Sim &s = *c.s;
-    int64_t tx = link_tx(s, c.link, p.size);
+    int64_t tx = link_txo(s, c.ob, p.size);
     c.st.tc_at = c.now + tx;
-    cpush(s, &s.h_tc[c.h], ccode(tx, c.ord++), c.cur);
+    cpush(s, c.st.ta, &s.h_tc[c.h], ccode(tx, c.ord++), c.cur);
     p.ord = (uint8_t)(c.ord < 255 ? c.ord : 255);
     c.ord++;
     c.nout++;
@@ -937,7 +968,7 @@ RD_HD void h_transmit(HCtx &c, Pkt p)
 RD_HD void h_schedule_ns(HCtx &c, int64_t at)
 {
     c.st.ns_at = at;
-    cpush(*c.s, &c.s->h_ns[c.h], ccode(at - c.now, c.ord++), c.cur);
+    cpush(*c.s, c.st.ta, &c.s->h_ns[c.h], ccode(at - c.now, c.ord++), c.cur);
 }
 // RdmaEgressQueue::GetNextQindex: -1 the ACK queue, -1024 none, else the QP's place in the group;
 // finished queue pairs leave the group here (src/rdma/rstep.cuh:1106-1144)
@@ -1123,10 +1154,10 @@ RD_HD void dcqcn_cnp(HCtx &c, int f, SndQP &q)
         q.alpha_cnp = 0;
         q.t_alpha = c.now + c.P->alpha_int;
         due_note(c, q.t_alpha);
-        cpush(s, &s.qc[f].alpha, ccode(c.P->alpha_int, c.ord++), c.cur);
+        cpush(s, c.st.ta, &s.qc[f].alpha, ccode(c.P->alpha_int, c.ord++), c.cur);
         q.t_dec = c.now + c.P->dec_int + 1;
         due_note(c, q.t_dec);
-        cpush(s, &s.qc[f].dec, ccode(c.P->dec_int + 1, c.ord++), c.cur);
+        cpush(s, c.st.ta, &s.qc[f].dec, ccode(c.P->dec_int + 1, c.ord++), c.cur);
         q.target = q.rate = rate_of((double)q.rate * c.P->rate_first_cnp);
         q.first_cnp = 0;
     }
@@ -1185,13 +1216,13 @@ RD_HD void dcqcn_timer(HCtx &c, int f, int kind)
         q.alpha_cnp = 0;
         q.t_alpha = c.now + P.alpha_int;
         due_note(c, q.t_alpha);
-        cpush(s, &s.qc[f].alpha, ccode(P.alpha_int, c.ord++), c.cur);
+        cpush(s, c.st.ta, &s.qc[f].alpha, ccode(P.alpha_int, c.ord++), c.cur);
     }
     else if (kind == 1)
     {
         q.t_dec = c.now + P.dec_int;
         due_note(c, q.t_dec);
-        cpush(s, &s.qc[f].dec, ccode(P.dec_int, c.ord++), c.cur);
+        cpush(s, c.st.ta, &s.qc[f].dec, ccode(P.dec_int, c.ord++), c.cur);
         if (q.dec_cnp)
         {
             if (P.clamp || q.rp_stage != 0)
@@ -1202,14 +1233,14 @@ RD_HD void dcqcn_timer(HCtx &c, int f, int kind)
             q.dec_cnp = 0;
             q.t_rp = c.now + P.rp_int;
             due_note(c, q.t_rp);
-            cpush(s, &s.qc[f].rp, ccode(P.rp_int, c.ord++), c.cur);
+            cpush(s, c.st.ta, &s.qc[f].rp, ccode(P.rp_int, c.ord++), c.cur);
         }
     }
     else
     {
         q.t_rp = c.now + P.rp_int;
         due_note(c, q.t_rp);
-        cpush(s, &s.qc[f].rp, ccode(P.rp_int, c.ord++), c.cur);
+        cpush(s, c.st.ta, &s.qc[f].rp, ccode(P.rp_int, c.ord++), c.cur);
         if (q.rp_stage >= P.rpg_threshold)
         {
             q.target += q.rp_stage == P.rpg_threshold ? P.rai : P.rhai;
@@ -1394,6 +1425,8 @@ RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, ui
     c.ldelay = s.li[c.link].delay;
     ob_open(c.ob, c.link, c.st.ob_last);
     const int64_t in_delay = s.li[lin].delay;
+    uint32_t in_sz = 0; // the in-link's last frame size and its tx time
+    int64_t in_tx = 0;
     // the DCQCN timers due in this slot, in event order (their intervals are at least a slot: none
     // set in it falls due in it)
     Due *due = s.due + (size_t)h * MAX_DUE;
@@ -1434,7 +1467,12 @@ RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, ui
     for (;;)
     {
         const LFrame *af = ai < an ? &s.msg[abase + ai] : nullptr;
-        const uint32_t acode = af ? ccode(link_tx(s, lin, af->p.size) + in_delay, af->p.ord) : 0;
+        if (af && af->p.size != in_sz)
+        {
+            in_sz = af->p.size;
+            in_tx = link_tx(s, lin, in_sz);
+        }
+        const uint32_t acode = af ? ccode(in_tx + in_delay, af->p.ord) : 0;
         while (di < ndue && due_now(s, due[di]) != due[di].at)
             di++; // cancelled or moved
         if (sf_at != c.st.next_start)
@@ -1479,7 +1517,7 @@ RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, ui
             count(s, EV_LATE);
         c.now = bt;
         if (bk == 0)
-            cpush(s, &c.cur, acode, af->c);
+            cpush(s, c.st.ta, &c.cur, acode, af->c);
         else
             c.cur = bk == 1 ? s.h_tc[h] : bk == 2 ? s.h_ns[h] : bk == 3 ? due_chain(s, due[di]) : s.fi[sf].start_chain;
         c.ord = 0;
@@ -1566,6 +1604,7 @@ struct SCtx
     uint32_t cur_code;
     bool cur_ok;
     bool safe; // a safe slot: this context runs one egress port's events (see switch_slot)
+    int ep;    // the running event's port (its tail positions: one writer per port and slot)
     int ord;
 };
 RD_HD const Chain &sc_cur(SCtx &c)
@@ -1573,7 +1612,7 @@ RD_HD const Chain &sc_cur(SCtx &c)
     if (!c.cur_ok)
     {
         if (c.cur_code)
-            cpush(*c.s, &c.cur, c.cur_code, *c.cur_src);
+            cpush(*c.s, c.pt[c.ep].ta, &c.cur, c.cur_code, *c.cur_src);
         else
             c.cur = *c.cur_src;
         c.cur_ok = true;
@@ -1590,10 +1629,10 @@ RD_HD void s_transmit(SCtx &c, int port, Pkt p)
 {
     const Sim &s = *c.s;
     const int lo = c.pc[port].link_out;
-    int64_t tx = link_tx(s, lo, p.size);
+    int64_t tx = link_txo(s, c.w[port].ob, p.size);
     c.pt[port].tc_at = c.now + tx;
     const Chain &cur = sc_cur(c);
-    cpush(s, &c.tc[port], ccode(tx, c.ord++), cur);
+    cpush(s, c.pt[c.ep].ta, &c.tc[port], ccode(tx, c.ord++), cur);
     p.ord = (uint8_t)(c.ord < 255 ? c.ord : 255);
     c.ord++;
     c.w[port].nout++;
@@ -1838,6 +1877,7 @@ RD_HD void sw_ctx(SCtx &c, const Sim &s, int x, int64_t slot, uint32_t *pend, in
     c.safe = false;
     c.cur_ok = false;
     c.ord = 0;
+    c.ep = 0;
     c.now = slot * s.D;
 }
 // a port's slot set-up: its arrivals' cursor and its out bucket
@@ -1894,6 +1934,7 @@ RD_HD void sw_serial_events(const Sim &s, SCtx &c, int deg, int64_t t1)
         const int p = best;
         c.now = bt;
         c.ord = 0;
+        c.ep = p;
         c.cur_ok = false;
         if (bk == 1)
         {
@@ -2042,6 +2083,7 @@ RD_HD void sw_egress_events(const Sim &s, SCtx &c, SwWork &k, int o, int deg, in
         else
             arr = before_q(s, ta, acode, fa->c, 0, c.tc[o]);
         c.ord = 0;
+        c.ep = o;
         c.cur_ok = false;
         if (!arr)
         {

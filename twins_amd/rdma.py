@@ -15,7 +15,7 @@ BAD = (4, 5, 6, 7, 12, 32)  # counters that make a run inexact (the reference's 
 
 class PackCfg(ctypes.Structure):
     _fields_ = [("q3cap", ctypes.c_int32), ("tail_log2", ctypes.c_int32), ("team_blocks", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("hosts_per_wave", ctypes.c_int32)]
 
 
 class RunStats(ctypes.Structure):
@@ -82,11 +82,11 @@ class World:
 class Batch:
     """Worlds side by side on the GPU (the reference's --batch)."""
 
-    def __init__(self, worlds, q3cap=0, tail_log2=0, team_blocks=0):
+    def __init__(self, worlds, q3cap=0, tail_log2=0, team_blocks=0, hosts_per_wave=0):
         L = lib()
         self.worlds = list(worlds)
         arr = (ctypes.c_void_p * len(self.worlds))(*[w._h for w in self.worlds])
-        cfg = PackCfg(q3cap, tail_log2, team_blocks, 0)
+        cfg = PackCfg(q3cap, tail_log2, team_blocks, hosts_per_wave)
         self._h = ctypes.c_void_p()
         _check(L.trd_batch_create(arr, len(self.worlds), ctypes.byref(cfg), ctypes.byref(self._h)))
 
