@@ -31,6 +31,28 @@ def test_gpu_reference_jobs_short():
     run_batch(rdma_util.jobs(max_ref_s=3.0), "short")
 
 
+def test_gpu_reference_jobs_medium():
+    """The 36 jobs the reference's CPU build runs in 3-50 s (FatTree k=16, leaf-spine 256 hosts,
+    PFC sweeps), in one batch."""
+    run_batch(rdma_util.jobs(min_ref_s=3.0, max_ref_s=50.0), "medium")
+
+
+@pytest.mark.parametrize("name", ["hpcc", "dcqcn"])
+def test_gpu_config3(name):
+    """Config 3 (512-host leaf-spine, all-to-all + 160-to-1 incast) against the reference's output."""
+    from twins_amd import rdma
+    import util
+    runs = {l.split("|")[0]: l.rstrip("\n").split("|") for l in open(os.path.join(rdma_util.RG, "config3_manifest.txt")) if not l.startswith("#")}
+    _, kv, counters = runs[name]
+    cfgd = os.path.join(util.GOLDEN, "configs")
+    print(f"rdma config 3 {name}", flush=True)
+    w = rdma.World(rdma_util.CONFIG, os.path.join(cfgd, "ls32.txt"), os.path.join(cfgd, "config3_flows.csv"), kv.split())
+    b = rdma.Batch([w], q3cap=16384)
+    b.run()
+    assert b.fct_text(0) == rdma_util.ref_fct(f"config3_{name}")
+    assert b.summary_line() == counters
+
+
 def test_gpu_cli_one_job(tmp_path):
     """The `twins-rdma` command line on the GPU: the reference's output file and summary lines."""
     import util
