@@ -42,7 +42,7 @@ def test_gpu_config3(name):
     """Config 3 (512-host leaf-spine, all-to-all + 160-to-1 incast) against the reference's output."""
     from twins_amd import rdma
     import util
-    runs = {l.split("|")[0]: l.rstrip("\n").split("|") for l in open(os.path.join(rdma_util.RG, "config3_manifest.txt")) if not l.startswith("#")}
+    runs = {l.split("|")[0]: l.rstrip("\n").split("|", 2) for l in open(os.path.join(rdma_util.RG, "config3_manifest.txt")) if not l.startswith("#")}
     _, kv, counters = runs[name]
     cfgd = os.path.join(util.GOLDEN, "configs")
     print(f"rdma config 3 {name}", flush=True)
