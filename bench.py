@@ -64,6 +64,12 @@ def oracle_run(flows, args, tcp, out):
     return sec, ev, rows
 
 
+def rdma_traffic_bytes():
+    """HBM bytes of one config-3 step of the RDMA kernel, measured with rocprofv3 (see profiles/)."""
+    p = os.path.join(ROOT, "profiles", "rdma_team_kernel_config3_traffic.json")
+    return json.load(open(p))["traffic_bytes_per_step"] if os.path.exists(p) else None
+
+
 def traffic_bytes():
     """The measured HBM bytes per launch (profiles/tcp_worlds_kernel_config2_traffic.json), or None."""
     p = os.path.join(ROOT, "profiles", "tcp_worlds_kernel_config2_traffic.json")
@@ -321,7 +327,11 @@ def rdma_leg(a):
         "parity": {"flows_identical_to_reference": same, "flows": total,
                    "reference": "src/rdma (the reference's model) compiled for the CPU: oracle/_ref/twins-rdma-cpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "rdma_team_kernel",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": rdma_traffic_bytes(),
+                     "traffic_source": "HBM bytes per step (2 launches: a launch ends after 1 s) of this kernel on this workload: "
+                                       "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy's readings "
+                                       "(profiles/rdma_team_kernel_config3.md)",
+                     "kernel": "rdma_team_kernel",
                      "kernel_ms_per_step": round(1000 * kern / a.steps, 3), "bytes_per_event": RDMA_BYTES_PER_EVENT},
         "cpu_baseline": cpu,
     }
