@@ -57,6 +57,16 @@ def test_cli_errors(cpu_bin, kv, msg, tmp_path):
     assert r.returncode == 1 and msg in r.stderr
 
 
+def test_cli_refuses_what_is_not_modelled(cpu_bin, tmp_path):
+    r = run_cli(cpu_bin, "ft4.topo", "inc1.csv", ["--txtrace=x.txt"], tmp_path / "x.fct")
+    assert r.returncode == 1 and "not modelled" in r.stderr
+    dag = tmp_path / "x.dag"
+    dag.write_text("flow 0 0 1 1000\nroot 0 0\n")
+    r = subprocess.run([cpu_bin, rdma_util.CONFIG, f"--topo={rdma_util.INPUTS}/ft4.topo", f"--flows={dag}", f"--out={tmp_path}/y.fct"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "DAG flow files are not modelled" in r.stderr
+
+
 def test_cli_missing_file(cpu_bin, tmp_path):
     r = run_cli(cpu_bin, "nonexistent.topo", "inc1.csv", [], tmp_path / "x.fct")
     assert r.returncode == 1 and "cannot open" in r.stderr

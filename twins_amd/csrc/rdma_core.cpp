@@ -141,6 +141,11 @@ extern "C" int trd_world_create(const char *config_path, const char *const *over
             return fail(TW_EINVAL, "bad override %s", kv.c_str());
         set_override(config, kv.substr(0, eq), kv.substr(eq + 1));
     }
+    {
+        const size_t n = strlen(flow_path);
+        if (n > 4 && strcmp(flow_path + n - 4, ".dag") == 0)
+            return fail(TW_EINVAL, "%s: DAG flow files are not modelled", flow_path);
+    }
     const uint64_t bps = rate_bps ? rate_bps : 100000000000ull;
     const int64_t delay = delay_ns > 0 ? delay_ns : 1000;
     Cfg cfg{config};

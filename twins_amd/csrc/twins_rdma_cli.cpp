@@ -78,6 +78,11 @@ int main(int argc, char **argv)
             pc.q3cap = atoi(v.c_str());
         else if (k == "--tail")
             pc.tail_log2 = atoi(v.c_str());
+        else if (k == "--txtrace" || k == "--sent" || k == "--astra" || k.compare(0, 5, "--nvl") == 0 || k.compare(0, 5, "--isl") == 0)
+        {
+            fprintf(stderr, "%s: not modelled in this build\n", k.c_str());
+            return 1;
+        }
         else if (k == "--serial" || k == "--spread" || k == "--win" || k == "--gwin" || k == "--persist" || k.compare(0, 6, "--rank") == 0)
             continue;
         else if (k.compare(0, 2, "--") != 0 && eq != string::npos)
