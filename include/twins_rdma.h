@@ -69,6 +69,36 @@ int trd_counters_get(const trd_batch *b, uint64_t *ev40);
 int trd_world_stats_get(const trd_batch *b, int32_t world, int64_t *slots, double *device_s);
 void trd_batch_destroy(trd_batch *b);
 
+/* The serving loop for RDMA requests (with include/twins.h's TW_REQ_* states): each request is a
+ * world, packed into a device arena the server allocates once (no hipMalloc or hipFree while kernels
+ * run), and every launch of the slot kernel runs all resident, unfinished requests' worlds; requests
+ * that arrive during a launch (at most 1 s) join the next. Launches use at most max_blocks
+ * workgroups (0: an eighth of the CUs, which a TCP serving loop, tw_server_*, leaves free: both run
+ * side by side). One host thread drives a server: submit, then poll until a request is done.
+ * No counterpart in the reference (it has no serving loop). */
+typedef struct trd_server trd_server;
+typedef struct trd_server_cfg
+{
+    int32_t q3cap, tail_log2;  /* per request, as trd_pack_cfg (0: 4096 frames, 2^22 tail nodes) */
+    int32_t hosts_per_wave;    /* 0: 16 */
+    int32_t max_blocks;        /* workgroups a launch uses at most (0: a CU in eight) */
+    uint64_t arena_bytes;      /* 0: half of the free device memory at creation */
+} trd_server_cfg;
+typedef struct trd_server_stats
+{
+    int32_t waiting, resident, done, failed;
+    uint64_t arena_used, arena_bytes, launches;
+} trd_server_stats;
+int trd_server_create(const trd_server_cfg *cfg, trd_server **out);
+/* the world is copied; *id: the request's number (0, 1, ... in submission order) */
+int trd_server_submit(trd_server *s, const trd_world *w, int32_t *id);
+/* finished launches' results, admissions, the next launch; *st (may be null): the requests' states */
+int trd_server_poll(trd_server *s, trd_server_stats *st);
+int trd_server_state(const trd_server *s, int32_t id, int32_t *state);
+/* a done request's flows (as trd_fct_get) and counters; its device memory returns to the arena */
+int trd_server_result(trd_server *s, int32_t id, int64_t *start_ns, int64_t *fct_ns, uint64_t *ev40);
+void trd_server_destroy(trd_server *s);
+
 const char *trd_last_error(void);
 
 #ifdef __cplusplus

@@ -11,12 +11,12 @@ namespace rd
 static SwWork g_work; // a switch's slot workspace (LDS on the GPU)
 int launch_capacity(int) { return 64; }
 // every world of the queue to its end, one after the other (device memory is host memory here)
-int launch_worlds(const Sim *sp, const LaunchCfg *cfg, int, int, hipStream_t)
+int launch_worlds(const LaunchCfg *cfg, int, int, hipStream_t)
 {
-    const Sim &s = *sp;
     for (int qi = 0; qi < cfg->nq; qi++)
     {
-        const int w = cfg->queue[qi];
+        const int qe = cfg->queue[qi], w = qe & ((1 << QSHIFT) - 1);
+        const Sim &s = *cfg->sims[qe >> QSHIFT];
         const World &wd = s.wd[w];
         WorldRun &r = s.run[w];
         uint32_t *pend = s.pend + wd.pend_off;

@@ -50,6 +50,26 @@ def test_cpu_build_batch_equals_single_runs(cpu_bin, tmp_path):
     assert rdma_util.sum_counters(got) == rdma_util.sum_counters([j[5] for j in sel])
 
 
+def test_serving_loop_on_cpu(tmp_path):
+    """The serving loop (trd_server_*) with the kernel's steps on the CPU: four reference jobs in an
+    arena that holds the two k=8 worlds only one at a time (admission waits for memory), every
+    request's flows identical to the reference's."""
+    d = os.path.dirname(rdma_util.CPU_BIN)
+    subprocess.run(["make", "-C", d, "server_cpu"], check=True, capture_output=True)
+    jobs = [("a", "ft4.topo", "inc1.csv", "CC_MODE=1,BUFFER_SIZE=1", "inc1_b1_cc1"),
+            ("b", "fattree8.txt", "k8_incast16.csv", "CC_MODE=3,BUFFER_SIZE=2", "k8_incast16_b2_cc3"),
+            ("c", "fattree8.txt", "k8_incast16.csv", "CC_MODE=1", "k8_incast16_cc1"),
+            ("d", "ft4.topo", "inc1.csv", "CC_MODE=3,BUFFER_SIZE=1", "inc1_b1_cc3")]
+    args = [os.path.join(d, "server_cpu"), rdma_util.CONFIG, rdma_util.INPUTS, str(tmp_path), "700"]
+    for j in jobs:
+        args += list(j[:4])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "most waiting 1" in r.stdout or "most waiting 2" in r.stdout, r.stdout
+    for name, *_, ref in jobs:
+        assert (tmp_path / (name + ".fct")).read_text() == rdma_util.ref_fct(ref), name
+
+
 @pytest.mark.parametrize("kv,msg", [(["CC_MODE=2"], "only 1 (DCQCN), 3 (HPCC) and 8 (DCTCP)"), (["CC_MODE=3", "MULTI_RATE=1"], "MULTI_RATE 1"),
                                     (["L2_ACK_INTERVAL=0"], "L2_ACK_INTERVAL 0"), (["--bogus=1"], "unknown argument")])
 def test_cli_errors(cpu_bin, kv, msg, tmp_path):

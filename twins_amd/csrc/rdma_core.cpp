@@ -20,6 +20,7 @@
 #include "rdma_model.h"
 #include "rdma_kernel.h"
 #include "mrg_host.h"
+#include "arena.h"
 #include "../../include/twins.h"
 #include "../../include/twins_rdma.h"
 
@@ -473,7 +474,10 @@ struct trd_batch
 {
     rd::Sim s = {};     // device pointers
     rd::Sim *sim_d = nullptr;
+    const rd::Sim **sims_d = nullptr; // [1] sim_d, for the launch configuration
     vector<void *> allocs;
+    Arena *arena = nullptr; // a server's batch: its memory comes from the server's arena
+    bool own_stream = true;
     int W = 0, H = 0, S = 0, F = 0;
     vector<int> fbase, nf;
     vector<int64_t> start; // [F] flows' starts, by global id
@@ -493,12 +497,15 @@ struct trd_batch
     ~trd_batch()
     {
         for (void *p : allocs)
-            (void)hipFree(p);
+            if (arena)
+                arena->release(p);
+            else
+                (void)hipFree(p);
         if (e0)
             (void)hipEventDestroy(e0);
         if (e1)
             (void)hipEventDestroy(e1);
-        if (st)
+        if (st && own_stream)
             (void)hipStreamDestroy(st);
     }
     template <typename T>
@@ -506,7 +513,12 @@ struct trd_batch
     {
         void *d = nullptr;
         const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-        if (hipMalloc(&d, bytes) != hipSuccess)
+        if (arena)
+        {
+            if (!(d = arena->alloc(bytes)))
+                return fail(TW_ENOMEM, "the device arena has no %zu free bytes", bytes);
+        }
+        else if (hipMalloc(&d, bytes) != hipSuccess)
             return fail(TW_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
         allocs.push_back(d);
         if (hipMemsetAsync(d, 0, bytes, st) != hipSuccess)
@@ -820,6 +832,9 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
         b.hs[wi] = worlds[wi]->H;
         b.ss[wi] = worlds[wi]->S;
     }
+    TRY(b.upload(&b.sims_d, vector<const rd::Sim *>{b.sim_d}));
+    if (b.arena)
+        return TW_OK; // (a server's batch: the server launches it)
     // launches: the queue of worlds, the teams' barrier state
     const int cap = rd::launch_capacity(b.threads);
     if (cap <= 0)
@@ -908,13 +923,14 @@ extern "C" int trd_batch_run(trd_batch *b, trd_run_stats *st)
         L.queue = b->queue_d;
         L.qnext = b->qnext_d;
         L.teams = b->teams_d;
+        L.sims = b->sims_d; // (queue entries: batch 0's worlds)
         if (hipMemcpyAsync(b->queue_d, queue.data(), sizeof(int) * queue.size(), hipMemcpyHostToDevice, b->st) != hipSuccess ||
             hipMemcpyAsync(b->cfg_d, &L, sizeof L, hipMemcpyHostToDevice, b->st) != hipSuccess ||
             hipMemsetAsync(b->qnext_d, 0, sizeof(unsigned), b->st) != hipSuccess ||
             hipMemsetAsync(b->teams_d, 0, sizeof(rd::TeamState) * nteams, b->st) != hipSuccess)
             return fail(TW_EDEVICE, "launch set-up failed");
         (void)hipEventRecord(b->e0, b->st);
-        int e = rd::launch_worlds(b->sim_d, b->cfg_d, nteams * T, b->threads, b->st);
+        int e = rd::launch_worlds(b->cfg_d, nteams * T, b->threads, b->st);
         if (e)
             return fail(TW_EDEVICE, "rdma kernel launch failed (%d)", e);
         (void)hipEventRecord(b->e1, b->st);
@@ -993,3 +1009,270 @@ extern "C" int trd_world_stats_get(const trd_batch *b, int32_t world, int64_t *s
     return TW_OK;
 }
 extern "C" void trd_batch_destroy(trd_batch *b) { delete b; }
+
+// ------------------------------------------------------------------ the serving loop (RDMA requests)
+// Requests are worlds; each is packed into the server's device arena as a batch of its own (no
+// hipMalloc or hipFree while kernels run, so a TCP serving loop's persistent kernel is never
+// waited for), and one launch of the team kernel runs every resident, unfinished request's world
+// (LaunchCfg::sims: one Sim per request). A launch ends after 1 s of device time; requests that
+// arrived meanwhile join the next one. Launches use at most max_blocks workgroups (by default the
+// eighth of the CUs a TCP serving loop leaves free), so that both loops' teams always fit on the
+// GPU together.
+struct RSrvReq
+{
+    std::unique_ptr<trd_world> w;
+    trd_batch *b = nullptr;
+    int state = TW_REQ_WAITING;
+};
+struct trd_server
+{
+    trd_server_cfg cfg = {};
+    Arena arena;
+    void *arena_mem = nullptr;
+    hipStream_t ks = nullptr, up = nullptr;
+    hipEvent_t done_ev = nullptr;
+    bool in_flight = false;
+    int max_blocks = 0, hpw = 16;
+    uint64_t launches = 0;
+    vector<RSrvReq> reqs;
+    vector<int> waiting, launched;
+    size_t wait_head = 0;
+    // the launch's configuration, in device memory, and its host copies (kept until the launch ends)
+    int *queue_d = nullptr;
+    unsigned *qnext_d = nullptr;
+    rd::TeamState *teams_d = nullptr;
+    rd::LaunchCfg *cfg_d = nullptr;
+    const rd::Sim **sims_d = nullptr;
+    int cap = 0; // requests one launch takes at most
+    vector<int> queue_h;
+    vector<const rd::Sim *> sims_h;
+    rd::LaunchCfg L = {};
+    ~trd_server()
+    {
+        if (ks)
+            (void)hipStreamSynchronize(ks);
+        for (auto &r : reqs)
+            delete r.b;
+        for (void *p : {(void *)queue_d, (void *)qnext_d, (void *)teams_d, (void *)cfg_d, (void *)sims_d, arena_mem})
+            if (p)
+                (void)hipFree(p);
+        if (done_ev)
+            (void)hipEventDestroy(done_ev);
+        if (ks)
+            (void)hipStreamDestroy(ks);
+        if (up)
+            (void)hipStreamDestroy(up);
+    }
+};
+
+extern "C" int trd_server_create(const trd_server_cfg *cfg, trd_server **out)
+{
+    if (!out)
+        return fail(TW_EINVAL, "trd_server_create: null output");
+    int ndev = 0, dev = 0, cus = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(TW_EDEVICE, "no GPU: the RDMA model runs on the GPU only");
+    std::unique_ptr<trd_server> s(new trd_server);
+    if (cfg)
+        s->cfg = *cfg;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return fail(TW_EDEVICE, "hipDeviceGetAttribute failed");
+    const int resident = rd::launch_capacity(256);
+    if (resident <= 0)
+        return fail(TW_EDEVICE, "the rdma kernel fits no workgroup on the GPU");
+    s->max_blocks = s->cfg.max_blocks > 0 ? std::min(s->cfg.max_blocks, resident) : std::max(1, std::min(resident, cus / 8));
+    s->hpw = s->cfg.hosts_per_wave > 0 ? std::min(s->cfg.hosts_per_wave, 64) : 16;
+    s->cap = 4096;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess)
+        return fail(TW_EDEVICE, "hipMemGetInfo failed");
+    const size_t want = s->cfg.arena_bytes ? s->cfg.arena_bytes : fr / 2;
+    if (hipMalloc(&s->arena_mem, want) != hipSuccess)
+        return fail(TW_ENOMEM, "arena of %zu bytes", want);
+    s->arena.base = (char *)s->arena_mem;
+    s->arena.size = want;
+    s->arena.free_[0] = want;
+    if (hipStreamCreateWithFlags(&s->ks, hipStreamNonBlocking) != hipSuccess || hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->done_ev, hipEventDisableTiming) != hipSuccess)
+        return fail(TW_EDEVICE, "hipStreamCreate failed");
+    if (hipMalloc((void **)&s->queue_d, sizeof(int) * s->cap) != hipSuccess || hipMalloc((void **)&s->qnext_d, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc((void **)&s->teams_d, sizeof(rd::TeamState) * s->max_blocks) != hipSuccess ||
+        hipMalloc((void **)&s->cfg_d, sizeof(rd::LaunchCfg)) != hipSuccess || hipMalloc((void **)&s->sims_d, sizeof(void *) * s->cap) != hipSuccess)
+        return fail(TW_ENOMEM, "hipMalloc failed");
+    *out = s.release();
+    return TW_OK;
+}
+
+extern "C" int trd_server_submit(trd_server *s, const trd_world *w, int32_t *id)
+{
+    if (!s || !w || !id)
+        return fail(TW_EINVAL, "trd_server_submit: null argument");
+    RSrvReq r;
+    r.w.reset(new trd_world(*w));
+    s->reqs.push_back(std::move(r));
+    *id = (int32_t)s->reqs.size() - 1;
+    s->waiting.push_back(*id);
+    return TW_OK;
+}
+
+// the end of a launch: its requests' worlds that finished
+static int rsrv_collect(trd_server &s)
+{
+    vector<rd::WorldRun> runs(s.launched.size());
+    for (size_t i = 0; i < s.launched.size(); i++)
+        if (hipMemcpyAsync(&runs[i], s.reqs[s.launched[i]].b->s.run, sizeof(rd::WorldRun), hipMemcpyDeviceToHost, s.ks) != hipSuccess)
+            return fail(TW_EDEVICE, "hipMemcpy failed");
+    vector<rd::TeamState> ts(s.max_blocks);
+    if (hipMemcpyAsync(ts.data(), s.teams_d, sizeof(rd::TeamState) * s.max_blocks, hipMemcpyDeviceToHost, s.ks) != hipSuccess ||
+        hipStreamSynchronize(s.ks) != hipSuccess)
+        return fail(TW_EDEVICE, "the rdma kernel failed: %s", hipGetErrorString(hipGetLastError()));
+    bool err = false;
+    for (int t = 0; t < s.L.T && t < s.max_blocks; t++)
+        err |= ts[t].err != 0;
+    for (size_t i = 0; i < s.launched.size(); i++)
+    {
+        RSrvReq &r = s.reqs[s.launched[i]];
+        if (err)
+            r.state = TW_REQ_FAILED;
+        else if (runs[i].finished)
+            r.state = TW_REQ_DONE;
+    }
+    s.launched.clear();
+    return err ? fail(TW_EDEVICE, "a team barrier timed out") : TW_OK;
+}
+
+extern "C" int trd_server_poll(trd_server *s, trd_server_stats *st)
+{
+    if (!s)
+        return fail(TW_EINVAL, "trd_server_poll: null server");
+    int rc = TW_OK;
+    if (s->in_flight)
+    {
+        const hipError_t q = hipEventQuery(s->done_ev);
+        if (q == hipSuccess)
+        {
+            s->in_flight = false;
+            rc = rsrv_collect(*s);
+        }
+        else if (q != hipErrorNotReady)
+            return fail(TW_EDEVICE, "the rdma kernel failed: %s", hipGetErrorString(q));
+    }
+    // admissions, in arrival order, while the arena has room
+    while (s->wait_head < s->waiting.size())
+    {
+        RSrvReq &r = s->reqs[s->waiting[s->wait_head]];
+        std::unique_ptr<trd_batch> b(new trd_batch);
+        b->arena = &s->arena;
+        b->st = s->up;
+        b->own_stream = false;
+        trd_pack_cfg pc = {};
+        pc.q3cap = s->cfg.q3cap;
+        pc.tail_log2 = s->cfg.tail_log2 > 0 ? s->cfg.tail_log2 : 22;
+        trd_world *wp = r.w.get();
+        const int e = pack(*b, &wp, 1, pc);
+        if (e == TW_ENOMEM)
+        {
+            (void)hipStreamSynchronize(s->up); // (the partial batch's memsets end before its memory returns)
+            b.reset();
+            if (s->arena.used > 0)
+                break; // it waits for memory
+            // (it does not fit even an empty arena: it never will)
+        }
+        s->wait_head++;
+        if (e)
+        {
+            r.state = TW_REQ_FAILED;
+            continue;
+        }
+        r.b = b.release();
+        r.state = TW_REQ_RESIDENT;
+    }
+    // a launch over the resident, unfinished requests
+    if (!s->in_flight)
+    {
+        s->queue_h.clear();
+        s->sims_h.clear();
+        s->launched.clear();
+        int mx = 1;
+        for (size_t i = 0; i < s->reqs.size() && (int)s->launched.size() < s->cap; i++)
+        {
+            RSrvReq &r = s->reqs[i];
+            if (r.state != TW_REQ_RESIDENT)
+                continue;
+            s->queue_h.push_back((int)s->sims_h.size() << rd::QSHIFT);
+            s->sims_h.push_back(r.b->sim_d);
+            s->launched.push_back((int)i);
+            mx = std::max(mx, (r.b->hs[0] + s->hpw - 1) / s->hpw + r.b->ss[0]);
+        }
+        if (!s->launched.empty())
+        {
+            rd::LaunchCfg &L = s->L;
+            L = rd::LaunchCfg{};
+            L.T = std::max(1, std::min(std::min(32, (mx + 3) / 4), s->max_blocks));
+            L.nq = (int)s->queue_h.size();
+            L.hpw = s->hpw;
+            L.queue = s->queue_d;
+            L.qnext = s->qnext_d;
+            L.teams = s->teams_d;
+            L.sims = s->sims_d;
+            const int nteams = std::max(1, std::min(L.nq, s->max_blocks / L.T));
+            if (hipMemcpyAsync(s->queue_d, s->queue_h.data(), sizeof(int) * L.nq, hipMemcpyHostToDevice, s->ks) != hipSuccess ||
+                hipMemcpyAsync(s->sims_d, s->sims_h.data(), sizeof(void *) * L.nq, hipMemcpyHostToDevice, s->ks) != hipSuccess ||
+                hipMemcpyAsync(s->cfg_d, &L, sizeof L, hipMemcpyHostToDevice, s->ks) != hipSuccess ||
+                hipMemsetAsync(s->qnext_d, 0, sizeof(unsigned), s->ks) != hipSuccess ||
+                hipMemsetAsync(s->teams_d, 0, sizeof(rd::TeamState) * s->max_blocks, s->ks) != hipSuccess)
+                return fail(TW_EDEVICE, "launch set-up failed");
+            const int e = rd::launch_worlds(s->cfg_d, nteams * L.T, 256, s->ks);
+            if (e)
+                return fail(TW_EDEVICE, "rdma kernel launch failed (%d)", e);
+            if (hipEventRecord(s->done_ev, s->ks) != hipSuccess)
+                return fail(TW_EDEVICE, "hipEventRecord failed");
+            s->in_flight = true;
+            s->launches++;
+        }
+    }
+    if (st)
+    {
+        *st = trd_server_stats{};
+        for (auto &r : s->reqs)
+        {
+            st->waiting += r.state == TW_REQ_WAITING;
+            st->resident += r.state == TW_REQ_RESIDENT;
+            st->done += r.state == TW_REQ_DONE || r.state == TW_REQ_TAKEN;
+            st->failed += r.state == TW_REQ_FAILED;
+        }
+        st->arena_used = s->arena.used;
+        st->arena_bytes = s->arena.size;
+        st->launches = s->launches;
+    }
+    return rc;
+}
+
+extern "C" int trd_server_state(const trd_server *s, int32_t id, int32_t *state)
+{
+    if (!s || id < 0 || id >= (int32_t)s->reqs.size() || !state)
+        return fail(TW_EINVAL, "trd_server_state: bad request %d", id);
+    *state = s->reqs[id].state;
+    return TW_OK;
+}
+
+extern "C" int trd_server_result(trd_server *s, int32_t id, int64_t *start_ns, int64_t *fct_ns, uint64_t *ev40)
+{
+    if (!s || id < 0 || id >= (int32_t)s->reqs.size())
+        return fail(TW_EINVAL, "trd_server_result: bad request %d", id);
+    RSrvReq &r = s->reqs[id];
+    if (r.state != TW_REQ_DONE)
+        return fail(TW_ESTATE, "trd_server_result: request %d is not done (state %d)", id, r.state);
+    int e = trd_fct_get(r.b, 0, start_ns, fct_ns);
+    if (!e && ev40)
+        e = trd_counters_get(r.b, ev40);
+    if (e)
+        return e;
+    delete r.b; // (its memory returns to the arena; no kernel uses it any more)
+    r.b = nullptr;
+    r.w.reset();
+    r.state = TW_REQ_TAKEN;
+    return TW_OK;
+}
+
+extern "C" void trd_server_destroy(trd_server *s) { delete s; }

@@ -15,16 +15,20 @@ struct TeamState
     int stop;
     unsigned long long wake;
 };
-// one launch: teams of T workgroups take worlds queue[0 .. nq) in order (qnext: the next index)
+// one launch: teams of T workgroups take worlds queue[0 .. nq) in order (qnext: the next index);
+// an entry is sim << QSHIFT | world: world `world` of the batch whose Sim is sims[sim] (a launch can
+// run the worlds of several separately packed batches: the serving loop's requests)
+constexpr int QSHIFT = 20;
 struct LaunchCfg
 {
     int T, nq, xcd_local, hpw; // hpw: hosts of a host item (side by side on a wave's lanes, at most 64)
     const int *queue;
     unsigned *qnext;
     TeamState *teams;
+    const Sim *const *sims;
 };
 // the workgroups of 256 threads the GPU holds at once (-1: unknown)
 int launch_capacity(int threads);
-// one launch of nblocks workgroups (teams of cfg->T); cfg_dev and sim are device copies; 0 or a hipError_t
-int launch_worlds(const Sim *sim, const LaunchCfg *cfg_dev, int nblocks, int threads, hipStream_t st);
+// one launch of nblocks workgroups (teams of cfg->T); cfg_dev is a device copy; 0 or a hipError_t
+int launch_worlds(const LaunchCfg *cfg_dev, int nblocks, int threads, hipStream_t st);
 } // namespace rd

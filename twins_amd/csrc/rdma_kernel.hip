@@ -19,9 +19,8 @@ constexpr int WAVES = 4;                               // waves of a 256-thread 
 
 __device__ __forceinline__ unsigned ld_acquire(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
 
-__global__ void __launch_bounds__(256) rdma_team_kernel(const Sim *__restrict__ sp, const LaunchCfg *__restrict__ lp)
+__global__ void __launch_bounds__(256) rdma_team_kernel(const LaunchCfg *__restrict__ lp)
 {
-    const Sim &s = *sp;
     const LaunchCfg L = *lp;
     __shared__ SwWork work[WAVES];
     __shared__ int64_t s_slot;
@@ -104,7 +103,8 @@ __global__ void __launch_bounds__(256) rdma_team_kernel(const Sim *__restrict__ 
         __syncthreads();
         if (s_w >= L.nq)
             return;
-        const int w = L.queue[s_w];
+        const int qe = L.queue[s_w], w = qe & ((1 << QSHIFT) - 1);
+        const Sim &s = *L.sims[qe >> QSHIFT];
         const World wd = s.wd[w];
         WorldRun &r = s.run[w];
         uint32_t *pend = s.pend + wd.pend_off;
@@ -224,9 +224,9 @@ int launch_capacity(int threads)
     return per_cu * cus;
 }
 
-int launch_worlds(const Sim *sim, const LaunchCfg *cfg_dev, int nblocks, int threads, hipStream_t st)
+int launch_worlds(const LaunchCfg *cfg_dev, int nblocks, int threads, hipStream_t st)
 {
-    hipLaunchKernelGGL(rdma_team_kernel, dim3(nblocks), dim3(threads), 0, st, sim, cfg_dev);
+    hipLaunchKernelGGL(rdma_team_kernel, dim3(nblocks), dim3(threads), 0, st, cfg_dev);
     return (int)hipGetLastError();
 }
 } // namespace rd

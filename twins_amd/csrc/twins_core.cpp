@@ -76,6 +76,7 @@ static uint32_t flow_hash(uint32_t a, uint32_t b, uint16_t pa, uint16_t pb)
 }
 
 #include "mrg_host.h"
+#include "arena.h"
 
 // ------------------------------------------------------------------ a world on the host
 struct tw_world
@@ -769,61 +770,6 @@ extern "C" int tw_world_create_args(int argc, const char *const *argv, tw_world 
 // A device-memory arena: one allocation made up front, carved first-fit in 256-byte units (the
 // serving loop's requests: no hipMalloc or hipFree, which may wait for the device, while its
 // persistent kernel runs).
-struct Arena
-{
-    char *base = nullptr;
-    size_t size = 0;
-    std::map<size_t, size_t> free_; // offset -> bytes
-    size_t used = 0;
-    std::mutex mu;
-    void *alloc(size_t n)
-    {
-        n = (n + 255) & ~(size_t)255;
-        std::lock_guard<std::mutex> g(mu);
-        for (auto it = free_.begin(); it != free_.end(); ++it)
-            if (it->second >= n)
-            {
-                size_t off = it->first, left = it->second - n;
-                free_.erase(it);
-                if (left)
-                    free_[off + n] = left;
-                used += n;
-                sizes[off] = n;
-                return base + off;
-            }
-        return nullptr;
-    }
-    void release(void *p)
-    {
-        std::lock_guard<std::mutex> g(mu);
-        size_t off = (char *)p - base;
-        auto sz = sizes.find(off);
-        if (sz == sizes.end())
-            return;
-        size_t n = sz->second;
-        sizes.erase(sz);
-        used -= n;
-        auto nx = free_.lower_bound(off);
-        if (nx != free_.end() && nx->first == off + n)
-        {
-            n += nx->second;
-            free_.erase(nx);
-        }
-        auto pv = free_.lower_bound(off);
-        if (pv != free_.begin())
-        {
-            --pv;
-            if (pv->first + pv->second == off)
-            {
-                pv->second += n;
-                return;
-            }
-        }
-        free_[off] = n;
-    }
-    std::map<size_t, size_t> sizes;
-};
-
 struct tw_batch
 {
     int runs = 9;

@@ -23,6 +23,16 @@ class RunStats(ctypes.Structure):
                 ("kernel_s", ctypes.c_double), ("flows_done", ctypes.c_uint32), ("launches", ctypes.c_int32)]
 
 
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("q3cap", ctypes.c_int32), ("tail_log2", ctypes.c_int32), ("hosts_per_wave", ctypes.c_int32),
+                ("max_blocks", ctypes.c_int32), ("arena_bytes", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("waiting", ctypes.c_int32), ("resident", ctypes.c_int32), ("done", ctypes.c_int32), ("failed", ctypes.c_int32),
+                ("arena_used", ctypes.c_uint64), ("arena_bytes", ctypes.c_uint64), ("launches", ctypes.c_uint64)]
+
+
 _bound = False
 
 
@@ -43,6 +53,12 @@ def lib():
         L.trd_counters_get.argtypes = [vp, ctypes.POINTER(u64)]
         L.trd_world_stats_get.argtypes = [vp, i32, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double)]
         L.trd_batch_destroy.argtypes = [vp]
+        L.trd_server_create.argtypes = [ctypes.POINTER(ServerCfg), ctypes.POINTER(vp)]
+        L.trd_server_submit.argtypes = [vp, vp, ctypes.POINTER(i32)]
+        L.trd_server_poll.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.trd_server_state.argtypes = [vp, i32, ctypes.POINTER(i32)]
+        L.trd_server_result.argtypes = [vp, i32, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(u64)]
+        L.trd_server_destroy.argtypes = [vp]
         _bound = True
     return L
 
@@ -126,6 +142,50 @@ class Batch:
     def close(self):
         if self._h:
             lib().trd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop for RDMA requests: submit worlds, poll, take results (include/twins_rdma.h)."""
+
+    def __init__(self, q3cap=0, tail_log2=0, hosts_per_wave=0, max_blocks=0, arena_bytes=0):
+        self._h = ctypes.c_void_p()
+        _check(lib().trd_server_create(ctypes.byref(ServerCfg(q3cap, tail_log2, hosts_per_wave, max_blocks, arena_bytes)),
+                                       ctypes.byref(self._h)))
+        self.flows = []
+
+    def submit(self, world):
+        i = ctypes.c_int32()
+        _check(lib().trd_server_submit(self._h, world._h, ctypes.byref(i)))
+        self.flows.append(world.flows)
+        return i.value
+
+    def poll(self):
+        st = ServerStats()
+        _check(lib().trd_server_poll(self._h, ctypes.byref(st)))
+        return st
+
+    def state(self, i):
+        v = ctypes.c_int32()
+        _check(lib().trd_server_state(self._h, i, ctypes.byref(v)))
+        return v.value
+
+    def result(self, i):
+        """(start_ns, fct_ns, counters) of a done request; its device memory is released."""
+        n = self.flows[i]
+        a, b, ev = (ctypes.c_int64 * max(1, n))(), (ctypes.c_int64 * max(1, n))(), (ctypes.c_uint64 * 40)()
+        _check(lib().trd_server_result(self._h, i, a, b, ev))
+        return list(a)[:n], list(b)[:n], list(ev)
+
+    def close(self):
+        if self._h:
+            lib().trd_server_destroy(self._h)
             self._h = None
 
     def __del__(self):
