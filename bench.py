@@ -186,77 +186,131 @@ def sweep_leg(a):
 
 
 def serve_leg(a):
-    """Config 5: the serving loop; returns the "serve" object of the JSON line."""
+    """Config 5: the serving loops, TCP and RDMA side by side; returns the "serve" object of the JSON line."""
     import twins_amd
-    from twins_amd import workloads
+    from twins_amd import rdma, workloads
     t0 = time.time()
-    reqs = workloads.config5_requests(os.path.join(a.sweep_dir, "requests"), a.serve_requests, seed=5, rate_per_s=a.serve_rate)
+    reqs = workloads.config5_requests(os.path.join(a.sweep_dir, "requests"), a.serve_requests, seed=5, rate_per_s=a.serve_rate,
+                                      rdma_share=a.serve_rdma)
+    cfg = os.path.join(ROOT, "tests", "golden", "rdma", "inputs", "hpcc.cfg")
+    is_rdma = [isinstance(r[1], dict) for r in reqs]
+
+    def build(i):
+        r = reqs[i][1]
+        return rdma.World(cfg, r["topo"], r["flows"], r["overrides"]) if is_rdma[i] else twins_amd.World(r)
     with cf.ThreadPoolExecutor(a.cpu_cores) as ex:
-        worlds = list(ex.map(lambda r: twins_amd.World(r[1]), reqs))
-    log(f"serve: {len(reqs)} requests generated and built in {time.time() - t0:.1f} s")
-    srv = twins_amd.Server(idle_s=2.0)
-    n = len(reqs)
+        worlds = list(ex.map(build, range(len(reqs))))
+    n, n_rdma = len(reqs), sum(is_rdma)
+    log(f"serve: {n} requests ({n_rdma} RDMA) generated and built in {time.time() - t0:.1f} s")
+    # the GPU split: the RDMA loop's launches take rdma_blocks workgroups, the TCP loop's teams the rest
+    resident = twins_amd.launch_capacity(9, 256)
+    rblocks = a.serve_rdma_blocks if n_rdma else 0
+    srv = twins_amd.Server(idle_s=2.0, small_teams=resident - rblocks - 16 * 4 - 4 * 8 if n_rdma else 0)
+    rsrv = rdma.Server(max_blocks=rblocks) if n_rdma else None
+    sid = {}
     arrival, finish, results = {}, {}, {}
     events = 0
     submitted = 0
     t_start = time.perf_counter()
     last_log = t_start
+    inexact = 0
     while len(finish) < n:
         now = time.perf_counter() - t_start
         while submitted < n and reqs[submitted][0] <= now:
-            rid = srv.submit(worlds[submitted])
-            arrival[rid] = reqs[submitted][0]
+            sid[submitted] = (rsrv if is_rdma[submitted] else srv).submit(worlds[submitted])
+            arrival[submitted] = reqs[submitted][0]
             submitted += 1
         st = srv.poll()
-        for rid in list(arrival):
-            if rid in finish:
+        rst = rsrv.poll() if rsrv else None
+        for i in list(arrival):
+            if i in finish:
                 continue
-            s_ = srv.state(rid)
-            if s_ in (twins_amd.REQ_DONE, twins_amd.REQ_FAILED):
-                est, done, ev = srv.result(rid)
-                finish[rid] = time.perf_counter() - t_start
-                events += ev[26]
-                if rid % a.serve_check == 0:
-                    results[rid] = list(zip(est, done))
+            if is_rdma[i]:
+                s_ = rsrv.state(sid[i])
+                if s_ in (twins_amd.REQ_DONE, twins_amd.REQ_FAILED):
+                    if s_ == twins_amd.REQ_FAILED:
+                        raise RuntimeError(f"RDMA request {i} failed")
+                    start, fct, ev = rsrv.result(sid[i])
+                    finish[i] = time.perf_counter() - t_start
+                    events += ev[38]
+                    inexact += any(ev[j] for j in rdma.BAD)
+                    if i % a.serve_check == 0 and worlds[i].hosts <= 128:  # (the reference's CPU build: up to k=8)
+                        results[i] = "".join(f"{j} {start[j]} {fct[j]}\n" for j in range(len(fct)) if fct[j] >= 0)
+            else:
+                s_ = srv.state(sid[i])
+                if s_ in (twins_amd.REQ_DONE, twins_amd.REQ_FAILED):
+                    est, done, ev = srv.result(sid[i])
+                    finish[i] = time.perf_counter() - t_start
+                    events += ev[26]
+                    if i % a.serve_check == 0:
+                        results[i] = list(zip(est, done))
         if time.perf_counter() - last_log > 10:
             last_log = time.perf_counter()
-            log(f"serve: {now:.0f} s, {submitted} submitted, {len(finish)} done, resident {st.resident}, waiting {st.waiting}, "
-                f"arena {st.arena_used / 2**30:.1f}/{st.arena_bytes / 2**30:.1f} GiB")
+            log(f"serve: {now:.0f} s, {submitted} submitted, {len(finish)} done; TCP resident {st.resident}, waiting {st.waiting}, "
+                f"arena {st.arena_used / 2**30:.1f}/{st.arena_bytes / 2**30:.1f} GiB" +
+                (f"; RDMA resident {rst.resident}, waiting {rst.waiting}, arena {rst.arena_used / 2**30:.1f}/{rst.arena_bytes / 2**30:.1f} GiB"
+                 if rst else ""))
         time.sleep(0.0005)
     wall = max(finish.values()) - reqs[0][0]
     srv.close()
+    if rsrv:
+        rsrv.close()
     lat = sorted(finish[r] - arrival[r] for r in finish)
-    pct = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))]
-    log(f"serve: {n} requests in {wall:.1f} s, p50 {pct(0.5):.3f} s, p99 {pct(0.99):.3f} s")
-    by_k = {}
+    pct = lambda xs, p: xs[min(len(xs) - 1, int(p * len(xs)))]
+    log(f"serve: {n} requests in {wall:.1f} s, p50 {pct(lat, 0.5):.3f} s, p99 {pct(lat, 0.99):.3f} s")
+    by = {}
     for r in finish:
-        by_k.setdefault(worlds[r].info.hosts, []).append(finish[r] - arrival[r])
-    for h, xs in sorted(by_k.items()):
+        by.setdefault(("rdma " if is_rdma[r] else "tcp ") + str(worlds[r].hosts if is_rdma[r] else worlds[r].info.hosts), []).append(
+            finish[r] - arrival[r])
+    for h, xs in sorted(by.items()):
         xs.sort()
         log(f"  {h} hosts: {len(xs)} requests, p50 {xs[len(xs) // 2]:.3f} s, max {xs[-1]:.3f} s")
-    # every serve_check-th request against the oracle (all host cores)
-    def check(rid):
-        out = os.path.join(a.sweep_dir, f"s{rid:04d}.csv")
-        _, ev, rows = oracle_world(reqs[rid][1], out)
-        return rid, ev, rows
+    kind_lat = {}
+    for kind in (False, True):
+        xs = sorted(finish[r] - arrival[r] for r in finish if is_rdma[r] == kind)
+        if xs:
+            kind_lat["rdma" if kind else "tcp"] = {"requests": len(xs), "p50_s": round(pct(xs, 0.5), 4), "p99_s": round(pct(xs, 0.99), 4)}
+    # every serve_check-th request against the oracle (TCP) or the reference's RDMA build, on all host cores
+    ref = os.path.join(ROOT, "oracle", "_ref", "twins-rdma-cpu")
+
+    def check(i):
+        if is_rdma[i]:
+            if not os.path.exists(ref):
+                return i, None
+            r = reqs[i][1]
+            out = os.path.join(a.sweep_dir, f"s{i:04d}.fct")
+            subprocess.run([ref, cfg, f"--topo={r['topo']}", f"--flows={r['flows']}", f"--out={out}", "--gwin=0"] + r["overrides"],
+                           capture_output=True, timeout=1800)
+            return i, open(out).read().splitlines()
+        out = os.path.join(a.sweep_dir, f"s{i:04d}.csv")
+        _, ev, rows = oracle_world(reqs[i][1], out)
+        return i, rows
     same = total = 0
     with cf.ThreadPoolExecutor(a.cpu_cores) as ex:
-        for rid, ev, rows in ex.map(check, sorted(results)):
-            same += sum(x == y for x, y in zip(rows, results[rid]))
+        for i, rows in ex.map(check, sorted(results)):
+            if rows is None:
+                continue
+            got = results[i].splitlines() if is_rdma[i] else results[i]
+            same += sum(x == y for x, y in zip(rows, got)) if len(rows) == len(got) else 0
             total += len(rows)
-    log(f"serve: oracle parity {same}/{total} flows of {len(results)} requests")
+    log(f"serve: parity {same}/{total} flows of {len(results)} requests checked")
     return {
-        "workload": f"config 5: {n} requests arriving Poisson at {a.serve_rate:g}/s (seed 5), each a FatTree k in {{4,8,16,32}} "
-                    f"world with U{{1000..64000}} Pareto(1.2) flows and a TCP variant drawn per request, a quarter with a link "
-                    f"failure and recovery; TCP only (the RDMA model is not built yet)",
+        "workload": f"config 5: {n} requests arriving Poisson at {a.serve_rate:g}/s (seed 5); TCP requests: a FatTree k in "
+                    f"{{4,8,16,32}} world with U{{1000..64000}} Pareto(1.2) flows and a TCP variant drawn per request, a quarter with "
+                    f"a link failure and recovery; RDMA requests ({n_rdma}, a share of {a.serve_rdma:g}): FatTree k in {{4,8,16}} "
+                    f"with the same kind of flows, CC_MODE drawn from DCQCN, HPCC, DCTCP; two serving loops side by side on the "
+                    f"GPU (the RDMA loop's launches on {rblocks} workgroups, the TCP loop's teams on the rest)",
         "metric": "sweep-request p50/p99 latency (arrival to flow times ready) and packet-events/s",
-        "p50_s": round(pct(0.5), 4),
-        "p99_s": round(pct(0.99), 4),
+        "p50_s": round(pct(lat, 0.5), 4),
+        "p99_s": round(pct(lat, 0.99), 4),
+        "by_kind": kind_lat,
         "value": round(events / wall, 1),
         "unit": "packet-events/s",
         "requests": n,
         "wall_s": round(wall, 3),
-        "parity": {"flows_identical_to_oracle": same, "flows": total, "requests_checked": len(results)},
+        "rdma_inexact_requests": inexact,
+        "parity": {"flows_identical_to_oracle": same, "flows": total, "requests_checked": len(results),
+                   "oracle": "TCP: oracle/tcp_oracle; RDMA: the reference's RDMA build (oracle/_ref)"},
     }
 
 
@@ -354,6 +408,8 @@ def main():
     ap.add_argument("--serve_requests", type=int, default=1000)
     ap.add_argument("--serve_rate", type=float, default=100.0)
     ap.add_argument("--serve_check", type=int, default=20, help="every n-th request is checked against the oracle")
+    ap.add_argument("--serve_rdma", type=float, default=0.25, help="the share of the serving loop's requests that are RDMA worlds")
+    ap.add_argument("--serve_rdma_blocks", type=int, default=64, help="workgroups the RDMA serving loop's launches take")
     ap.add_argument("--rdma", type=int, default=1, help="also run config 3, the RDMA model (the \"rdma\" object)")
     a = ap.parse_args()
     if a.gpus != 1:

@@ -250,18 +250,44 @@ def config4_worlds(outdir, n_worlds=256, seed=4):
     return worlds
 
 
-def config5_requests(outdir, n_requests=1000, seed=5, rate_per_s=100.0, ks=(4, 8, 16, 32), max_flows=64000, fail_share=0.25):
+def config5_requests(outdir, n_requests=1000, seed=5, rate_per_s=100.0, ks=(4, 8, 16, 32), max_flows=64000, fail_share=0.25,
+                     rdma_share=0.0, rdma_ks=(4, 8, 16)):
     """The serving loop's requests: Poisson arrivals at rate_per_s, each a FatTree world of k drawn
     from ks with U{1000..max_flows} Pareto-sized flows and a TCP variant drawn per request; a
     fail_share of them carry one link failure and recovery (then as a FatTree topology file).
-    Returns [(arrival_s, `twins` argument list)] and writes the files into outdir."""
+    With rdma_share > 0 that share of the requests are RDMA worlds instead (`twins-rdma`: a FatTree
+    topology file of k drawn from rdma_ks, the same kind of flow file, CC_MODE drawn from DCQCN, HPCC
+    and DCTCP; no link failures: the RDMA model has none).
+    Returns [(arrival_s, args)]: a `twins` argument list for TCP, a dict {topo, flows, overrides} for
+    RDMA; writes the files into outdir."""
     os.makedirs(outdir, exist_ok=True)
     rng = np.random.Generator(np.random.PCG64(seed))
     t = 0.0
     out = []
     topo_files = {}
+
+    def topo_file(k):
+        if k not in topo_files:
+            topo_files[k] = os.path.join(outdir, f"fattree{k}.txt")
+            text = fattree_topology(k)
+            if not os.path.exists(topo_files[k]) or open(topo_files[k]).read() != text:
+                with open(topo_files[k], "w") as f:
+                    f.write(text)
+        return topo_files[k]
+
     for r in range(n_requests):
         t += float(rng.exponential(1.0 / rate_per_s))
+        if rdma_share > 0 and rng.random() < rdma_share:
+            k = int(rng.choice(rdma_ks))
+            nflows = int(rng.integers(1000, max_flows + 1))
+            cc = int(rng.choice([1, 3, 8]))
+            ff = os.path.join(outdir, f"r{r:04d}_flows.csv")
+            text = pareto_flows(seed * 1000003 + r, nflows, k ** 3 // 4)
+            if not os.path.exists(ff) or os.path.getsize(ff) != len(text):
+                with open(ff, "w") as f:
+                    f.write(text)
+            out.append((t, {"topo": topo_file(k), "flows": ff, "overrides": [f"CC_MODE={cc}"]}))
+            continue
         k = int(rng.choice(ks))
         H = k ** 3 // 4
         nflows = int(rng.integers(1000, max_flows + 1))
@@ -273,14 +299,10 @@ def config5_requests(outdir, n_requests=1000, seed=5, rate_per_s=100.0, ks=(4, 8
                 f.write(text)
         args = [f"--flows={ff}", f"--tcp={tcp}"]
         if rng.random() < fail_share:
-            if k not in topo_files:
-                topo_files[k] = os.path.join(outdir, f"fattree{k}.txt")
-                with open(topo_files[k], "w") as f:
-                    f.write(fattree_topology(k))
             # a core-aggregation link of pod 0 down at 100-400 us, back up 300 us later
             core, agg = k * k // 2 + k * k // 2, k * k // 2
             down = int(rng.integers(100, 401)) * 1000
-            args = ["--topo=graph", f"--topofile={topo_files[k]}", f"--fail={down}:s{core}:s{agg},{down + 300000}:s{core}:s{agg}:up"] + args
+            args = ["--topo=graph", f"--topofile={topo_file(k)}", f"--fail={down}:s{core}:s{agg},{down + 300000}:s{core}:s{agg}:up"] + args
         else:
             args = [f"--k={k}"] + args
         out.append((t, args))
