@@ -205,7 +205,9 @@ def serve_leg(a):
     # the GPU split: the RDMA loop's launches take rdma_blocks workgroups, the TCP loop's teams the rest
     resident = twins_amd.launch_capacity(9, 256)
     rblocks = a.serve_rdma_blocks if n_rdma else 0
-    srv = twins_amd.Server(idle_s=2.0, small_teams=resident - rblocks - 16 * 4 - 4 * 8 if n_rdma else 0)
+    # (the TCP loop also leaves an eighth of the CUs free for the uploads' fill kernels)
+    cus = resident  # (one slot-kernel workgroup per CU)
+    srv = twins_amd.Server(idle_s=2.0, small_teams=resident - cus // 8 - rblocks - 16 * 4 - 4 * 8 if n_rdma else 0)
     rsrv = rdma.Server(max_blocks=rblocks) if n_rdma else None
     sid = {}
     arrival, finish, results = {}, {}, {}

@@ -71,17 +71,18 @@ void trd_batch_destroy(trd_batch *b);
 
 /* The serving loop for RDMA requests (with include/twins.h's TW_REQ_* states): each request is a
  * world, packed into a device arena the server allocates once (no hipMalloc or hipFree while kernels
- * run), and every launch of the slot kernel runs all resident, unfinished requests' worlds; requests
- * that arrive during a launch (at most 1 s) join the next. Launches use at most max_blocks
- * workgroups (0: an eighth of the CUs, which a TCP serving loop, tw_server_*, leaves free: both run
- * side by side). One host thread drives a server: submit, then poll until a request is done.
+ * run); launches of the slot kernel run the resident, unfinished requests' worlds, small worlds and
+ * large ones on two streams at once (a quarter of the workgroups for the small); requests that
+ * arrive during a launch (at most 1 s) join the next. Launches use at most max_blocks workgroups
+ * (0: an eighth of the CUs; a TCP serving loop, tw_server_*, sized to leave them free runs side by
+ * side). One host thread drives a server: submit, then poll until a request is done.
  * No counterpart in the reference (it has no serving loop). */
 typedef struct trd_server trd_server;
 typedef struct trd_server_cfg
 {
     int32_t q3cap, tail_log2;  /* per request, as trd_pack_cfg (0: 4096 frames, 2^22 tail nodes) */
     int32_t hosts_per_wave;    /* 0: 16 */
-    int32_t max_blocks;        /* workgroups a launch uses at most (0: a CU in eight) */
+    int32_t max_blocks;        /* workgroups the launches use at most, at least 2 (0: a CU in eight) */
     uint64_t arena_bytes;      /* 0: half of the free device memory at creation */
 } trd_server_cfg;
 typedef struct trd_server_stats

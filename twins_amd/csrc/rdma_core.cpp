@@ -1013,53 +1013,69 @@ extern "C" void trd_batch_destroy(trd_batch *b) { delete b; }
 // ------------------------------------------------------------------ the serving loop (RDMA requests)
 // Requests are worlds; each is packed into the server's device arena as a batch of its own (no
 // hipMalloc or hipFree while kernels run, so a TCP serving loop's persistent kernel is never
-// waited for), and one launch of the team kernel runs every resident, unfinished request's world
-// (LaunchCfg::sims: one Sim per request). A launch ends after 1 s of device time; requests that
-// arrived meanwhile join the next one. Launches use at most max_blocks workgroups (by default the
-// eighth of the CUs a TCP serving loop leaves free), so that both loops' teams always fit on the
-// GPU together.
+// waited for), and a launch of the team kernel runs resident, unfinished requests' worlds
+// (LaunchCfg::sims: one Sim per request). Worlds of two sizes launch on two streams at once: small
+// worlds on small teams, large ones on large teams, each class within its share of max_blocks
+// workgroups (by default a CU in eight, which a TCP serving loop can leave free: then both loops'
+// teams always fit on the GPU together). A launch ends after 1 s of device time; requests that
+// arrived meanwhile join its class's next launch.
 struct RSrvReq
 {
     std::unique_ptr<trd_world> w;
     trd_batch *b = nullptr;
     int state = TW_REQ_WAITING;
+    int items = 0; // work items per slot
 };
-struct trd_server
+// one class's launches: its stream, its configuration in device memory and host copies (kept until
+// the launch ends)
+struct RSrvLane
 {
-    trd_server_cfg cfg = {};
-    Arena arena;
-    void *arena_mem = nullptr;
-    hipStream_t ks = nullptr, up = nullptr;
+    hipStream_t ks = nullptr;
     hipEvent_t done_ev = nullptr;
     bool in_flight = false;
-    int max_blocks = 0, hpw = 16;
-    uint64_t launches = 0;
-    vector<RSrvReq> reqs;
-    vector<int> waiting, launched;
-    size_t wait_head = 0;
-    // the launch's configuration, in device memory, and its host copies (kept until the launch ends)
+    int blocks = 0;
     int *queue_d = nullptr;
     unsigned *qnext_d = nullptr;
     rd::TeamState *teams_d = nullptr;
     rd::LaunchCfg *cfg_d = nullptr;
     const rd::Sim **sims_d = nullptr;
-    int cap = 0; // requests one launch takes at most
-    vector<int> queue_h;
+    vector<int> queue_h, launched;
     vector<const rd::Sim *> sims_h;
     rd::LaunchCfg L = {};
+};
+constexpr int RSRV_SMALL_ITEMS = 32; // worlds of at most this many work items are small (teams of up to 8)
+constexpr int RSRV_CAP = 4096;       // requests one launch takes at most
+struct trd_server
+{
+    trd_server_cfg cfg = {};
+    Arena arena;
+    void *arena_mem = nullptr;
+    hipStream_t up = nullptr;
+    RSrvLane lane[2]; // small, large
+    int max_blocks = 0, hpw = 16;
+    uint64_t launches = 0;
+    vector<RSrvReq> reqs;
+    vector<int> waiting;
+    size_t wait_head = 0;
     ~trd_server()
     {
-        if (ks)
-            (void)hipStreamSynchronize(ks);
+        for (auto &l : lane)
+            if (l.ks)
+                (void)hipStreamSynchronize(l.ks);
         for (auto &r : reqs)
             delete r.b;
-        for (void *p : {(void *)queue_d, (void *)qnext_d, (void *)teams_d, (void *)cfg_d, (void *)sims_d, arena_mem})
-            if (p)
-                (void)hipFree(p);
-        if (done_ev)
-            (void)hipEventDestroy(done_ev);
-        if (ks)
-            (void)hipStreamDestroy(ks);
+        for (auto &l : lane)
+        {
+            for (void *p : {(void *)l.queue_d, (void *)l.qnext_d, (void *)l.teams_d, (void *)l.cfg_d, (void *)l.sims_d})
+                if (p)
+                    (void)hipFree(p);
+            if (l.done_ev)
+                (void)hipEventDestroy(l.done_ev);
+            if (l.ks)
+                (void)hipStreamDestroy(l.ks);
+        }
+        if (arena_mem)
+            (void)hipFree(arena_mem);
         if (up)
             (void)hipStreamDestroy(up);
     }
@@ -1080,9 +1096,12 @@ extern "C" int trd_server_create(const trd_server_cfg *cfg, trd_server **out)
     const int resident = rd::launch_capacity(256);
     if (resident <= 0)
         return fail(TW_EDEVICE, "the rdma kernel fits no workgroup on the GPU");
-    s->max_blocks = s->cfg.max_blocks > 0 ? std::min(s->cfg.max_blocks, resident) : std::max(1, std::min(resident, cus / 8));
+    s->max_blocks = s->cfg.max_blocks > 0 ? std::min(s->cfg.max_blocks, resident) : std::max(2, std::min(resident, cus / 8));
+    if (s->max_blocks < 2)
+        return fail(TW_EINVAL, "max_blocks: at least 2 (one per class of worlds)");
     s->hpw = s->cfg.hosts_per_wave > 0 ? std::min(s->cfg.hosts_per_wave, 64) : 16;
-    s->cap = 4096;
+    s->lane[0].blocks = std::max(1, s->max_blocks / 4);
+    s->lane[1].blocks = s->max_blocks - s->lane[0].blocks;
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess)
         return fail(TW_EDEVICE, "hipMemGetInfo failed");
@@ -1092,13 +1111,17 @@ extern "C" int trd_server_create(const trd_server_cfg *cfg, trd_server **out)
     s->arena.base = (char *)s->arena_mem;
     s->arena.size = want;
     s->arena.free_[0] = want;
-    if (hipStreamCreateWithFlags(&s->ks, hipStreamNonBlocking) != hipSuccess || hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&s->done_ev, hipEventDisableTiming) != hipSuccess)
+    if (hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess)
         return fail(TW_EDEVICE, "hipStreamCreate failed");
-    if (hipMalloc((void **)&s->queue_d, sizeof(int) * s->cap) != hipSuccess || hipMalloc((void **)&s->qnext_d, sizeof(unsigned)) != hipSuccess ||
-        hipMalloc((void **)&s->teams_d, sizeof(rd::TeamState) * s->max_blocks) != hipSuccess ||
-        hipMalloc((void **)&s->cfg_d, sizeof(rd::LaunchCfg)) != hipSuccess || hipMalloc((void **)&s->sims_d, sizeof(void *) * s->cap) != hipSuccess)
-        return fail(TW_ENOMEM, "hipMalloc failed");
+    for (auto &l : s->lane)
+    {
+        if (hipStreamCreateWithFlags(&l.ks, hipStreamNonBlocking) != hipSuccess || hipEventCreateWithFlags(&l.done_ev, hipEventDisableTiming) != hipSuccess)
+            return fail(TW_EDEVICE, "hipStreamCreate failed");
+        if (hipMalloc((void **)&l.queue_d, sizeof(int) * RSRV_CAP) != hipSuccess || hipMalloc((void **)&l.qnext_d, sizeof(unsigned)) != hipSuccess ||
+            hipMalloc((void **)&l.teams_d, sizeof(rd::TeamState) * l.blocks) != hipSuccess ||
+            hipMalloc((void **)&l.cfg_d, sizeof(rd::LaunchCfg)) != hipSuccess || hipMalloc((void **)&l.sims_d, sizeof(void *) * RSRV_CAP) != hipSuccess)
+            return fail(TW_ENOMEM, "hipMalloc failed");
+    }
     *out = s.release();
     return TW_OK;
 }
@@ -1109,36 +1132,83 @@ extern "C" int trd_server_submit(trd_server *s, const trd_world *w, int32_t *id)
         return fail(TW_EINVAL, "trd_server_submit: null argument");
     RSrvReq r;
     r.w.reset(new trd_world(*w));
+    r.items = (w->H + s->hpw - 1) / s->hpw + w->S;
     s->reqs.push_back(std::move(r));
     *id = (int32_t)s->reqs.size() - 1;
     s->waiting.push_back(*id);
     return TW_OK;
 }
 
-// the end of a launch: its requests' worlds that finished
-static int rsrv_collect(trd_server &s)
+// the end of a class's launch: its requests' worlds that finished
+static int rsrv_collect(trd_server &s, RSrvLane &l)
 {
-    vector<rd::WorldRun> runs(s.launched.size());
-    for (size_t i = 0; i < s.launched.size(); i++)
-        if (hipMemcpyAsync(&runs[i], s.reqs[s.launched[i]].b->s.run, sizeof(rd::WorldRun), hipMemcpyDeviceToHost, s.ks) != hipSuccess)
+    vector<rd::WorldRun> runs(l.launched.size());
+    for (size_t i = 0; i < l.launched.size(); i++)
+        if (hipMemcpyAsync(&runs[i], s.reqs[l.launched[i]].b->s.run, sizeof(rd::WorldRun), hipMemcpyDeviceToHost, l.ks) != hipSuccess)
             return fail(TW_EDEVICE, "hipMemcpy failed");
-    vector<rd::TeamState> ts(s.max_blocks);
-    if (hipMemcpyAsync(ts.data(), s.teams_d, sizeof(rd::TeamState) * s.max_blocks, hipMemcpyDeviceToHost, s.ks) != hipSuccess ||
-        hipStreamSynchronize(s.ks) != hipSuccess)
+    vector<rd::TeamState> ts(l.blocks);
+    if (hipMemcpyAsync(ts.data(), l.teams_d, sizeof(rd::TeamState) * l.blocks, hipMemcpyDeviceToHost, l.ks) != hipSuccess ||
+        hipStreamSynchronize(l.ks) != hipSuccess)
         return fail(TW_EDEVICE, "the rdma kernel failed: %s", hipGetErrorString(hipGetLastError()));
     bool err = false;
-    for (int t = 0; t < s.L.T && t < s.max_blocks; t++)
-        err |= ts[t].err != 0;
-    for (size_t i = 0; i < s.launched.size(); i++)
+    for (auto &t : ts)
+        err |= t.err != 0;
+    for (size_t i = 0; i < l.launched.size(); i++)
     {
-        RSrvReq &r = s.reqs[s.launched[i]];
+        RSrvReq &r = s.reqs[l.launched[i]];
         if (err)
             r.state = TW_REQ_FAILED;
         else if (runs[i].finished)
             r.state = TW_REQ_DONE;
     }
-    s.launched.clear();
+    l.launched.clear();
     return err ? fail(TW_EDEVICE, "a team barrier timed out") : TW_OK;
+}
+
+// a launch of class c over its resident, unfinished requests
+static int rsrv_launch(trd_server &s, int c)
+{
+    RSrvLane &l = s.lane[c];
+    l.queue_h.clear();
+    l.sims_h.clear();
+    l.launched.clear();
+    int mx = 1;
+    for (size_t i = 0; i < s.reqs.size() && (int)l.launched.size() < RSRV_CAP; i++)
+    {
+        RSrvReq &r = s.reqs[i];
+        if (r.state != TW_REQ_RESIDENT || (r.items > RSRV_SMALL_ITEMS) != (c == 1))
+            continue;
+        l.queue_h.push_back((int)l.sims_h.size() << rd::QSHIFT);
+        l.sims_h.push_back(r.b->sim_d);
+        l.launched.push_back((int)i);
+        mx = std::max(mx, r.items);
+    }
+    if (l.launched.empty())
+        return TW_OK;
+    rd::LaunchCfg &L = l.L;
+    L = rd::LaunchCfg{};
+    L.T = std::max(1, std::min(std::min(32, (mx + 3) / 4), l.blocks));
+    L.nq = (int)l.queue_h.size();
+    L.hpw = s.hpw;
+    L.queue = l.queue_d;
+    L.qnext = l.qnext_d;
+    L.teams = l.teams_d;
+    L.sims = l.sims_d;
+    const int nteams = std::max(1, std::min(L.nq, l.blocks / L.T));
+    if (hipMemcpyAsync(l.queue_d, l.queue_h.data(), sizeof(int) * L.nq, hipMemcpyHostToDevice, l.ks) != hipSuccess ||
+        hipMemcpyAsync(l.sims_d, l.sims_h.data(), sizeof(void *) * L.nq, hipMemcpyHostToDevice, l.ks) != hipSuccess ||
+        hipMemcpyAsync(l.cfg_d, &L, sizeof L, hipMemcpyHostToDevice, l.ks) != hipSuccess ||
+        hipMemsetAsync(l.qnext_d, 0, sizeof(unsigned), l.ks) != hipSuccess ||
+        hipMemsetAsync(l.teams_d, 0, sizeof(rd::TeamState) * l.blocks, l.ks) != hipSuccess)
+        return fail(TW_EDEVICE, "launch set-up failed");
+    const int e = rd::launch_worlds(l.cfg_d, nteams * L.T, 256, l.ks);
+    if (e)
+        return fail(TW_EDEVICE, "rdma kernel launch failed (%d)", e);
+    if (hipEventRecord(l.done_ev, l.ks) != hipSuccess)
+        return fail(TW_EDEVICE, "hipEventRecord failed");
+    l.in_flight = true;
+    s.launches++;
+    return TW_OK;
 }
 
 extern "C" int trd_server_poll(trd_server *s, trd_server_stats *st)
@@ -1146,17 +1216,19 @@ extern "C" int trd_server_poll(trd_server *s, trd_server_stats *st)
     if (!s)
         return fail(TW_EINVAL, "trd_server_poll: null server");
     int rc = TW_OK;
-    if (s->in_flight)
-    {
-        const hipError_t q = hipEventQuery(s->done_ev);
-        if (q == hipSuccess)
+    for (auto &l : s->lane)
+        if (l.in_flight)
         {
-            s->in_flight = false;
-            rc = rsrv_collect(*s);
+            const hipError_t q = hipEventQuery(l.done_ev);
+            if (q == hipSuccess)
+            {
+                l.in_flight = false;
+                if (int e = rsrv_collect(*s, l))
+                    rc = e;
+            }
+            else if (q != hipErrorNotReady)
+                return fail(TW_EDEVICE, "the rdma kernel failed: %s", hipGetErrorString(q));
         }
-        else if (q != hipErrorNotReady)
-            return fail(TW_EDEVICE, "the rdma kernel failed: %s", hipGetErrorString(q));
-    }
     // admissions, in arrival order, while the arena has room
     while (s->wait_head < s->waiting.size())
     {
@@ -1187,50 +1259,10 @@ extern "C" int trd_server_poll(trd_server *s, trd_server_stats *st)
         r.b = b.release();
         r.state = TW_REQ_RESIDENT;
     }
-    // a launch over the resident, unfinished requests
-    if (!s->in_flight)
-    {
-        s->queue_h.clear();
-        s->sims_h.clear();
-        s->launched.clear();
-        int mx = 1;
-        for (size_t i = 0; i < s->reqs.size() && (int)s->launched.size() < s->cap; i++)
-        {
-            RSrvReq &r = s->reqs[i];
-            if (r.state != TW_REQ_RESIDENT)
-                continue;
-            s->queue_h.push_back((int)s->sims_h.size() << rd::QSHIFT);
-            s->sims_h.push_back(r.b->sim_d);
-            s->launched.push_back((int)i);
-            mx = std::max(mx, (r.b->hs[0] + s->hpw - 1) / s->hpw + r.b->ss[0]);
-        }
-        if (!s->launched.empty())
-        {
-            rd::LaunchCfg &L = s->L;
-            L = rd::LaunchCfg{};
-            L.T = std::max(1, std::min(std::min(32, (mx + 3) / 4), s->max_blocks));
-            L.nq = (int)s->queue_h.size();
-            L.hpw = s->hpw;
-            L.queue = s->queue_d;
-            L.qnext = s->qnext_d;
-            L.teams = s->teams_d;
-            L.sims = s->sims_d;
-            const int nteams = std::max(1, std::min(L.nq, s->max_blocks / L.T));
-            if (hipMemcpyAsync(s->queue_d, s->queue_h.data(), sizeof(int) * L.nq, hipMemcpyHostToDevice, s->ks) != hipSuccess ||
-                hipMemcpyAsync(s->sims_d, s->sims_h.data(), sizeof(void *) * L.nq, hipMemcpyHostToDevice, s->ks) != hipSuccess ||
-                hipMemcpyAsync(s->cfg_d, &L, sizeof L, hipMemcpyHostToDevice, s->ks) != hipSuccess ||
-                hipMemsetAsync(s->qnext_d, 0, sizeof(unsigned), s->ks) != hipSuccess ||
-                hipMemsetAsync(s->teams_d, 0, sizeof(rd::TeamState) * s->max_blocks, s->ks) != hipSuccess)
-                return fail(TW_EDEVICE, "launch set-up failed");
-            const int e = rd::launch_worlds(s->cfg_d, nteams * L.T, 256, s->ks);
-            if (e)
-                return fail(TW_EDEVICE, "rdma kernel launch failed (%d)", e);
-            if (hipEventRecord(s->done_ev, s->ks) != hipSuccess)
-                return fail(TW_EDEVICE, "hipEventRecord failed");
-            s->in_flight = true;
-            s->launches++;
-        }
-    }
+    for (int c = 0; c < 2; c++)
+        if (!s->lane[c].in_flight)
+            if (int e = rsrv_launch(*s, c))
+                return e;
     if (st)
     {
         *st = trd_server_stats{};
