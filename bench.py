@@ -410,7 +410,7 @@ def main():
     ap.add_argument("--serve_requests", type=int, default=1000)
     ap.add_argument("--serve_rate", type=float, default=100.0)
     ap.add_argument("--serve_check", type=int, default=20, help="every n-th request is checked against the oracle")
-    ap.add_argument("--serve_rdma", type=float, default=0.25, help="the share of the serving loop's requests that are RDMA worlds")
+    ap.add_argument("--serve_rdma", type=float, default=0.1, help="the share of the serving loop's requests that are RDMA worlds")
     ap.add_argument("--serve_rdma_blocks", type=int, default=64, help="workgroups the RDMA serving loop's launches take")
     ap.add_argument("--rdma", type=int, default=1, help="also run config 3, the RDMA model (the \"rdma\" object)")
     a = ap.parse_args()

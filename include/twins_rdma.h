@@ -71,8 +71,9 @@ void trd_batch_destroy(trd_batch *b);
 
 /* The serving loop for RDMA requests (with include/twins.h's TW_REQ_* states): each request is a
  * world, packed into a device arena the server allocates once (no hipMalloc or hipFree while kernels
- * run); launches of the slot kernel run the resident, unfinished requests' worlds, small worlds and
- * large ones on two streams at once (a quarter of the workgroups for the small); requests that
+ * run); launches of the slot kernel run the resident, unfinished requests' worlds, small worlds (a
+ * workgroup each, a quarter of the workgroups) and large ones (teams of up to 16) on two streams at
+ * once; requests that
  * arrive during a launch (at most 1 s) join the next. Launches use at most max_blocks workgroups
  * (0: an eighth of the CUs; a TCP serving loop, tw_server_*, sized to leave them free runs side by
  * side). One host thread drives a server: submit, then poll until a request is done.

@@ -1187,7 +1187,9 @@ static int rsrv_launch(trd_server &s, int c)
         return TW_OK;
     rd::LaunchCfg &L = l.L;
     L = rd::LaunchCfg{};
-    L.T = std::max(1, std::min(std::min(32, (mx + 3) / 4), l.blocks));
+    // (small worlds: a workgroup each, as many at once as the class's share holds; large: teams of up
+    // to 16, so that several run at once)
+    L.T = c == 0 ? 1 : std::max(1, std::min(std::min(16, (mx + 3) / 4), l.blocks));
     L.nq = (int)l.queue_h.size();
     L.hpw = s.hpw;
     L.queue = l.queue_d;
