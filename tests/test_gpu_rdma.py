@@ -66,3 +66,35 @@ def test_gpu_cli_one_job(tmp_path):
     job = [j for j in rdma_util.jobs() if j[0] == "k8_incast16_b2_cc3"][0]
     assert [l for l in r.stdout.splitlines() if l.startswith("drops ")] == [job[5]]
     assert "flows done 160 of 160" in r.stdout
+
+
+def test_gpu_rdma_serving_loop():
+    """The RDMA serving loop: reference jobs of both size classes submitted one after another while
+    earlier ones run; every request's flows identical to the reference's, whichever requests
+    shared its launches."""
+    import time
+    import twins_amd
+    from twins_amd import rdma
+    I = rdma_util.INPUTS
+    sel = [j for j in rdma_util.jobs(max_ref_s=3.0) if j[1] in ("ft4.topo", "fattree8.txt")][:10]
+    print(f"rdma serving loop: {len(sel)} requests", flush=True)
+    srv = rdma.Server()
+    ids = []
+    for n, t, f, kv, _, _ in sel:
+        w = rdma.World(rdma_util.CONFIG, f"{I}/{t}", f"{I}/{f}", kv)
+        ids.append((n, srv.submit(w)))
+        srv.poll()
+    t0 = time.time()
+    done = {}
+    while len(done) < len(ids) and time.time() - t0 < 100:
+        srv.poll()
+        for n, i in ids:
+            if n not in done and srv.state(i) in (twins_amd.REQ_DONE, twins_amd.REQ_FAILED):
+                assert srv.state(i) == twins_amd.REQ_DONE, n
+                start, fct, ev = srv.result(i)
+                done[n] = "".join(f"{j} {start[j]} {fct[j]}\n" for j in range(len(fct)) if fct[j] >= 0)
+        time.sleep(0.001)
+    srv.close()
+    assert len(done) == len(ids)
+    bad = [n for n, _ in ids if done[n] != rdma_util.ref_fct(n)]
+    assert not bad, bad

@@ -526,15 +526,28 @@ struct trd_batch
         *p = (T *)d;
         return TW_OK;
     }
+    // host copies of the uploads, kept until the stream has taken them (one synchronization at the
+    // end of the packing, not one per array: a serving loop's packing waits less for the device)
+    vector<vector<char>> staged;
     template <typename T>
     int upload(T **p, const vector<T> &v)
     {
         int e = alloc(p, v.size());
         if (e)
             return e;
-        if (!v.empty() && hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess)
+        if (v.empty())
+            return TW_OK;
+        staged.emplace_back((const char *)v.data(), (const char *)v.data() + v.size() * sizeof(T));
+        if (hipMemcpyAsync(*p, staged.back().data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess)
             return fail(TW_EDEVICE, "hipMemcpy failed");
-        return hipStreamSynchronize(st) == hipSuccess ? TW_OK : fail(TW_EDEVICE, "hipStreamSynchronize failed");
+        return TW_OK;
+    }
+    int sync_uploads()
+    {
+        if (hipStreamSynchronize(st) != hipSuccess)
+            return fail(TW_EDEVICE, "hipStreamSynchronize failed");
+        staged.clear();
+        return TW_OK;
     }
     template <typename T>
     int upload_c(const T **p, const vector<T> &v)
@@ -833,6 +846,7 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
         b.ss[wi] = worlds[wi]->S;
     }
     TRY(b.upload(&b.sims_d, vector<const rd::Sim *>{b.sim_d}));
+    TRY(b.sync_uploads());
     if (b.arena)
         return TW_OK; // (a server's batch: the server launches it)
     // launches: the queue of worlds, the teams' barrier state
