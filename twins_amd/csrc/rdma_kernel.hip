@@ -19,7 +19,8 @@ constexpr int WAVES = 4;                               // waves of a 256-thread 
 
 __device__ __forceinline__ unsigned ld_acquire(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
 
-__global__ void __launch_bounds__(256) rdma_team_kernel(const LaunchCfg *__restrict__ lp)
+// (its LDS holds one workgroup per CU: one wave per SIMD, so all registers may be used)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) rdma_team_kernel(const LaunchCfg *__restrict__ lp)
 {
     const LaunchCfg L = *lp;
     __shared__ SwWork work[WAVES];

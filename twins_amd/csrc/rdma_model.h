@@ -17,8 +17,12 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define RD_HD __host__ __device__ __forceinline__
+// (a cold path kept out of line: ties between simultaneous events are rare, and a chain comparison
+// inlined at each of its call sites made the kernel's code far larger than the instruction cache)
+#define RD_COLD __host__ __device__ __noinline__
 #else
 #define RD_HD inline
+#define RD_COLD inline
 #endif
 
 namespace rd
@@ -255,6 +259,9 @@ struct OutBucket
     int64_t last;
     uint32_t txs[2]; // its last two frame sizes and their tx times (a table lookup saved)
     int32_t txv[2];
+    uint64_t moff;   // its link's ring (a copy: no load of the link per frame)
+    uint32_t coff, nb, cap;
+    int64_t delay;   // its link's delay
 };
 
 // a switch port's cursor over its slot's arrivals and its out bucket (per port, in HBM)
@@ -266,6 +273,9 @@ struct PortRun
     int nout;
     uint64_t abase;
     OutBucket ob;
+    int64_t idelay;   // its in-link's delay and rate class (read-only after the slot's set-up)
+    uint32_t irc;
+    uint64_t ibps;
 };
 
 struct Due
@@ -399,6 +409,12 @@ RD_HD int64_t ns_seconds(double x)
 }
 // DataRate(bps).CalculateTxTime(bytes) in ns
 RD_HD int64_t tx_ns(uint64_t bps, uint32_t bytes) { return ns_seconds((double)bytes * 8 / (double)bps); }
+RD_HD int64_t rc_tx(const Sim &s, uint32_t rc, uint64_t bps, uint32_t bytes)
+{
+    if (bytes < (uint32_t)TXT_N)
+        return s.txt[(size_t)rc * TXT_N + bytes];
+    return tx_ns(bps, bytes);
+}
 RD_HD int64_t link_tx(const Sim &s, int link, uint32_t bytes)
 {
     const Link &l = s.li[link];
@@ -425,12 +441,18 @@ RD_HD int bucket(const Sim &s, int link, int64_t slot, uint64_t *base)
     *base = l.moff + (uint64_t)b * l.cap;
     return (v >> 16) == slot ? (int)(v & 0xFFFF) : 0;
 }
-RD_HD void ob_open(OutBucket &o, int link, int64_t last)
+RD_HD void ob_open(const Sim &s, OutBucket &o, int link, int64_t last)
 {
+    const Link l = s.li[link];
     o.hi = 0;
     o.link = link;
     o.last = last;
     o.txs[0] = o.txs[1] = 0;
+    o.moff = l.moff;
+    o.coff = l.coff;
+    o.nb = l.nb;
+    o.cap = l.cap;
+    o.delay = l.delay;
 }
 RD_HD int64_t link_txo(const Sim &s, OutBucket &o, uint32_t bytes)
 {
@@ -454,20 +476,19 @@ RD_HD void ob_flush(const Sim &s, OutBucket &o)
 // a place in the bucket of the frame's arrival slot (null: full, counted)
 RD_HD LFrame *ob_push(const Sim &s, uint32_t *pend, int pend_n, int64_t ts, OutBucket &o)
 {
-    const Link &l = s.li[o.link];
     if (ts >= o.hi)
     {
         ob_flush(s, o);
         int64_t a = ts / s.D;
-        int b = (int)(a % (int64_t)l.nb);
-        o.cnt = l.coff + b;
+        int b = (int)(a % (int64_t)o.nb);
+        o.cnt = o.coff + b;
         o.lo = a * s.D;
         o.hi = o.lo + s.D;
-        o.base = l.moff + (uint64_t)b * l.cap;
+        o.base = o.moff + (uint64_t)b * o.cap;
         o.n = (o.last >> 16) == a ? (uint32_t)(o.last & 0xFFFF) : 0;
         pend[a % pend_n] = 1;
     }
-    if (o.n >= l.cap)
+    if (o.n >= o.cap)
     {
         count(s, EV_TX_OVF);
         return nullptr;
@@ -648,7 +669,7 @@ RD_HD void memo_put(const Sim &s, uint64_t ha, uint64_t hb, int r)
 // decides (larger: an earlier ancestor); a chain ending first runs first; with equal leads the
 // deepest differing ordinal decides; distinct roots decide by start order. *by_lead: decided by a
 // lead or an end (a function of the leads alone).
-RD_HD int ccmp_walk(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b, bool *inexact, bool *by_lead)
+RD_COLD int ccmp_walk(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b, bool *inexact, bool *by_lead)
 {
     *inexact = false;
     *by_lead = false;
@@ -701,7 +722,7 @@ RD_HD int ccmp_walk(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, cons
 }
 
 // ccmp_walk with the root rule for lockstep lineages and the memo (src/rdma/rstep.cuh:504-575)
-RD_HD int ccmp(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b, bool *inexact)
+RD_COLD int ccmp(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b, bool *inexact)
 {
     uint64_t ha[NANC + 1], hb[NANC + 1];
     ha[0] = pa ? chash(a.hash, pa) : a.hash;
@@ -791,7 +812,7 @@ RD_HD int pcmp(const Run *a, const Run *b)
     return 0;
 }
 // does (ta, cpush(pa, a)) run before (tb, cpush(pb, b))? Undecided ties keep b (src/rdma/rstep.cuh:635-657)
-RD_HD bool before(const Sim &s, int64_t ta, uint32_t pa, const Chain &a, int64_t tb, uint32_t pb, const Chain &b)
+RD_COLD bool before(const Sim &s, int64_t ta, uint32_t pa, const Chain &a, int64_t tb, uint32_t pb, const Chain &b)
 {
     if (ta != tb)
         return ta < tb;
@@ -805,7 +826,7 @@ RD_HD bool before(const Sim &s, int64_t ta, uint32_t pa, const Chain &a, int64_t
 }
 // a tie first by the prefixes (and the root rule), then by before(): the quick test of
 // src/rdma/rstep.cuh:626-633 that the reference applies to arrivals, TransmitCompletes and sends
-RD_HD bool before_q(const Sim &s, int64_t t, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b)
+RD_COLD bool before_q(const Sim &s, int64_t t, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b)
 {
     Run xa[PRE], xb[PRE];
     cprefix(xa, pa, a);
@@ -1423,7 +1444,7 @@ RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, ui
     c.st = s.hs[h];
     c.link = s.h_link_out[h];
     c.ldelay = s.li[c.link].delay;
-    ob_open(c.ob, c.link, c.st.ob_last);
+    ob_open(s, c.ob, c.link, c.st.ob_last);
     const int64_t in_delay = s.li[lin].delay;
     uint32_t in_sz = 0; // the in-link's last frame size and its tx time
     int64_t in_tx = 0;
@@ -1628,7 +1649,6 @@ RD_HD uint32_t shared_used(const Port &p) { return p.ing > RESERVE ? p.ing - RES
 RD_HD void s_transmit(SCtx &c, int port, Pkt p)
 {
     const Sim &s = *c.s;
-    const int lo = c.pc[port].link_out;
     int64_t tx = link_txo(s, c.w[port].ob, p.size);
     c.pt[port].tc_at = c.now + tx;
     const Chain &cur = sc_cur(c);
@@ -1636,7 +1656,7 @@ RD_HD void s_transmit(SCtx &c, int port, Pkt p)
     p.ord = (uint8_t)(c.ord < 255 ? c.ord : 255);
     c.ord++;
     c.w[port].nout++;
-    int64_t ts = c.now + tx + s.li[lo].delay;
+    int64_t ts = c.now + tx + c.w[port].ob.delay;
     LFrame *f = ob_push(s, c.pend, c.pend_n, ts, c.w[port].ob);
     if (!f)
         return;
@@ -1854,9 +1874,8 @@ RD_HD void sw_peek(const Sim &s, const SCtx &c, int p)
     if (w.ai < w.an)
     {
         const LFrame &f = s.msg[w.abase + w.ai];
-        const int lin = c.pc[p].link_in;
         w.ats = f.ts;
-        w.acode = ccode(link_tx(s, lin, f.p.size) + s.li[lin].delay, f.p.ord);
+        w.acode = ccode(rc_tx(s, w.irc, w.ibps, f.p.size) + w.idelay, f.p.ord);
     }
 }
 
@@ -1887,7 +1906,13 @@ RD_HD void sw_port_open(const Sim &s, SCtx &c, int p, int64_t slot)
     r.ai = 0;
     r.nout = 0;
     r.an = bucket(s, c.pc[p].link_in, slot, &r.abase);
-    ob_open(r.ob, c.pc[p].link_out, c.pt[p].ob_last);
+    {
+        const Link &li = s.li[c.pc[p].link_in];
+        r.idelay = li.delay;
+        r.irc = li.rc;
+        r.ibps = li.bps;
+    }
+    ob_open(s, r.ob, c.pc[p].link_out, c.pt[p].ob_last);
 }
 // a port's slot end: its bucket's count out; busy (sent, or holds frames)?
 RD_HD bool sw_port_close(const Sim &s, SCtx &c, int p, unsigned long long *nout)
@@ -2006,6 +2031,7 @@ struct SwWork
 {
     Port pt[KS];
     PortRun w[KS];
+    PortCfg pc[KS]; // (a copy: no load of a port's configuration per event)
     int cnt[KS], first[KS];
     uint32_t abytes[KS], ing0[KS];
     int16_t mc[KS][KS]; // [egress][ingress]: the ingress's next arrival for the egress (-1: none)
@@ -2037,8 +2063,8 @@ struct SwWork
 RD_HD const LFrame &sw_arr(const Sim &s, const SwWork &k, int p, int j) { return s.msg[k.w[p].abase + (uint64_t)(j - k.first[p])]; }
 RD_HD uint32_t sw_acode(const Sim &s, const SCtx &c, int p, const LFrame &f)
 {
-    const int lin = c.pc[p].link_in;
-    return ccode(link_tx(s, lin, f.p.size) + s.li[lin].delay, f.p.ord);
+    const PortRun &r = c.w[p];
+    return ccode(rc_tx(s, r.irc, r.ibps, f.p.size) + r.idelay, f.p.ord);
 }
 
 // egress port o's events of a safe slot (src/rdma/rstep.cuh:2806-3005)
@@ -2124,9 +2150,14 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
     const int64_t t1 = (slot + 1) * s.D;
     SCtx c;
     sw_ctx(c, s, x, slot, pend, pend_n, k.pt, k.w);
+    c.pc = k.pc;
+#if defined(RD_PROF) && defined(__HIP_DEVICE_COMPILE__)
+    long long pt0 = wall_clock64();
+#endif
     // ports in, arrivals counted
     for (int p = lane; p < deg; p += nlanes)
     {
+        k.pc[p] = s.pc[si.pbase + p];
         k.pt[p] = s.ps[si.pbase + p];
         k.ing0[p] = k.pt[p].ing;
         k.abytes[p] = 0;
@@ -2139,6 +2170,9 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
         k.nout = 0;
     }
     RD_WAVE_SYNC();
+#if defined(RD_PROF) && defined(__HIP_DEVICE_COMPILE__)
+    long long pt1 = wall_clock64();
+#endif
     int total = 0;
     for (int p = 0; p < deg; p++)
         total += k.cnt[p];
@@ -2204,6 +2238,9 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
     if (lane == 0)
         at_add64(&s.ev[safe ? 36 : 37], 1ull); // (profiling builds: safe and serial switch slots)
 #endif
+#if defined(RD_PROF) && defined(__HIP_DEVICE_COMPILE__)
+    long long pt2 = wall_clock64();
+#endif
     if (safe)
     {
         c.safe = true;
@@ -2216,6 +2253,9 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
         s.sw[x] = c.sw;
     }
     RD_WAVE_SYNC();
+#if defined(RD_PROF) && defined(__HIP_DEVICE_COMPILE__)
+    long long pt3 = wall_clock64();
+#endif
     // ports out
     unsigned long long nout = 0;
     bool act = false;
@@ -2250,6 +2290,19 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
     }
     const bool busy = k.act != 0;
     RD_WAVE_SYNC(); // (the workspace is reused by the next switch)
+#if defined(RD_PROF) && defined(__HIP_DEVICE_COMPILE__)
+    // (profiling builds: the phases' time, summed over switch slots: ports in, routes, events, ports out;
+    // and the arrivals)
+    if (lane == 0)
+    {
+        const long long pt4 = wall_clock64();
+        at_add64(&s.ev[24], (unsigned long long)(pt1 - pt0));
+        at_add64(&s.ev[25], (unsigned long long)(pt2 - pt1));
+        at_add64(&s.ev[26], (unsigned long long)(pt3 - pt2));
+        at_add64(&s.ev[27], (unsigned long long)(pt4 - pt3));
+        at_add64(&s.ev[28], (unsigned long long)total);
+    }
+#endif
     return busy;
 }
 
