@@ -276,6 +276,9 @@ struct PortRun
     int64_t idelay;   // its in-link's delay and rate class (read-only after the slot's set-up)
     uint32_t irc;
     uint64_t ibps;
+    uint32_t isz[2];  // the in-link's tx times of full data frames and of ACKs (read-only likewise)
+    int32_t itx[2];
+    Pkt qhead;        // the head of its data queue when not empty (a copy: a dequeue waits for no load)
 };
 
 struct Due
@@ -1624,6 +1627,7 @@ struct SCtx
     const Chain *cur_src;
     uint32_t cur_code;
     bool cur_ok;
+    int pbase; // its first port in the per-port arrays
     bool safe; // a safe slot: this context runs one egress port's events (see switch_slot)
     int ep;    // the running event's port (its tail positions: one writer per port and slot)
     int ord;
@@ -1668,7 +1672,7 @@ RD_HD void s_enqueue(SCtx &c, int port, int q, const Pkt &p)
 {
     const Sim &s = *c.s;
     Port &pt = c.pt[port];
-    const size_t pi = (size_t)s.si[c.x].pbase + port;
+    const size_t pi = (size_t)c.pbase + port;
     if (q == 0)
     {
         if (pt.q0_n >= s.q0_cap)
@@ -1687,6 +1691,8 @@ RD_HD void s_enqueue(SCtx &c, int port, int q, const Pkt &p)
             return;
         }
         s.q3[pi * s.q3_cap + ring_add(pt.q3_head, pt.q3_n, s.q3_cap)] = p;
+        if (pt.q3_n == 0)
+            c.w[port].qhead = p;
         pt.q3_n++;
     }
     pt.bytes_total += p.size;
@@ -1697,7 +1703,7 @@ RD_HD void s_start_q0(SCtx &c, int port)
     Port &pt = c.pt[port];
     if (pt.tc_at >= 0)
         return;
-    const size_t pi = (size_t)s.si[c.x].pbase + port;
+    const size_t pi = (size_t)c.pbase + port;
     Pkt p = s.q0[pi * s.q0_cap + pt.q0_head];
     pt.q0_head = ring_add(pt.q0_head, 1, s.q0_cap);
     pt.q0_n--;
@@ -1792,10 +1798,12 @@ RD_HD void s_dequeue(SCtx &c, int port)
     }
     if ((pt.paused >> PG & 1) || pt.q3_n == 0)
         return;
-    Pkt p = s.q3[((size_t)s.si[c.x].pbase + port) * s.q3_cap + pt.q3_head];
+    const Pkt p = c.w[port].qhead;
     pt.q3_head = ring_add(pt.q3_head, 1, s.q3_cap);
     pt.q3_n--;
     pt.bytes_total -= p.size;
+    if (pt.q3_n > 0) // (the next head's load overlaps this frame's transmission)
+        c.w[port].qhead = s.q3[((size_t)c.pbase + port) * s.q3_cap + pt.q3_head];
     s_start_q3(c, port, p);
 }
 // SwitchNode's route: the egress port of a data frame or ACK (-1: none) and its queue
@@ -1866,6 +1874,11 @@ RD_HD void s_forward(SCtx &c, int inport, Pkt p, int out, int q)
     s_dequeue(c, out);
 }
 
+// a frame's tx time on a port's in-link
+RD_HD int64_t port_in_tx(const Sim &s, const PortRun &r, uint32_t bytes)
+{
+    return bytes == r.isz[0] ? r.itx[0] : bytes == r.isz[1] ? r.itx[1] : rc_tx(s, r.irc, r.ibps, bytes);
+}
 // a port's next arrival of the slot
 RD_HD void sw_peek(const Sim &s, const SCtx &c, int p)
 {
@@ -1875,7 +1888,7 @@ RD_HD void sw_peek(const Sim &s, const SCtx &c, int p)
     {
         const LFrame &f = s.msg[w.abase + w.ai];
         w.ats = f.ts;
-        w.acode = ccode(rc_tx(s, w.irc, w.ibps, f.p.size) + w.idelay, f.p.ord);
+        w.acode = ccode(port_in_tx(s, w, f.p.size) + w.idelay, f.p.ord);
     }
 }
 
@@ -1892,6 +1905,7 @@ RD_HD void sw_ctx(SCtx &c, const Sim &s, int x, int64_t slot, uint32_t *pend, in
     c.w = w;
     c.tc = s.p_tc + si.pbase;
     c.pc = s.pc + si.pbase;
+    c.pbase = si.pbase;
     c.sw = s.sw[x];
     c.safe = false;
     c.cur_ok = false;
@@ -1911,7 +1925,13 @@ RD_HD void sw_port_open(const Sim &s, SCtx &c, int p, int64_t slot)
         r.idelay = li.delay;
         r.irc = li.rc;
         r.ibps = li.bps;
+        r.isz[0] = c.P->mtu + data_hdr(*c.P);
+        r.isz[1] = ack_size(*c.P);
+        r.itx[0] = (int32_t)rc_tx(s, r.irc, r.ibps, r.isz[0]);
+        r.itx[1] = (int32_t)rc_tx(s, r.irc, r.ibps, r.isz[1]);
     }
+    if (c.pt[p].q3_n > 0)
+        r.qhead = s.q3[((size_t)c.pbase + p) * s.q3_cap + c.pt[p].q3_head];
     ob_open(s, r.ob, c.pc[p].link_out, c.pt[p].ob_last);
 }
 // a port's slot end: its bucket's count out; busy (sent, or holds frames)?
@@ -2064,7 +2084,7 @@ RD_HD const LFrame &sw_arr(const Sim &s, const SwWork &k, int p, int j) { return
 RD_HD uint32_t sw_acode(const Sim &s, const SCtx &c, int p, const LFrame &f)
 {
     const PortRun &r = c.w[p];
-    return ccode(rc_tx(s, r.irc, r.ibps, f.p.size) + r.idelay, f.p.ord);
+    return ccode(port_in_tx(s, r, f.p.size) + r.idelay, f.p.ord);
 }
 
 // egress port o's events of a safe slot (src/rdma/rstep.cuh:2806-3005)
