@@ -38,7 +38,7 @@ typedef struct trd_pack_cfg
 {
     int32_t q3cap;       /* frames a switch port's data queue holds (0: 4096; src/rdma/rmain.cu:1037) */
     int32_t tail_log2;   /* the shared ring of chain tails holds 2^tail_log2 nodes (0: 24) */
-    int32_t team_blocks; /* workgroups (of 256 threads) per world (0: from the largest world, at most 32) */
+    int32_t team_blocks; /* workgroups (of 256 threads) per world (0: from the largest world, at most 32, or 16 beyond 8 worlds) */
     int32_t hosts_per_wave; /* hosts side by side on a wave's lanes (0: 16 for batches of up to 8 worlds, else 64) */
 } trd_pack_cfg;
 

@@ -910,7 +910,7 @@ extern "C" int trd_batch_run(trd_batch *b, trd_run_stats *st)
         int mx = 1;
         for (int v : items)
             mx = std::max(mx, v);
-        T = std::min(32, (mx + 3) / 4);
+        T = std::min(b->W <= 8 ? 32 : 16, (mx + 3) / 4); // (large batches: more teams at once)
     }
     T = std::max(1, std::min(T, cap));
     // worlds by work items, largest first
