@@ -37,7 +37,7 @@ void trd_world_destroy(trd_world *w);
 typedef struct trd_pack_cfg
 {
     int32_t q3cap;       /* frames a switch port's data queue holds (0: 4096; src/rdma/rmain.cu:1037) */
-    int32_t tail_log2;   /* the shared ring of chain tails holds 2^tail_log2 nodes (0: 24) */
+    int32_t tail_log2;   /* a world's ring of chain tails holds at most 2^tail_log2 nodes (0: 24; a world of E entities 2^(log2 E + 14), at least 2^16) */
     int32_t team_blocks; /* workgroups (of 256 threads) per world (0: from the largest world, at most 32, or 16 beyond 8 worlds) */
     int32_t hosts_per_wave; /* hosts side by side on a wave's lanes (0: 16 for batches of up to 8 worlds, else 64) */
 } trd_pack_cfg;
@@ -81,7 +81,7 @@ void trd_batch_destroy(trd_batch *b);
 typedef struct trd_server trd_server;
 typedef struct trd_server_cfg
 {
-    int32_t q3cap, tail_log2;  /* per request, as trd_pack_cfg (0: 4096 frames, 2^22 tail nodes) */
+    int32_t q3cap, tail_log2;  /* per request, as trd_pack_cfg (0: 4096 frames, at most 2^22 tail nodes) */
     int32_t hosts_per_wave;    /* 0: 16 */
     int32_t max_blocks;        /* workgroups the launches use at most, at least 2 (0: a CU in eight) */
     uint64_t arena_bytes;      /* 0: half of the free device memory at creation */

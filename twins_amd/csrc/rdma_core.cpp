@@ -761,6 +761,7 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
             hs[h].due_min = INT64_MAX;
             hs[h].next_start = start_off[h];
             hs[h].start_end = start_off[h + 1];
+            hs[h].ta.wbits = (uint64_t)h_world[h] << rd::TAIL_WSHIFT;
         }
         TRY(b.upload(&s.hs, hs));
     }
@@ -794,6 +795,7 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
                 pt = rd::Port{};
                 pt.ob_last = -(1ll << 16);
                 pt.tc_at = -1;
+                pt.ta.wbits = (uint64_t)s_world[x] << rd::TAIL_WSHIFT;
                 // ConfigEcnStream(port, node * SwitchMmu::pCnt + device)
                 mrg_stream6(1, (uint64_t)q.seed * 257 + (uint64_t)(p + 1), 1, pt.rng);
             }
@@ -822,10 +824,23 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
     }
     TRY(b.alloc(&s.pend, (size_t)pend_total));
     {
-        const int lg = cfg.tail_log2 > 0 ? std::min(cfg.tail_log2, 30) : 24;
-        TRY(b.alloc(&s.tail, (size_t)1 << lg));
-        s.tail_mask = ((uint64_t)1 << lg) - 1;
-        TRY(b.alloc(&s.tail_cnt, 1));
+        // a ring of tail nodes per world: 2^(log2(entities) + 14) nodes, from 2^16 to 2^tail_log2
+        const int lgmax = cfg.tail_log2 > 0 ? std::min(cfg.tail_log2, 30) : 24;
+        vector<rd::TailRing> rings(W);
+        uint64_t total = 0;
+        for (int wi = 0; wi < W; wi++)
+        {
+            int lg = 0;
+            while ((1 << lg) < wd[wi].nh + wd[wi].ns)
+                lg++;
+            lg = std::max(16, std::min(lgmax, lg + 14));
+            rings[wi].off = total;
+            rings[wi].mask = ((uint64_t)1 << lg) - 1;
+            total += (uint64_t)1 << lg;
+        }
+        TRY(b.alloc(&s.tail, (size_t)total));
+        TRY(b.upload_c(&s.rings, rings));
+        TRY(b.alloc(&s.tail_cnt, W));
         const uint64_t mcap = 1ull << 22;
         TRY(b.alloc(&s.memo, mcap));
         s.memo_mask = mcap - 1;

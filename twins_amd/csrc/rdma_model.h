@@ -187,8 +187,17 @@ struct Route
 struct TailAlloc
 {
     uint64_t next, end;
+    uint64_t wbits; // its world, in a position's top bits (see TailRing)
 };
 constexpr uint64_t TAIL_CHUNK = 16;
+// Each world has a ring of tail nodes of its own (as a single run has), so that how fast a ring
+// wraps does not depend on the worlds beside it: a position is world << TAIL_WSHIFT | (1 + the
+// world's count), its node at tail[off + (count & mask)].
+constexpr int TAIL_WSHIFT = 40;
+struct TailRing
+{
+    uint64_t off, mask;
+};
 
 // a NIC (QbbNetDevice + RdmaEgressQueue state) and the host's flow starts
 struct Host
@@ -342,8 +351,8 @@ struct Sim
     const int32_t *txt; // [R][TXT_N]
     uint32_t *pend;
     TailNode *tail;
-    unsigned long long *tail_cnt;
-    uint64_t tail_mask;
+    unsigned long long *tail_cnt; // [W]
+    const TailRing *rings;        // [W]
     unsigned long long *memo;
     uint64_t memo_mask;
     int64_t *fct;
@@ -515,8 +524,14 @@ RD_HD uint64_t chash(uint64_t h, uint32_t code)
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
     return z ^ (z >> 31);
 }
+// a tail position's node in the rings
+RD_HD uint64_t tail_slot(const Sim &s, uint64_t pos)
+{
+    const TailRing r = s.rings[pos >> TAIL_WSHIFT];
+    return r.off + (((pos & ((1ull << TAIL_WSHIFT) - 1)) - 1) & r.mask);
+}
 // the chain of an event scheduled with `code` by the event of chain par; the oldest run spills to
-// the shared tail ring when the inline runs are full
+// its world's tail ring when the inline runs are full
 RD_HD void cpush(const Sim &s, TailAlloc &ta, Chain *dst, uint32_t code, const Chain &par)
 {
     Chain c;
@@ -539,11 +554,11 @@ RD_HD void cpush(const Sim &s, TailAlloc &ta, Chain *dst, uint32_t code, const C
         {
             if (ta.next == ta.end)
             {
-                ta.next = at_add64(s.tail_cnt, TAIL_CHUNK) + 1;
+                ta.next = (at_add64(&s.tail_cnt[ta.wbits >> TAIL_WSHIFT], TAIL_CHUNK) + 1) | ta.wbits;
                 ta.end = ta.next + TAIL_CHUNK;
             }
             const uint64_t pos = ta.next++;
-            TailNode &n = s.tail[(pos - 1) & s.tail_mask];
+            TailNode &n = s.tail[tail_slot(s, pos)];
             n.code = par.r[RUNS - 1].code;
             n.cnt = par.r[RUNS - 1].cnt;
             n.next = par.tail;
@@ -589,7 +604,7 @@ RD_HD void cur_load(const Sim &s, Cursor &k)
         k.code = 0;
         return;
     }
-    const TailNode n = s.tail[(k.node - 1) & s.tail_mask];
+    const TailNode n = s.tail[tail_slot(s, k.node)];
     if (n.pos != k.node)
     {
         k.lost = true;
