@@ -138,6 +138,18 @@ int tw_fct_get(tw_batch *b, int world, int64_t *established_ns, int64_t *complet
  * ties (25), packet-events (26); executed slots in [31]. (src/main.cu:1054-1075) */
 int tw_counters_get(tw_batch *b, int world, uint64_t ev[32]);
 
+/* The queue trace (the reference's --qtrace=F --qtrace_from=T0 --qtrace_until=T1; src/main.cu:
+ * 169-174, 985-994): every enqueue at a host or switch queue at T0 <= t < T1 (ps), the first
+ * max_records of them (0: 20,000,000). Call before tw_batch_run; T1 = 0 records nothing, as in
+ * the reference. */
+int tw_qtrace_enable(tw_batch *b, int64_t from_ps, int64_t until_ps, uint64_t max_records);
+/* Writes the trace after tw_batch_run in the reference's format (src/main.cu:1077-1100): one line
+ * "ts_ps node dev src dst kind seq-or-ack qlen" per enqueue (ns-3 node and device numbers, kind
+ * D / A / S / SA, qlen the queue-disc length after it), stable-sorted by time, node and device,
+ * so that each queue's records keep their order (every world's records, by world at equal keys).
+ * *records: the records written. */
+int tw_qtrace_write(tw_batch *b, const char *path, uint64_t *records);
+
 /* One world's run: slots executed, the device time its team spent on it (seconds of the GPU's
  * constant clock, summed over launches), the team's workgroups, whether it finished, and its
  * packet-events. No counterpart in the reference (the autotuner's measurements). */

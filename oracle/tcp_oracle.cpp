@@ -307,10 +307,26 @@ struct Msg
     std::vector<Frame> fr;
 };
 
+// ---------------------------------------------------------------- the queue trace (--qtrace)
+// every enqueue at qt_from <= now < qt_until: time, queue id, the frame, the queue-disc length after
+// it (src/step.cuh:266-279); written by write_qtrace as src/main.cu:1077-1100 does
+struct QRec
+{
+    int64_t ts;
+    int queue;
+    uint32_t flow, sa;
+    int len, qlen;
+    uint8_t flags, to_sender;
+};
+static bool qt_on = false;
+static int64_t qt_from = 0, qt_until = 0;
+static std::vector<QRec> qtrace;
+
 // ---------------------------------------------------------------- egress FIFO (src/step.cuh:167-317)
 struct Queue
 {
     std::vector<Frame> ring; // capacity: the largest queue limit + 2
+    int id = 0;              // the reference's queue id: host h, or H + switch * k + port
     int head = 0, size = 0;
     int64_t link_free = 0, last_start = 0;
     Chain last_chain{};
@@ -383,6 +399,8 @@ struct Fifo
             ev[0]++;
             return false;
         }
+        if (qt_on && now < qt_until && now >= qt_from)
+            qtrace.push_back({now, Q.id, g.flow, g.sa, g.len, nq + 1, g.flags, g.to_sender});
         const int cap = (int)Q.ring.size();
         if (Q.size >= cap)
         {
@@ -2339,7 +2357,7 @@ int main(int argc, char **argv)
     int rtt_ring = 65536, ooo_segs = 512;
     long long seed = 1, run = 1;
     int64_t max_slots = 2000000;
-    std::string flows_file, out_file = "fct.csv", tcp = "dctcp", topo = "fattree", topo_file, fail_spec, ecmp_from;
+    std::string flows_file, out_file = "fct.csv", tcp = "dctcp", topo = "fattree", topo_file, fail_spec, ecmp_from, qtrace_file;
     for (int i = 1; i < argc; i++)
     {
         std::string a = argv[i];
@@ -2353,6 +2371,12 @@ int main(int argc, char **argv)
             out_file = val("--out=");
         else if (is("--tcp="))
             tcp = val("--tcp=");
+        else if (is("--qtrace="))
+            qtrace_file = val("--qtrace="), qt_on = true;
+        else if (is("--qtrace_from="))
+            qt_from = std::stoll(val("--qtrace_from="));
+        else if (is("--qtrace_until="))
+            qt_until = std::stoll(val("--qtrace_until="));
         else if (is("--max_slots="))
             max_slots = std::stoll(val("--max_slots="));
         else if (is("--ecnK="))
@@ -2746,6 +2770,7 @@ int main(int argc, char **argv)
     {
         Host &x = s.hosts[h];
         x.cc = (uint8_t)cc;
+        x.nic.id = h;
         x.nic.ring.resize(ring);
         x.nic.red_k = ecn_k;
         x.nic.qcap = queue;
@@ -2753,6 +2778,7 @@ int main(int argc, char **argv)
     s.ports.resize((size_t)S * k);
     for (auto &p : s.ports)
     {
+        p.id = H + (int)(&p - s.ports.data());
         p.ring.resize(ring);
         p.red_k = ecn_k;
         p.qcap = queue;
@@ -2846,6 +2872,31 @@ int main(int argc, char **argv)
         fprintf(stderr, "warning: a capacity overflowed (active/ooo/due/rtt/ring-ovf); the results may differ from ns-3\n");
     if (ev[25])
         fprintf(stderr, "warning: %llu ties between simultaneous events were undecided by their recorded histories (%d runs)\n", ev[25], RUNS);
+    if (qt_on)
+    {
+        // src/main.cu:1077-1100: stable by (time, node, device), ns-3's node and device numbers
+        auto node = [&](int q) { return q < H ? q : H + (q - H) / k; };
+        auto dev = [&](int q) { return q < H ? 1 : (q - H) % k + 1; };
+        std::stable_sort(qtrace.begin(), qtrace.end(), [&](const QRec &a, const QRec &b) {
+            if (a.ts != b.ts)
+                return a.ts < b.ts;
+            if (node(a.queue) != node(b.queue))
+                return node(a.queue) < node(b.queue);
+            return dev(a.queue) < dev(b.queue);
+        });
+        FILE *q = fopen(qtrace_file.c_str(), "w");
+        if (!q)
+            die("cannot write the queue trace");
+        for (const QRec &r : qtrace)
+        {
+            const Conn &c = s.F[r.flow];
+            const char *kind = ((r.flags & F_SYN) && (r.flags & F_ACK)) ? "SA" : (r.flags & F_DATA) ? "D" : (r.flags & F_SYN) ? "S" : "A";
+            fprintf(q, "%lld %d %d %d %d %s %u %d\n", (long long)r.ts, node(r.queue), dev(r.queue), r.to_sender ? c.dst : c.src,
+                    r.to_sender ? c.src : c.dst, kind, r.sa, r.qlen);
+        }
+        fclose(q);
+        printf("qtrace: %llu records\n", (unsigned long long)qtrace.size());
+    }
     FILE *o = fopen(out_file.c_str(), "w");
     if (!o)
         die("cannot write the output");

@@ -51,3 +51,23 @@ def test_kernel_code_on_cpu_overflow_regime_matches_oracle(harness, tmp_path):
     assert ovf(ro.stdout) == ovf(rh.stdout) > 0
     same, n = util.same_times(o, h)
     assert same == n
+
+
+QT_CASES = ["k8_r30-newreno-sprayfin", "ls_incast16-newreno-base", "ft8f_a2a30-dctcp-base"]
+
+
+@pytest.mark.parametrize("name", QT_CASES)
+def test_kernel_code_qtrace_equals_oracle(harness, name, tmp_path):
+    """--qtrace (every enqueue in [from, until), src/main.cu:1077-1100) from the kernel code is the
+    oracle's, byte for byte, over a window of the run (spray, leaf-spine, failures)."""
+    by_name = {c[0]: c for c in util.cases()}
+    _, fl, ref, args = by_name[name]
+    win = ["--qtrace_from=200000000", "--qtrace_until=400000000"]
+    o, h = tmp_path / "o.txt", tmp_path / "h.txt"
+    subprocess.run([util.oracle_bin()] + args + [f"--flows={fl}", f"--out={tmp_path / 'o.csv'}", f"--qtrace={o}"] + win, cwd=util.GOLDEN,
+                   check=True, capture_output=True, timeout=600)
+    r = subprocess.run([harness] + args + [f"--flows={fl}", f"--out={tmp_path / 'h.csv'}", f"--qtrace={h}"] + win, cwd=util.GOLDEN,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    a, b = o.read_bytes(), h.read_bytes()
+    assert len(a) > 10000 and a == b

@@ -61,3 +61,18 @@ def same_times(ref_csv, out_csv):
 
 def undecided(log):
     return "warning" in log and "undecided" in log
+
+
+def read_qtrace(path):
+    """Records of a queue trace (the reference's --qtrace format): (ts, node, dev, src, dst, kind,
+    seq-or-ack, qlen) per line."""
+    out = []
+    with open(path) as f:
+        for line in f:
+            v = line.split()
+            out.append((int(v[0]), int(v[1]), int(v[2]), int(v[3]), int(v[4]), v[5], int(v[6]), int(v[7])))
+    return out
+
+
+def packet_events(log):
+    return int(log.split("packet-events ")[1].split()[0])

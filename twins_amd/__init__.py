@@ -91,6 +91,8 @@ def lib():
                                        ctypes.POINTER(ctypes.c_uint64)]
         L.tw_server_destroy.argtypes = [vp]
         L.tw_batch_destroy.argtypes = [vp]
+        L.tw_qtrace_enable.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64]
+        L.tw_qtrace_write.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
     return _lib
 
@@ -163,6 +165,16 @@ class Batch:
         est, done = (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)()
         _check(lib().tw_fct_get(self._h, w, est, done))
         return list(est), list(done)
+
+    def qtrace_enable(self, from_ps=0, until_ps=0, max_records=0):
+        """Trace every enqueue at from_ps <= t < until_ps (the reference's --qtrace); before run()."""
+        _check(lib().tw_qtrace_enable(self._h, from_ps, until_ps, max_records))
+
+    def qtrace_write(self, path):
+        """The trace in the reference's text format; returns the records written."""
+        n = ctypes.c_uint64()
+        _check(lib().tw_qtrace_write(self._h, str(path).encode(), ctypes.byref(n)))
+        return n.value
 
     def world_stats(self, w):
         st = WorldStats()

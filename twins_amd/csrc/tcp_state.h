@@ -134,6 +134,28 @@ struct Due
 };
 
 // one world of a batch
+// One enqueue of the queue trace (the reference's --qtrace, src/model.h:312-320): the frame as it
+// entered the queue, and the queue-disc length after it. Node, device and hosts follow on the host
+// from the link and the connection.
+struct QRec
+{
+    int64_t ts;
+    int32_t link;   // global link id
+    uint32_t flow;  // connection (global id)
+    uint32_t sa;
+    uint16_t len, qlen;
+    uint8_t flags, to_sender, pad[6];
+};
+static_assert(sizeof(QRec) == 32, "trace record layout");
+// the trace's buffer: enqueues at from <= now < until, the first cap of them (rec null: no trace)
+struct QTrace
+{
+    QRec *rec;
+    unsigned long long *cnt;
+    uint64_t cap;
+    int64_t from, until;
+};
+
 struct WorldDesc
 {
     int32_t kind, k, hk, H, S, ecmp_n, leaves, spines, hpl, n_fail;
@@ -272,6 +294,7 @@ struct BatchT
     uint32_t *scr_ref;
     uint8_t *scr_up;
     int32_t scr_waves;
+    QTrace qt; // the queue trace (off unless asked for)
 };
 
 } // namespace tw

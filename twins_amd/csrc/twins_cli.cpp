@@ -16,6 +16,7 @@ static int usage()
     fprintf(stderr, "usage: twins (--k=<4..32> | --topo=leafspine --leaves=L --spines=S --hpl=N | --topo=graph --topofile=F)\n"
                     "         (--flows=<csv> [--out=csv] [--tcp=dctcp|newreno|cubic] [--ecnK=30] [--queue=200] [--fin=0|1]\n"
                     "          [--ecmp_from=csv] [--spray=1 [--RngSeed=1 --RngRun=1]] [--chain_runs=9|25]\n"
+                    "          [--qtrace=file --qtrace_from=ps --qtrace_until=ps] (every enqueue in [from, until))\n"
                     "          [--block_threads=256 --blocks=N --team_blocks=T --hosts_per_wave=H] (launch shape; 0: automatic)\n"
                     "          | --worlds=<file>: one world per line, flows=..,tcp=..,ecnK=..,queue=..,fin=..,ecmp_from=..,out=..)\n"
                     "         leaf-spine: max(N+S, L) <= 32\n");
@@ -27,7 +28,8 @@ int main(int argc, char **argv)
     std::vector<std::string> common; // topology and defaults, passed to every world
     std::string flows, out = "fct.csv", worlds_file, tcp = "dctcp", ecmp_from;
     int runs = 9, threads = 0, blocks = 0, team = 0, hpw = 0;
-    std::string ecn_k, queue, fin;
+    std::string ecn_k, queue, fin, qtrace;
+    long long qtrace_from = 0, qtrace_until = 0;
     for (int i = 1; i < argc; i++)
     {
         std::string a = argv[i];
@@ -59,7 +61,13 @@ int main(int argc, char **argv)
             team = atoi(val("--team_blocks=").c_str());
         else if (is("--hosts_per_wave="))
             hpw = atoi(val("--hosts_per_wave=").c_str());
-        else if (a == "--general" || is("--graph_k=") || is("--host_lanes=") || is("--hash_check=") || is("--qtrace"))
+        else if (is("--qtrace="))
+            qtrace = val("--qtrace=");
+        else if (is("--qtrace_from="))
+            qtrace_from = atoll(val("--qtrace_from=").c_str());
+        else if (is("--qtrace_until="))
+            qtrace_until = atoll(val("--qtrace_until=").c_str());
+        else if (a == "--general" || is("--graph_k=") || is("--host_lanes=") || is("--hash_check="))
             ; // the reference's kernel-shape and debugging switches: nothing to choose here
         else if (is("--k=") || is("--topo=") || is("--topofile=") || is("--fail=") || is("--leaves=") || is("--spines=") ||
                  is("--hpl=") || is("--spray=") || is("--RngSeed=") || is("--RngRun=") || is("--max_slots=") || is("--rtt_ring=") ||
@@ -171,6 +179,12 @@ int main(int argc, char **argv)
         fprintf(stderr, "%s\n", tw_last_error());
         return 1;
     }
+    // the queue trace (src/main.cu:169-174, 985-994)
+    if (!qtrace.empty() && tw_qtrace_enable(b, qtrace_from, qtrace_until, 0))
+    {
+        fprintf(stderr, "%s\n", tw_last_error());
+        return 1;
+    }
     tw_run_stats st;
     if (tw_batch_run(b, &st))
     {
@@ -215,6 +229,16 @@ int main(int argc, char **argv)
         fprintf(stderr, "warning: %llu ties between simultaneous events were undecided by their recorded histories (%d runs); "
                         "the results may differ from ns-3: rerun with --chain_runs=25 (longer histories)\n",
                 ev[25], runs);
+    if (!qtrace.empty())
+    {
+        uint64_t n = 0;
+        if (tw_qtrace_write(b, qtrace.c_str(), &n))
+        {
+            fprintf(stderr, "%s\n", tw_last_error());
+            return 1;
+        }
+        printf("qtrace: %llu records\n", (unsigned long long)n); // (src/main.cu:1077-1100)
+    }
     // fct.csv per world (src/main.cu:1143-1153)
     for (size_t i = 0; i < hw.size(); i++)
     {

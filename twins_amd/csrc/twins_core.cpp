@@ -793,6 +793,7 @@ struct tw_batch
     WorldRun *run = nullptr;
     std::vector<WorldRun> run_host;
     int64_t *mg_est = nullptr, *mg_done = nullptr;
+    QTrace qt{}; // the queue trace's buffer, when enabled
     ~tw_batch()
     {
         if (gexec)
@@ -1432,6 +1433,95 @@ extern "C" int tw_counters_get(tw_batch *b, int world, uint64_t ev[32])
     for (int i = 0; i < 32; i++)
         ev[i] = r.ev[i];
     ev[31] = (uint64_t)r.slot;
+    return TW_OK;
+}
+
+// ------------------------------------------------------------------ the queue trace
+extern "C" int tw_qtrace_enable(tw_batch *b, int64_t from_ps, int64_t until_ps, uint64_t max_records)
+{
+    static_assert(offsetof(BatchT<9>, qt) == offsetof(BatchT<25>, qt), "one trace offset");
+    if (!b || b->for_server)
+        return fail(TW_EINVAL, "qtrace: a batch of its own");
+    if (b->qt.rec)
+        return fail(TW_EINVAL, "qtrace: already enabled");
+    QTrace q{};
+    q.cap = max_records ? max_records : 20000000;
+    q.from = from_ps;
+    q.until = until_ps;
+    int rc;
+    if ((rc = dev_alloc(*b, &q.rec, q.cap)) || (rc = dev_zero(*b, &q.cnt, 1)))
+        return rc;
+    HIPCHK(hipMemcpy((char *)b->batch_dev + offsetof(BatchT<9>, qt), &q, sizeof q, hipMemcpyHostToDevice));
+    b->qt = q;
+    return TW_OK;
+}
+
+extern "C" int tw_qtrace_write(tw_batch *b, const char *path, uint64_t *records)
+{
+    if (!b || !path)
+        return fail(TW_EINVAL, "qtrace: no batch or path");
+    if (!b->qt.rec)
+        return fail(TW_EINVAL, "qtrace: not enabled (tw_qtrace_enable)");
+    unsigned long long n = 0;
+    HIPCHK(hipMemcpy(&n, b->qt.cnt, sizeof n, hipMemcpyDeviceToHost));
+    n = std::min<unsigned long long>(n, b->qt.cap);
+    std::vector<QRec> q(n);
+    if (n)
+        HIPCHK(hipMemcpy(q.data(), b->qt.rec, n * sizeof(QRec), hipMemcpyDeviceToHost));
+    // world-local numbering: node and device of the reference's queue id (hosts 0..H-1 on device 1,
+    // then switch x's port p as node H + x on device p + 1), hosts of the connection
+    struct Line
+    {
+        int64_t ts;
+        int32_t world, node, dev, src, dst;
+        uint32_t sa;
+        int qlen;
+        const char *kind;
+    };
+    std::vector<Line> out(n);
+    std::vector<int32_t> link0(b->worlds.size() + 1, 0);
+    for (size_t i = 0; i < b->worlds.size(); i++)
+        link0[i + 1] = link0[i] + b->worlds[i]->L;
+    for (size_t i = 0; i < n; i++)
+    {
+        const QRec &r = q[i];
+        const int wi = (int)(std::upper_bound(link0.begin(), link0.end(), r.link) - link0.begin()) - 1;
+        if (wi < 0 || wi >= (int)b->worlds.size())
+            return fail(TW_EDEVICE, "qtrace: record %zu names link %d", i, r.link);
+        const tw_world &w = *b->worlds[wi];
+        const int l = r.link - link0[wi];
+        const uint32_t c = r.flow - (uint32_t)b->conn0[wi];
+        if (c >= (uint32_t)w.NC)
+            return fail(TW_EDEVICE, "qtrace: record %zu names connection %u", i, r.flow);
+        Line &o = out[i];
+        o.ts = r.ts;
+        o.world = wi;
+        o.node = l < w.H ? l : w.H + (l - w.H) / w.k;
+        o.dev = l < w.H ? 1 : (l - w.H) % w.k + 1;
+        o.src = r.to_sender ? w.cs[c].dst : w.cs[c].src;
+        o.dst = r.to_sender ? w.cs[c].src : w.cs[c].dst;
+        o.sa = r.sa;
+        o.qlen = r.qlen;
+        o.kind = ((r.flags & F_SYN) && (r.flags & F_ACK)) ? "SA" : (r.flags & F_DATA) ? "D" : (r.flags & F_SYN) ? "S" : "A";
+    }
+    std::stable_sort(out.begin(), out.end(), [](const Line &a, const Line &b) {
+        if (a.ts != b.ts)
+            return a.ts < b.ts;
+        if (a.node != b.node)
+            return a.node < b.node;
+        if (a.dev != b.dev)
+            return a.dev < b.dev;
+        return a.world < b.world;
+    });
+    FILE *f = fopen(path, "w");
+    if (!f)
+        return fail(TW_EIO, "cannot write %s", path);
+    for (const Line &o : out)
+        fprintf(f, "%lld %d %d %d %d %s %u %d\n", (long long)o.ts, o.node, o.dev, o.src, o.dst, o.kind, o.sa, o.qlen);
+    if (fclose(f))
+        return fail(TW_EIO, "cannot write %s", path);
+    if (records)
+        *records = n;
     return TW_OK;
 }
 
