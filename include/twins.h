@@ -76,6 +76,10 @@ typedef struct tw_world_info
     uint64_t device_bytes; /* device memory this world needs in a batch */
 } tw_world_info;
 int tw_world_info_get(const tw_world *w, tw_world_info *info);
+/* For a world made with given ECMP hashes (--ecmp_from=F, ns-3's hash_data / hash_ack columns):
+ * the connections whose hashes computed here (ns-3's Hash32 over the 5-tuple) equal F's, of all
+ * connections -- the reference's --hash_check=1 (src/main.cu:723-768). */
+int tw_world_hash_check(const tw_world *w, int32_t *same, int32_t *conns);
 int tw_world_destroy(tw_world *w);
 
 /* An explicit launch plan (the batched sweep's autotuner writes one). Team j has team_blocks[j]

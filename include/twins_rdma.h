@@ -65,6 +65,15 @@ int trd_batch_run(trd_batch *b, trd_run_stats *st);
 int trd_fct_get(const trd_batch *b, int32_t world, int64_t *start_ns, int64_t *fct_ns);
 /* the counters of src/rdma/rmain.cu:2257-2258, by the reference's indices (40 entries) */
 int trd_counters_get(const trd_batch *b, uint64_t *ev40);
+/* The transmission trace (the reference's --txtrace=F --from=T0 --until=T1; src/rdma/rmain.cu:
+ * 1064-1071, 1919-1927): every transmission start at a host or switch port at T0 <= t <= T1 (ns),
+ * the first max_records of them (0: 64 Mi, the reference's). Call before trd_batch_run. */
+int trd_txtrace_enable(trd_batch *b, int64_t from_ns, int64_t until_ns, uint64_t max_records);
+/* Writes one world's trace after the run in the reference's format (src/rdma/rmain.cu:2262-2282):
+ * "ns node dev kind src dst sport seq bytes ecn" per start (kind D A N P R; for PAUSE / RESUME src
+ * is the queue and the rest 0), sorted by time, node, device. *records: lines written; *seen:
+ * records the run made over all worlds (more than max_records: the rest were not kept). */
+int trd_txtrace_write(const trd_batch *b, int32_t world, const char *path, uint64_t *records, uint64_t *seen);
 /* a world's slots run and device time (its team's constant-clock time, s) */
 int trd_world_stats_get(const trd_batch *b, int32_t world, int64_t *slots, double *device_s);
 void trd_batch_destroy(trd_batch *b);

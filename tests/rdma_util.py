@@ -37,3 +37,18 @@ def sum_counters(lines):
         v = [int(x) for x in re.findall(r"\d+", l)]
         tot = v if tot is None else [a + b for a, b in zip(tot, v)]
     return tot
+
+
+# the transmission-trace fixtures (collect_txtrace.py): the reference's --txtrace files of these jobs
+# over this window (ns, both ends included)
+TX_WINDOW = (100000, 112000)
+
+
+def tx_jobs():
+    names = sorted(f[3:-4] for f in os.listdir(os.path.join(RG, "ref")) if f.startswith("tx_") and f.endswith(".txt"))
+    by = {j[0]: j for j in jobs()}
+    return [by[n] for n in names]
+
+
+def ref_txtrace(name):
+    return open(os.path.join(RG, "ref", f"tx_{name}.txt"), "rb").read()

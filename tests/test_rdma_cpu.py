@@ -77,8 +77,31 @@ def test_cli_errors(cpu_bin, kv, msg, tmp_path):
     assert r.returncode == 1 and msg in r.stderr
 
 
+TX = rdma_util.tx_jobs()
+
+
+@pytest.mark.parametrize("job", TX, ids=[j[0] for j in TX])
+def test_cpu_build_txtrace_equals_reference(cpu_bin, job, tmp_path):
+    """--txtrace (every transmission start in [from, until]) equals the reference's own trace file
+    of the job, byte for byte (PFC PAUSE/RESUME, INT-carrying ACKs, ECN marks, graph topologies)."""
+    name, topo, flows, kv, _, _ = job
+    tx = tmp_path / "tx.txt"
+    w0, w1 = rdma_util.TX_WINDOW
+    r = run_cli(cpu_bin, topo, flows, kv + [f"--txtrace={tx}", f"--from={w0}", f"--until={w1}"], tmp_path / "x.fct")
+    assert r.returncode == 0, r.stderr
+    assert tx.read_bytes() == rdma_util.ref_txtrace(name)
+
+
+def test_cli_txtrace_one_world_only(cpu_bin, tmp_path):
+    jobs = tmp_path / "jobs.txt"
+    jobs.write_text("a ft4.topo inc1.csv CC_MODE=1\n")
+    r = subprocess.run([cpu_bin, rdma_util.CONFIG, f"--batch={jobs}", f"--outdir={tmp_path}", f"--path={rdma_util.INPUTS}",
+                        f"--txtrace={tmp_path}/t.txt"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "--txtrace traces one world" in r.stderr
+
+
 def test_cli_refuses_what_is_not_modelled(cpu_bin, tmp_path):
-    r = run_cli(cpu_bin, "ft4.topo", "inc1.csv", ["--txtrace=x.txt"], tmp_path / "x.fct")
+    r = run_cli(cpu_bin, "ft4.topo", "inc1.csv", ["--sent=x.txt"], tmp_path / "x.fct")
     assert r.returncode == 1 and "not modelled" in r.stderr
     dag = tmp_path / "x.dag"
     dag.write_text("flow 0 0 1 1000\nroot 0 0\n")

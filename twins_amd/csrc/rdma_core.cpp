@@ -1027,6 +1027,88 @@ extern "C" int trd_counters_get(const trd_batch *b, uint64_t *ev40)
         return fail(TW_EDEVICE, "hipMemcpy failed");
     return TW_OK;
 }
+// ------------------------------------------------------------------ the transmission trace
+extern "C" int trd_txtrace_enable(trd_batch *b, int64_t from_ns, int64_t until_ns, uint64_t max_records)
+{
+    if (!b || b->arena)
+        return fail(TW_EINVAL, "txtrace: a batch of its own");
+    if (b->s.txt_rec.rec)
+        return fail(TW_EINVAL, "txtrace: already enabled");
+    if (b->ran)
+        return fail(TW_EINVAL, "txtrace: enable it before trd_batch_run");
+    rd::TxTrace t{};
+    t.cap = max_records ? max_records : 64ull << 20;
+    t.from = from_ns;
+    t.until = until_ns;
+    TRY(b->alloc(&t.rec, t.cap));
+    TRY(b->alloc(&t.cnt, 1)); // (zeroed on the batch's stream, ahead of its launches)
+    if (hipMemcpy((char *)b->sim_d + offsetof(rd::Sim, txt_rec), &t, sizeof t, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(TW_EDEVICE, "txtrace: device set-up failed");
+    b->s.txt_rec = t;
+    return TW_OK;
+}
+
+extern "C" int trd_txtrace_write(const trd_batch *b, int32_t world, const char *path, uint64_t *records, uint64_t *seen)
+{
+    if (!b || !path || world < 0 || world >= b->W)
+        return fail(TW_EINVAL, "txtrace: bad batch, world or path");
+    const rd::TxTrace &t = b->s.txt_rec;
+    if (!t.rec)
+        return fail(TW_EINVAL, "txtrace: not enabled (trd_txtrace_enable)");
+    unsigned long long n = 0;
+    if (hipMemcpy(&n, t.cnt, sizeof n, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(TW_EDEVICE, "hipMemcpy failed");
+    if (seen)
+        *seen = n;
+    n = std::min<unsigned long long>(n, t.cap);
+    vector<rd::TxRec> r(n);
+    if (n && hipMemcpy(r.data(), t.rec, n * sizeof(rd::TxRec), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(TW_EDEVICE, "hipMemcpy failed");
+    // this world's records in its own numbering: hosts 0..H-1, switches H.. (src/rdma/rmain.cu:2262-2282)
+    int h0 = 0, s0 = 0;
+    for (int i = 0; i < world; i++)
+        h0 += b->hs[i], s0 += b->ss[i];
+    const int nh = b->hs[world], ns = b->ss[world];
+    vector<rd::TxRec> mine;
+    for (rd::TxRec x : r)
+    {
+        int node;
+        if (x.node < b->H)
+        {
+            if (x.node < h0 || x.node >= h0 + nh)
+                continue;
+            node = x.node - h0;
+        }
+        else
+        {
+            const int sw = x.node - b->H;
+            if (sw < s0 || sw >= s0 + ns)
+                continue;
+            node = nh + sw - s0;
+        }
+        x.node = node;
+        if (x.kind < rd::K_PAUSE)
+            x.a -= (uint32_t)h0, x.b -= (uint32_t)h0;
+        mine.push_back(x);
+    }
+    // a link starts one transmission at a time: (time, node, device) orders them totally
+    std::sort(mine.begin(), mine.end(), [](const rd::TxRec &a, const rd::TxRec &b) {
+        return a.ts != b.ts ? a.ts < b.ts : a.node != b.node ? a.node < b.node : a.dev < b.dev;
+    });
+    FILE *f = fopen(path, "w");
+    if (!f)
+        return fail(TW_EIO, "cannot write %s", path);
+    const char kinds[] = "DANPR";
+    for (const rd::TxRec &x : mine)
+        fprintf(f, "%lld %d %d %c %u %u %u %u %u %u\n", (long long)x.ts, x.node, x.dev, x.kind < 5 ? kinds[x.kind] : '?', x.a, x.b, x.sport,
+                x.seq, (unsigned)x.size, (unsigned)x.ecn);
+    if (fclose(f))
+        return fail(TW_EIO, "cannot write %s", path);
+    if (records)
+        *records = mine.size();
+    return TW_OK;
+}
+
 extern "C" int trd_world_stats_get(const trd_batch *b, int32_t world, int64_t *slots, double *device_s)
 {
     if (!b || world < 0 || world >= b->W || b->runs.empty())

@@ -313,6 +313,28 @@ struct WorldRun
     int64_t ticks;
 };
 
+// One transmission start of the trace (the reference's --txtrace, src/rdma/rdma.h:624-631): node
+// (H + switch for switches) and device over all worlds; hosts of the frame (PAUSE / RESUME: the
+// queue) as all-world ids. The writer turns them into one world's numbers.
+struct TxRec
+{
+    int64_t ts;
+    int32_t node, dev;
+    uint32_t a, b, sport, seq;
+    uint16_t size;
+    uint8_t kind, ecn;
+    uint32_t pad;
+};
+static_assert(sizeof(TxRec) == 40, "TxRec");
+// the trace's buffer: starts at from <= now <= until, the first cap of them (rec null: no trace)
+struct TxTrace
+{
+    TxRec *rec;
+    unsigned long long *cnt;
+    uint64_t cap;
+    int64_t from, until;
+};
+
 struct Sim
 {
     int H, S, F, W, L, R;
@@ -357,6 +379,7 @@ struct Sim
     uint64_t memo_mask;
     int64_t *fct;
     unsigned long long *ev;
+    TxTrace txt_rec; // the transmission trace (off unless asked for)
 };
 
 // ------------------------------------------------------------------ atomics (device) / plain (CPU build)
@@ -986,6 +1009,39 @@ RD_HD bool win_bound(const Params &P, const SndQP &q)
     return q.snd_nxt - q.snd_una >= w;
 }
 
+// a transmission-trace record (src/rdma/rstep.cuh:915-942): out of line, off the path when tracing is off
+RD_COLD void trace_tx(const Sim &s, int64_t now, int node, int dev, const Pkt &p)
+{
+    const TxTrace &t = s.txt_rec;
+    if (now < t.from || now > t.until)
+        return;
+    const unsigned long long i = at_add64(t.cnt, 1ull);
+    if (i >= t.cap)
+        return;
+    TxRec r;
+    r.ts = now;
+    r.node = node;
+    r.dev = dev;
+    r.kind = p.kind;
+    r.size = p.size;
+    r.ecn = p.ecn;
+    r.pad = 0;
+    if (p.kind >= K_PAUSE)
+    {
+        r.a = p.q;
+        r.b = r.sport = r.seq = 0;
+    }
+    else
+    {
+        const bool data = p.kind == K_DATA;
+        r.a = data ? p.src : p.dst;
+        r.b = data ? p.dst : p.src;
+        r.sport = p.sport;
+        r.seq = p.seq;
+    }
+    t.rec[i] = r;
+}
+
 // QbbNetDevice::TransmitStart (src/rdma/rstep.cuh:1056-1095)
 RD_HD void h_transmit(HCtx &c, Pkt p)
 {
@@ -1003,6 +1059,8 @@ RD_HD void h_transmit(HCtx &c, Pkt p)
     f->ts = ts;
     f->c = c.cur;
     f->p = p;
+    if (s.txt_rec.rec)
+        trace_tx(s, c.now, c.h, 1, f->p);
 }
 RD_HD void h_schedule_ns(HCtx &c, int64_t at)
 {
@@ -1682,6 +1740,8 @@ RD_HD void s_transmit(SCtx &c, int port, Pkt p)
     f->ts = ts;
     f->c = cur;
     f->p = p;
+    if (s.txt_rec.rec)
+        trace_tx(s, c.now, c.node, port + 1, f->p);
 }
 RD_HD void s_enqueue(SCtx &c, int port, int q, const Pkt &p)
 {

@@ -15,7 +15,7 @@ static int usage()
 {
     fprintf(stderr, "usage: twins (--k=<4..32> | --topo=leafspine --leaves=L --spines=S --hpl=N | --topo=graph --topofile=F)\n"
                     "         (--flows=<csv> [--out=csv] [--tcp=dctcp|newreno|cubic] [--ecnK=30] [--queue=200] [--fin=0|1]\n"
-                    "          [--ecmp_from=csv] [--spray=1 [--RngSeed=1 --RngRun=1]] [--chain_runs=9|25]\n"
+                    "          [--ecmp_from=csv [--hash_check=1]] [--spray=1 [--RngSeed=1 --RngRun=1]] [--chain_runs=9|25]\n"
                     "          [--qtrace=file --qtrace_from=ps --qtrace_until=ps] (every enqueue in [from, until))\n"
                     "          [--block_threads=256 --blocks=N --team_blocks=T --hosts_per_wave=H] (launch shape; 0: automatic)\n"
                     "          | --worlds=<file>: one world per line, flows=..,tcp=..,ecnK=..,queue=..,fin=..,ecmp_from=..,out=..)\n"
@@ -30,6 +30,7 @@ int main(int argc, char **argv)
     int runs = 9, threads = 0, blocks = 0, team = 0, hpw = 0;
     std::string ecn_k, queue, fin, qtrace;
     long long qtrace_from = 0, qtrace_until = 0;
+    int hash_check = 0;
     for (int i = 1; i < argc; i++)
     {
         std::string a = argv[i];
@@ -67,7 +68,9 @@ int main(int argc, char **argv)
             qtrace_from = atoll(val("--qtrace_from=").c_str());
         else if (is("--qtrace_until="))
             qtrace_until = atoll(val("--qtrace_until=").c_str());
-        else if (a == "--general" || is("--graph_k=") || is("--host_lanes=") || is("--hash_check="))
+        else if (is("--hash_check="))
+            hash_check = atoi(val("--hash_check=").c_str());
+        else if (a == "--general" || is("--graph_k=") || is("--host_lanes="))
             ; // the reference's kernel-shape and debugging switches: nothing to choose here
         else if (is("--k=") || is("--topo=") || is("--topofile=") || is("--fail=") || is("--leaves=") || is("--spines=") ||
                  is("--hpl=") || is("--spray=") || is("--RngSeed=") || is("--RngRun=") || is("--max_slots=") || is("--rtt_ring=") ||
@@ -165,6 +168,23 @@ int main(int argc, char **argv)
             return 1;
         }
         hw.push_back(x);
+    }
+    if (hash_check)
+    {
+        // --hash_check=1: the computed ECMP hashes against --ecmp_from's, then stop (src/main.cu:763-768)
+        int same = 0, n = 0;
+        for (tw_world *x : hw)
+        {
+            int32_t a, c;
+            if (tw_world_hash_check(x, &a, &c))
+            {
+                fprintf(stderr, "%s\n", tw_last_error());
+                return 1;
+            }
+            same += a, n += c;
+        }
+        printf("ECMP hashes: %d of %d connections as ns-3's\n", same, n);
+        return same == n ? 0 : 1;
     }
     tw_pack_cfg cfg;
     memset(&cfg, 0, sizeof cfg);

@@ -105,6 +105,8 @@ struct tw_world
     int32_t cc = CC_DCTCP, ecn_k = RED_K, queue = QDISC_CAP, fin = 0;
     int64_t max_slots = 2000000;
     std::vector<TwMrgState> rng; // per switch (spray)
+    // given ECMP hashes (--ecmp_from): connections whose computed hashes equal them, of hash_n
+    int32_t hash_same = 0, hash_n = -1;
 };
 
 static int build_topology(tw_world &w, const tw_world_desc &d)
@@ -464,12 +466,18 @@ static int build_flows(tw_world &w, const tw_world_desc &d)
             w.cs[c].hash_ack = flow_hash(b, a, dport, sport);
         }
         if (given)
+        {
+            w.hash_same = w.hash_n = 0;
             for (int m = 0; m < M; m++)
                 if (is_first(m))
                 {
-                    w.cs[msg_conn[m]].hash_data = d.hash_data[m];
-                    w.cs[msg_conn[m]].hash_ack = d.hash_ack[m];
+                    ConnStatic &r = w.cs[msg_conn[m]];
+                    w.hash_n++;
+                    w.hash_same += r.hash_data == d.hash_data[m] && r.hash_ack == d.hash_ack[m];
+                    r.hash_data = d.hash_data[m];
+                    r.hash_ack = d.hash_ack[m];
                 }
+        }
     }
     // the sent list / RTT history and the out-of-order blocks, sized per connection
     const uint64_t rtt_ring = d.rtt_ring > 0 ? (uint64_t)d.rtt_ring : 65536, ooo = d.ooo_segs > 0 ? (uint64_t)d.ooo_segs : 512;
@@ -514,6 +522,17 @@ extern "C" int tw_world_create(const tw_world_desc *d, tw_world **out)
             mrg_stream6(d->rng_seed ? d->rng_seed : 1, (uint64_t)(w->H + x), d->rng_run ? d->rng_run : 1, w->rng[x].s);
     }
     *out = w.release();
+    return TW_OK;
+}
+
+extern "C" int tw_world_hash_check(const tw_world *w, int32_t *same, int32_t *conns)
+{
+    if (!w || !same || !conns)
+        return fail(TW_EINVAL, "tw_world_hash_check: null argument");
+    if (w->hash_n < 0)
+        return fail(TW_EINVAL, "tw_world_hash_check: the world was made without given ECMP hashes (--ecmp_from)");
+    *same = w->hash_same;
+    *conns = w->hash_n;
     return TW_OK;
 }
 

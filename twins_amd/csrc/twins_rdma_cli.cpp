@@ -3,7 +3,7 @@
 // 2178-2258, 2316-2373):
 //
 //   twins-rdma CONFIG --topo=F --flows=F --out=F [--rate=100Gbps --delay=1000] [--q3cap=N]
-//              [--tail=log2] [KEY=VALUE ...]
+//              [--tail=log2] [--txtrace=F --from=ns --until=ns] [KEY=VALUE ...]
 //   twins-rdma CONFIG --batch=JOBFILE --outdir=DIR [--path=DIR:DIR...] [options] [KEY=VALUE ...]
 //
 // out: "id start_ns fct_ns" per finished flow, by id. Exit status: 0 every flow done, 2 some
@@ -50,7 +50,8 @@ int main(int argc, char **argv)
     }
     const string config = argv[1];
     string topo, flows, out = "fct.txt", batch, outdir = ".", dirs, rate_s = "100Gbps";
-    int64_t delay = 1000;
+    int64_t delay = 1000, tx_from = 0, tx_until = INT64_MAX;
+    string txtrace;
     trd_pack_cfg pc = {};
     vector<string> kv;
     for (int i = 2; i < argc; i++)
@@ -78,7 +79,13 @@ int main(int argc, char **argv)
             pc.q3cap = atoi(v.c_str());
         else if (k == "--tail")
             pc.tail_log2 = atoi(v.c_str());
-        else if (k == "--txtrace" || k == "--sent" || k == "--astra" || k.compare(0, 5, "--nvl") == 0 || k.compare(0, 5, "--isl") == 0)
+        else if (k == "--txtrace")
+            txtrace = v;
+        else if (k == "--from")
+            tx_from = atoll(v.c_str());
+        else if (k == "--until")
+            tx_until = atoll(v.c_str());
+        else if (k == "--sent" || k == "--astra" || k.compare(0, 5, "--nvl") == 0 || k.compare(0, 5, "--isl") == 0)
         {
             fprintf(stderr, "%s: not modelled in this build\n", k.c_str());
             return 1;
@@ -108,6 +115,11 @@ int main(int argc, char **argv)
         }
         return f;
     };
+    if (!txtrace.empty() && !batch.empty())
+    {
+        fprintf(stderr, "--txtrace traces one world (no --batch)\n"); // (src/rdma/rmain.cu:1199-1202)
+        return 1;
+    }
     vector<Job> jobs;
     if (batch.empty())
         jobs.push_back({"run", topo, flows, kv});
@@ -178,6 +190,11 @@ int main(int argc, char **argv)
         fprintf(stderr, "%s\n", trd_last_error());
         return 1;
     }
+    if (!txtrace.empty() && trd_txtrace_enable(b, tx_from, tx_until, 0))
+    {
+        fprintf(stderr, "%s\n", trd_last_error());
+        return 1;
+    }
     trd_run_stats st;
     if (trd_batch_run(b, &st))
     {
@@ -212,6 +229,17 @@ int main(int argc, char **argv)
            (unsigned long long)ev[11], (unsigned long long)ev[8], (unsigned long long)ev[9], (unsigned long long)ev[4],
            (unsigned long long)ev[5], (unsigned long long)ev[6], (unsigned long long)ev[7], (unsigned long long)ev[12],
            (unsigned long long)ev[10]);
+    if (!txtrace.empty())
+    {
+        uint64_t n = 0, seen = 0;
+        if (trd_txtrace_write(b, 0, txtrace.c_str(), &n, &seen))
+        {
+            fprintf(stderr, "%s\n", trd_last_error());
+            return 1;
+        }
+        if (seen > n)
+            fprintf(stderr, "txtrace: %llu records, %llu kept\n", (unsigned long long)seen, (unsigned long long)n);
+    }
     const bool bad = ev[4] || ev[5] || ev[6] || ev[7] || ev[12] || ev[32];
     if (bad)
         fprintf(stderr, "warning: overflows or late events: the results are not exact\n");

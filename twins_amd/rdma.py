@@ -53,6 +53,8 @@ def lib():
         L.trd_counters_get.argtypes = [vp, ctypes.POINTER(u64)]
         L.trd_world_stats_get.argtypes = [vp, i32, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double)]
         L.trd_batch_destroy.argtypes = [vp]
+        L.trd_txtrace_enable.argtypes = [vp, i64, i64, u64]
+        L.trd_txtrace_write.argtypes = [vp, i32, ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.trd_server_create.argtypes = [ctypes.POINTER(ServerCfg), ctypes.POINTER(vp)]
         L.trd_server_submit.argtypes = [vp, vp, ctypes.POINTER(i32)]
         L.trd_server_poll.argtypes = [vp, ctypes.POINTER(ServerStats)]
@@ -133,6 +135,16 @@ class Batch:
         e = self.counters()
         return (f"drops {e[0]} marks {e[1]} pauses {e[2]} resumes {e[3]} nacks {e[11]} | undecided ties {e[8]}, on lost tails {e[9]} | "
                 f"overflows: tx {e[4]} queue {e[5]} ackq {e[6]} qps {e[7]} due {e[12]} | acks for done QPs {e[10]}")
+
+    def txtrace_enable(self, from_ns=0, until_ns=(1 << 63) - 1, max_records=0):
+        """Trace every transmission start at from_ns <= t <= until_ns (the reference's --txtrace); before run()."""
+        _check(lib().trd_txtrace_enable(self._h, from_ns, until_ns, max_records))
+
+    def txtrace_write(self, w, path):
+        """World w's trace in the reference's text format; returns (lines written, records made over all worlds)."""
+        n, seen = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().trd_txtrace_write(self._h, w, str(path).encode(), ctypes.byref(n), ctypes.byref(seen)))
+        return n.value, seen.value
 
     def world_stats(self, w):
         slots, dev = ctypes.c_int64(), ctypes.c_double()
