@@ -39,7 +39,8 @@ typedef struct trd_pack_cfg
     int32_t q3cap;       /* frames a switch port's data queue holds (0: 4096; src/rdma/rmain.cu:1037) */
     int32_t tail_log2;   /* a world's ring of chain tails holds at most 2^tail_log2 nodes (0: 24; a world of E entities 2^(log2 E + 14), at least 2^16) */
     int32_t team_blocks; /* workgroups (of 256 threads) per world (0: from the largest world, at most 32, or 16 beyond 8 worlds) */
-    int32_t hosts_per_wave; /* hosts side by side on a wave's lanes (0: 16 for batches of up to 8 worlds, else 64) */
+    int32_t hosts_per_wave; /* hosts side by side on a wave's lanes (0: batches of up to 8 worlds 8, or 16 where a world's
+                               work items would outnumber a 32-workgroup team's waves; larger batches 64) */
 } trd_pack_cfg;
 
 /* Worlds side by side on the GPU (the reference's --batch): one slot (the shortest link delay of

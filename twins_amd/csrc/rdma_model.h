@@ -1170,11 +1170,13 @@ RD_HD void hpcc_update(HCtx &c, SndQP &q, const Pkt &p, bool fast)
 {
     const Params &P = *c.P;
     uint32_t next_seq = (uint32_t)q.snd_nxt;
+    // (loops of MAX_HOP trips, so that the frame's hops are indexed statically)
     if (q.last_update_seq == 0)
     {
         q.last_update_seq = next_seq;
-        for (int i = 0; i < p.nhop && i < MAX_HOP; i++)
-            q.hop[i] = p.hop[i];
+        for (int i = 0; i < MAX_HOP; i++)
+            if (i < p.nhop)
+                q.hop[i] = p.hop[i];
         return;
     }
     if (p.nhop <= MAX_HOP)
@@ -1182,8 +1184,10 @@ RD_HD void hpcc_update(HCtx &c, SndQP &q, const Pkt &p, bool fast)
         double U = 0;
         uint64_t dt = 0;
         bool any = false;
-        for (int i = 0; i < p.nhop; i++)
+        for (int i = 0; i < MAX_HOP; i++)
         {
+            if (i >= p.nhop)
+                break;
             if (P.sample_feedback && hop_qlen(p.hop[i]) == 0 && fast)
                 continue;
             any = true;
@@ -1853,7 +1857,11 @@ RD_HD void s_start_q3(SCtx &c, int port, Pkt p)
     if (p.kind == K_DATA && P.cc == 3 && p.nhop < 0xFFFF)
     {
         const uint64_t hv = hop_make((uint64_t)c.now, pt.tx_bytes, pt.bytes_total, c.pc[port].bps, P.int_multi);
-        p.hop[p.nhop % MAX_HOP] = hv;
+        // (static indices only: a frame indexed at run time would leave the registers for scratch memory)
+        const int hi = p.nhop % MAX_HOP;
+        for (int k = 0; k < MAX_HOP; k++)
+            if (k == hi)
+                p.hop[k] = hv;
         p.nhop++;
     }
     pt.tx_bytes += p.size;
@@ -2163,17 +2171,35 @@ RD_HD uint32_t sw_acode(const Sim &s, const SCtx &c, int p, const LFrame &f)
 }
 
 // egress port o's events of a safe slot (src/rdma/rstep.cuh:2806-3005)
+#if defined(RD_PROF) && defined(__HIP_DEVICE_COMPILE__)
+// (profiling builds: a wave's time in a part of the egress loop, counted once per wave)
+#define RD_PROF_T(v) const long long v = wall_clock64()
+#define RD_PROF_ADD(i, d)                                                                                                         \
+    do                                                                                                                            \
+    {                                                                                                                             \
+        if ((int)(__builtin_ffsll((long long)__ballot(1)) - 1) == (int)__lane_id())                                               \
+            at_add64(&s.ev[i], (unsigned long long)(d));                                                                          \
+    } while (0)
+#else
+#define RD_PROF_T(v)
+#define RD_PROF_ADD(i, d)
+#endif
 RD_HD void sw_egress_events(const Sim &s, SCtx &c, SwWork &k, int o, int deg, int64_t t1)
 {
+    // the ingress ports with arrivals left for o (bit p: port p)
+    uint32_t live = 0;
+    for (int p = 0; p < deg; p++)
+        if (k.mc[o][p] >= 0)
+            live |= 1u << p;
     for (;;)
     {
+        RD_PROF_T(q0);
         // the next arrival for o: the ingress heads in port order, the earlier (time, chains) kept
         int bp = -1, kb = -1;
-        for (int p = 0; p < deg; p++)
+        for (uint32_t m = live; m; m &= m - 1)
         {
+            const int p = __builtin_ctz(m);
             const int j = k.mc[o][p];
-            if (j < 0)
-                continue;
             if (bp < 0)
             {
                 bp = p, kb = j;
@@ -2192,6 +2218,8 @@ RD_HD void sw_egress_events(const Sim &s, SCtx &c, SwWork &k, int o, int deg, in
         }
         const int64_t ta = kb >= 0 ? k.ats[kb] : INT64_MAX, tc = k.pt[o].tc_at;
         const bool htc = tc >= 0 && tc < t1;
+        RD_PROF_T(q1);
+        RD_PROF_ADD(29, q1 - q0);
         if (ta == INT64_MAX && !htc)
             break;
         bool arr;
@@ -2214,9 +2242,13 @@ RD_HD void sw_egress_events(const Sim &s, SCtx &c, SwWork &k, int o, int deg, in
             sc_cur(c);
             k.pt[o].tc_at = -1;
             s_dequeue(c, o);
+            RD_PROF_T(q2);
+            RD_PROF_ADD(30, q2 - q1);
             continue;
         }
         k.mc[o][bp] = k.onext[kb];
+        if (k.onext[kb] < 0)
+            live &= ~(1u << bp);
         c.now = ta;
         c.cur_src = &fa->c;
         c.cur_code = acode;
@@ -2233,6 +2265,8 @@ RD_HD void sw_egress_events(const Sim &s, SCtx &c, SwWork &k, int o, int deg, in
         }
         else
             s_forward(c, bp, pk, o, k.q[kb]);
+        RD_PROF_T(q3);
+        RD_PROF_ADD(31, q3 - q1);
     }
 }
 

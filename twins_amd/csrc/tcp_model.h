@@ -202,23 +202,21 @@ struct Model
             if (size > 0 && sent >= MAX_TX)
                 count(R, EV_TXCAP);
         }
-        // a queue-trace record (src/step.cuh:266-279): out of line, off the path when tracing is off
-        __device__ __noinline__ void trace(int64_t now, int qlen, uint32_t flow, uint32_t sa, uint16_t len, uint8_t flags, uint8_t to_sender)
+        // a queue-trace record (src/step.cuh:266-279), stored field by field (no record on the stack)
+        __device__ __forceinline__ void trace(int64_t now, int qlen, uint32_t flow, uint32_t sa, uint16_t len, uint8_t flags, uint8_t to_sender)
         {
             const unsigned long long i = atomicAdd(B->qt.cnt, 1ull);
             if (i < B->qt.cap)
             {
-                QRec r;
-                r.ts = now;
-                r.link = lid;
-                r.flow = flow;
-                r.sa = sa;
-                r.len = len;
-                r.qlen = (uint16_t)qlen;
-                r.flags = flags;
-                r.to_sender = to_sender;
-                r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = r.pad[4] = r.pad[5] = 0;
-                B->qt.rec[i] = r;
+                QRec *r = B->qt.rec + i;
+                r->ts = now;
+                r->link = lid;
+                r->flow = flow;
+                r->sa = sa;
+                r->len = len;
+                r->qlen = (uint16_t)qlen;
+                r->flags = flags;
+                r->to_sender = to_sender;
             }
         }
         // RED (MinTh = MaxTh = K, QW = 1) on the queue-disc length: the frame in ns-3's
