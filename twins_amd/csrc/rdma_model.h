@@ -1043,22 +1043,28 @@ RD_COLD void trace_tx(const Sim &s, int64_t now, int node, int dev, const Pkt &p
 }
 
 // QbbNetDevice::TransmitStart (src/rdma/rstep.cuh:1056-1095)
-RD_HD void h_transmit(HCtx &c, Pkt p)
+// (the packet by reference: one in a ring is copied straight into the frame, not through a stack copy)
+RD_HD void h_transmit(HCtx &c, const Pkt &p)
 {
     const Sim &s = *c.s;
     int64_t tx = link_txo(s, c.ob, p.size);
     c.st.tc_at = c.now + tx;
-    cpush(s, c.st.ta, &s.h_tc[c.h], ccode(tx, c.ord++), c.cur);
-    p.ord = (uint8_t)(c.ord < 255 ? c.ord : 255);
-    c.ord++;
+    // the frame first (the packet is not kept across the chain work), then the chains
+    const int ord = c.ord;
+    c.ord += 2;
     c.nout++;
     int64_t ts = c.now + tx + c.ldelay;
     LFrame *f = ob_push(s, c.pend, c.pend_n, ts, c.ob);
+    if (f)
+    {
+        f->ts = ts;
+        f->p = p;
+        f->p.ord = (uint8_t)(ord + 1 < 255 ? ord + 1 : 255);
+    }
+    cpush(s, c.st.ta, &s.h_tc[c.h], ccode(tx, ord), c.cur);
     if (!f)
         return;
-    f->ts = ts;
     f->c = c.cur;
-    f->p = p;
     if (s.txt_rec.rec)
         trace_tx(s, c.now, c.h, 1, f->p);
 }
@@ -1115,7 +1121,7 @@ RD_HD void h_dequeue(HCtx &c)
     int idx = next_qindex(c);
     if (idx == -1)
     {
-        Pkt p = s.ackq[(size_t)c.h * ACKQ_CAP + c.st.ackq_head];
+        const Pkt &p = s.ackq[(size_t)c.h * ACKQ_CAP + c.st.ackq_head]; // (not reused before the copy into the frame)
         c.st.ackq_head = ring_add(c.st.ackq_head, 1, ACKQ_CAP);
         c.st.ackq_n--;
         h_transmit(c, p);
@@ -1727,23 +1733,28 @@ RD_HD uint32_t pfc_thresh(const SCtx &c, int port)
 }
 RD_HD uint32_t shared_used(const Port &p) { return p.ing > RESERVE ? p.ing - RESERVE : 0; }
 
-RD_HD void s_transmit(SCtx &c, int port, Pkt p)
+RD_HD void s_transmit(SCtx &c, int port, const Pkt &p)
 {
     const Sim &s = *c.s;
     int64_t tx = link_txo(s, c.w[port].ob, p.size);
     c.pt[port].tc_at = c.now + tx;
-    const Chain &cur = sc_cur(c);
-    cpush(s, c.pt[c.ep].ta, &c.tc[port], ccode(tx, c.ord++), cur);
-    p.ord = (uint8_t)(c.ord < 255 ? c.ord : 255);
-    c.ord++;
+    // the frame first (the packet is not kept across the chain work), then the chains
+    const int ord = c.ord;
+    c.ord += 2;
     c.w[port].nout++;
     int64_t ts = c.now + tx + c.w[port].ob.delay;
     LFrame *f = ob_push(s, c.pend, c.pend_n, ts, c.w[port].ob);
+    if (f)
+    {
+        f->ts = ts;
+        f->p = p;
+        f->p.ord = (uint8_t)(ord + 1 < 255 ? ord + 1 : 255);
+    }
+    const Chain &cur = sc_cur(c);
+    cpush(s, c.pt[c.ep].ta, &c.tc[port], ccode(tx, ord), cur);
     if (!f)
         return;
-    f->ts = ts;
     f->c = cur;
-    f->p = p;
     if (s.txt_rec.rec)
         trace_tx(s, c.now, c.node, port + 1, f->p);
 }
@@ -1783,7 +1794,7 @@ RD_HD void s_start_q0(SCtx &c, int port)
     if (pt.tc_at >= 0)
         return;
     const size_t pi = (size_t)c.pbase + port;
-    Pkt p = s.q0[pi * s.q0_cap + pt.q0_head];
+    const Pkt &p = s.q0[pi * s.q0_cap + pt.q0_head]; // (its slot is not reused before the copy into the frame)
     pt.q0_head = ring_add(pt.q0_head, 1, s.q0_cap);
     pt.q0_n--;
     pt.bytes_total -= p.size;
