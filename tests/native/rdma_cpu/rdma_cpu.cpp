@@ -3,12 +3,16 @@
 // host code (rdma_core.cpp, twins_rdma_cli.cpp) and model (rdma_model.h) run as a CPU program
 // (with AddressSanitizer: bounds checks of the kernel code) whose results are compared with the
 // reference's RDMA fixtures. Never linked into the product.
+#include <cstdlib>
 #include "../../../twins_amd/csrc/rdma_model.h"
 #include "../../../twins_amd/csrc/rdma_kernel.h"
 
 namespace rd
 {
 static SwWork g_work; // a switch's slot workspace (LDS on the GPU)
+// TWINS_RDMA_CPU_SERIAL=1: every switch on the serial path of wide switches (switch_step), so that
+// the reference's jobs (switches of at most 32 ports) check that path too
+static const bool g_serial = getenv("TWINS_RDMA_CPU_SERIAL") != nullptr;
 int launch_capacity(int) { return 64; }
 // every world of the queue to its end, one after the other (device memory is host memory here)
 int launch_worlds(const LaunchCfg *cfg, int, int, hipStream_t)
@@ -38,8 +42,8 @@ int launch_worlds(const LaunchCfg *cfg, int, int, hipStream_t)
                     const int x = wd.s0 + (e - wd.nh);
                     bool busy = false;
                     if (s.sw_busy[x] || switch_has_input(s, x, r.slot))
-                        busy = s.si[x].deg <= KS ? switch_slot(s, x, r.slot, pend, wd.pend_n, g_work, 0, 1)
-                                                 : switch_step(s, x, r.slot, pend, wd.pend_n);
+                        busy = s.si[x].deg <= KS && !g_serial ? switch_slot(s, x, r.slot, pend, wd.pend_n, g_work, 0, 1)
+                                                              : switch_step(s, x, r.slot, pend, wd.pend_n);
                     s.sw_busy[x] = busy ? 1 : 0;
                     act |= busy ? 1u : 0u;
                 }

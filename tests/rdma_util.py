@@ -52,3 +52,15 @@ def tx_jobs():
 
 def ref_txtrace(name):
     return open(os.path.join(RG, "ref", f"tx_{name}.txt"), "rb").read()
+
+
+def wide_runs():
+    """The wide-switch fixture (make_wide.py): (name, overrides) per run of its world."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_wide", os.path.join(RG, "make_wide.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m.RUNS
+
+
+WIDE_TOPO, WIDE_FLOWS = "wide_ls2x40.topo", "wide_ls2x40.csv"

@@ -108,6 +108,21 @@ def test_gpu_txtrace():
         assert n == seen == b1.counters()[38] == 68070
 
 
+def test_gpu_wide_switches():
+    """A world whose leaves have 44 ports (more than a switch workspace's 32: the serial path in
+    device memory) under DCQCN, HPCC, DCTCP and with PFC, as one batch: every run's flow completions
+    equal the stored ones (make_wide.py)."""
+    from twins_amd import rdma
+    I = rdma_util.INPUTS
+    runs = rdma_util.wide_runs()
+    print(f"wide switches: {len(runs)} worlds", flush=True)
+    b = rdma.Batch([rdma.World(rdma_util.CONFIG, f"{I}/{rdma_util.WIDE_TOPO}", f"{I}/{rdma_util.WIDE_FLOWS}", kv) for _, kv in runs])
+    st = b.run()
+    assert st.flows_done == 240 * len(runs)
+    for i, (name, _) in enumerate(runs):
+        assert b.fct_text(i) == open(os.path.join(rdma_util.RG, "ref", f"wide_ls2x40_{name}.fct")).read(), name
+
+
 def test_gpu_rdma_serving_loop():
     """The RDMA serving loop: reference jobs of both size classes submitted one after another while
     earlier ones run; every request's flows identical to the reference's, whichever requests

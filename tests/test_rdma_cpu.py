@@ -77,6 +77,35 @@ def test_cli_errors(cpu_bin, kv, msg, tmp_path):
     assert r.returncode == 1 and msg in r.stderr
 
 
+@pytest.mark.parametrize("job", FAST, ids=[j[0] for j in FAST])
+def test_cpu_serial_switch_path_matches_reference_job(cpu_bin, job, tmp_path):
+    """Every switch on the serial path that switches of more than 32 ports take (switch_step, ports
+    in device memory, lane 0 runs all events): the reference's jobs stay identical, counters too."""
+    name, topo, flows, kv, _, counters = job
+    out = tmp_path / (name + ".fct")
+    I = rdma_util.INPUTS
+    r = subprocess.run([cpu_bin, rdma_util.CONFIG, f"--topo={I}/{topo}", f"--flows={I}/{flows}", f"--out={out}"] + kv,
+                       capture_output=True, text=True, timeout=300, env=dict(os.environ, TWINS_RDMA_CPU_SERIAL="1"))
+    assert r.returncode == 0, r.stderr
+    assert out.read_text() == rdma_util.ref_fct(name)
+    assert [l for l in r.stdout.splitlines() if l.startswith("drops ")] == [counters]
+
+
+@pytest.mark.parametrize("run", rdma_util.wide_runs(), ids=[r[0] for r in rdma_util.wide_runs()])
+def test_cpu_wide_switch_world(cpu_bin, run, tmp_path):
+    """Leaves of 44 ports (the serial path): the stored flow completions, every flow done, and no
+    flow faster than its bytes at the line rate plus two link delays."""
+    name, kv = run
+    out = tmp_path / "w.fct"
+    r = run_cli(cpu_bin, rdma_util.WIDE_TOPO, rdma_util.WIDE_FLOWS, kv, out)
+    assert r.returncode == 0, r.stderr
+    assert out.read_text() == open(os.path.join(rdma_util.RG, "ref", f"wide_ls2x40_{name}.fct")).read()
+    sizes = [int(l.split(",")[2]) for l in open(os.path.join(rdma_util.INPUTS, rdma_util.WIDE_FLOWS)).read().splitlines()[1:]]
+    fct = {int(l.split()[0]): int(l.split()[2]) for l in out.read_text().splitlines()}
+    assert len(fct) == len(sizes) == 240
+    assert all(fct[i] >= sizes[i] * 8 // 100 + 2000 for i in fct)
+
+
 TX = rdma_util.tx_jobs()
 
 
