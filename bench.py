@@ -248,8 +248,11 @@ def serve_leg(a):
                         results[i] = list(zip(est, done))
         if time.perf_counter() - last_log > 10:
             last_log = time.perf_counter()
+            # (the TCP requests admitted and not yet taken, by their worlds' device footprint)
+            held = sum(worlds[i].info.device_bytes for i in arrival if not is_rdma[i] and i not in finish and
+                       srv.state(sid[i]) != twins_amd.REQ_WAITING)
             log(f"serve: {now:.0f} s, {submitted} submitted, {len(finish)} done; TCP resident {st.resident}, waiting {st.waiting}, "
-                f"arena {st.arena_used / 2**30:.1f}/{st.arena_bytes / 2**30:.1f} GiB" +
+                f"arena {st.arena_used / 2**30:.1f}/{st.arena_bytes / 2**30:.1f} GiB (admitted worlds {held / 2**30:.1f} GiB)" +
                 (f"; RDMA resident {rst.resident}, waiting {rst.waiting}, arena {rst.arena_used / 2**30:.1f}/{rst.arena_bytes / 2**30:.1f} GiB"
                  if rst else ""))
         time.sleep(0.0005)
