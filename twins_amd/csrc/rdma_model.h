@@ -14,6 +14,14 @@
 #include <climits>
 #include <cmath>
 
+// Device memory, for the device compile: pointers of the Sim record carry the global address space,
+// so the compiler emits global (not flat) loads and stores through them and what is derived from
+// them (a flat access waits on LDS traffic too)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RD_G __attribute__((address_space(1)))
+#else
+#define RD_G
+#endif
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define RD_HD __host__ __device__ __forceinline__
@@ -339,46 +347,46 @@ struct Sim
 {
     int H, S, F, W, L, R;
     int64_t D; // the slot (ns)
-    const Params *Pw;
-    const World *wd;
-    WorldRun *run;
-    const int *h_world, *h_link_in, *h_link_out;
-    const SwInfo *si;
-    const int *s_world;
-    const PortCfg *pc;
-    const Route *rt;
-    const uint8_t *rtp;
-    const FlowInfo *fi;
-    const uint32_t *root_rank;
-    const int *start_list;
-    SndQP *qp;
-    QPChains *qc;
-    RcvQP *rx;
-    Host *hs;
-    Chain *h_tc, *h_ns;
-    Pkt *ackq;
-    int *qps;
+    const Params RD_G *Pw;
+    const World RD_G *wd;
+    WorldRun RD_G *run;
+    const int RD_G *h_world, *h_link_in, *h_link_out;
+    const SwInfo RD_G *si;
+    const int RD_G *s_world;
+    const PortCfg RD_G *pc;
+    const Route RD_G *rt;
+    const uint8_t RD_G *rtp;
+    const FlowInfo RD_G *fi;
+    const uint32_t RD_G *root_rank;
+    const int RD_G *start_list;
+    SndQP RD_G *qp;
+    QPChains RD_G *qc;
+    RcvQP RD_G *rx;
+    Host RD_G *hs;
+    Chain RD_G *h_tc, *h_ns;
+    Pkt RD_G *ackq;
+    int RD_G *qps;
     int qps_cap;
-    Due *due; // [H][MAX_DUE] a host's timers of the slot
-    Switch *sw;
-    int *sw_busy; // [S] it ended its last slot busy (it runs the next one)
-    Port *ps;
-    Chain *p_tc;
-    PortRun *pr;
-    Pkt *q0, *q3;
+    Due RD_G *due; // [H][MAX_DUE] a host's timers of the slot
+    Switch RD_G *sw;
+    int RD_G *sw_busy; // [S] it ended its last slot busy (it runs the next one)
+    Port RD_G *ps;
+    Chain RD_G *p_tc;
+    PortRun RD_G *pr;
+    Pkt RD_G *q0, *q3;
     int q0_cap, q3_cap;
-    const Link *li;
-    LFrame *msg;
-    int64_t *msg_cnt;
-    const int32_t *txt; // [R][TXT_N]
-    uint32_t *pend;
-    TailNode *tail;
-    unsigned long long *tail_cnt; // [W]
-    const TailRing *rings;        // [W]
-    unsigned long long *memo;
+    const Link RD_G *li;
+    LFrame RD_G *msg;
+    int64_t RD_G *msg_cnt;
+    const int32_t RD_G *txt; // [R][TXT_N]
+    uint32_t RD_G *pend;
+    TailNode RD_G *tail;
+    unsigned long long RD_G *tail_cnt; // [W]
+    const TailRing RD_G *rings;        // [W]
+    unsigned long long RD_G *memo;
     uint64_t memo_mask;
-    int64_t *fct;
-    unsigned long long *ev;
+    int64_t RD_G *fct;
+    unsigned long long RD_G *ev;
     TxTrace txt_rec; // the transmission trace (off unless asked for)
 };
 
