@@ -13,6 +13,15 @@
 #pragma once
 #include <cstdint>
 
+// Device memory, for the device compile: pointers of the batch record carry the global address
+// space, so the compiler emits global (not flat) loads and stores through them and what is derived
+// from them (a flat access waits on LDS traffic too)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define TW_G __attribute__((address_space(1)))
+#else
+#define TW_G
+#endif
+
 namespace tw
 {
 
@@ -240,59 +249,59 @@ struct BatchT
     typedef FrameT<RUNS> Frame;
     static constexpr int CW = RUNS + 1; // words of a chain
     int32_t nworlds, nhosts, nswitches, nlinks, nconns, nmsgs;
-    WorldDesc *worlds;
-    WorldRun *run;
+    WorldDesc TW_G *worlds;
+    WorldRun TW_G *run;
     // connections
-    ConnStatic *cs;
-    SndState *snd;
-    RcvState *rcv;
-    uint32_t *rchain, *schain, *dchain, *fchain; // [conn][CW] timer chains
-    uint32_t *it_start, *h_seq, *h_cnt, *oo_s, *oo_e;
-    uint8_t *it_fl;
-    int64_t *h_t;
+    ConnStatic TW_G *cs;
+    SndState TW_G *snd;
+    RcvState TW_G *rcv;
+    uint32_t TW_G *rchain, *schain, *dchain, *fchain; // [conn][CW] timer chains
+    uint32_t TW_G *it_start, *h_seq, *h_cnt, *oo_s, *oo_e;
+    uint8_t TW_G *it_fl;
+    int64_t TW_G *h_t;
     // messages
-    int32_t *mg_conn;
-    uint32_t *mg_end;
-    int64_t *mg_start, *mg_est, *mg_done;
-    int32_t *pend_msg;
+    int32_t TW_G *mg_conn;
+    uint32_t TW_G *mg_end;
+    int64_t TW_G *mg_start, *mg_est, *mg_done;
+    int32_t TW_G *pend_msg;
     // hosts
-    int32_t *host_world, *pend_next, *pend_end, *n_snd, *n_rcv, *act_snd, *act_rcv;
-    uint8_t *host_cc;
-    int64_t *wake;
-    int64_t *tmin_s, *tmin_r; // per host: lower bounds of its connections' sender / receiver timers
-    Due *due; // [host][MAX_DUE]
+    int32_t TW_G *host_world, *pend_next, *pend_end, *n_snd, *n_rcv, *act_snd, *act_rcv;
+    uint8_t TW_G *host_cc;
+    int64_t TW_G *wake;
+    int64_t TW_G *tmin_s, *tmin_r; // per host: lower bounds of its connections' sender / receiver timers
+    Due TW_G *due; // [host][MAX_DUE]
     // per link: the link whose frames reach the entity that sends on it (-1: an unused port)
-    int32_t *in_link;
-    int32_t *sw_world; // switch -> world
+    int32_t TW_G *in_link;
+    int32_t TW_G *sw_world; // switch -> world
     // queues, one per link
-    Frame *qring;
-    uint64_t *q_base;
-    int32_t *q_cap, *q_head, *q_size, *q_red_k, *q_limit;
-    int64_t *q_link_free, *q_last_start;
-    uint32_t *q_last_chain; // [link][CW]
+    Frame TW_G *qring;
+    uint64_t TW_G *q_base;
+    int32_t TW_G *q_cap, *q_head, *q_size, *q_red_k, *q_limit;
+    int64_t TW_G *q_link_free, *q_last_start;
+    uint32_t TW_G *q_last_chain; // [link][CW]
     // link messages [NRING][link][MAX_TX] and their tagged counts slot << 16 | n
-    Frame *msg;
-    long long *msg_cnt;
+    Frame TW_G *msg;
+    long long TW_G *msg_cnt;
     // topology-file tables (all worlds)
-    int32_t *g_host_sw, *g_host_port;
-    uint32_t *g_rt_off;
-    uint8_t *g_rt_n, *g_rt_ports, *g_port_dn;
-    int64_t *g_fail_t;
-    TwMrgState *sw_rng; // per-packet ECMP streams
+    int32_t TW_G *g_host_sw, *g_host_port;
+    uint32_t TW_G *g_rt_off;
+    uint8_t TW_G *g_rt_n, *g_rt_ports, *g_port_dn;
+    int64_t TW_G *g_fail_t;
+    TwMrgState TW_G *sw_rng; // per-packet ECMP streams
     // kernel work queues: queue q holds worlds world_order[wq_off[q] .. wq_off[q] + wq_len[q] - 1] in
     // the order its teams take them, queue_next[q] counts those taken; the plan's teams (team_of:
     // workgroup -> team << 8 | rank in the team) and their barriers
-    int32_t *world_order;
-    int32_t *wq_off, *wq_len;
-    int32_t *queue_next;
+    int32_t TW_G *world_order;
+    int32_t TW_G *wq_off, *wq_len;
+    int32_t TW_G *queue_next;
     int32_t nwq;
-    int32_t *team_of;
-    TeamPlan *plan;
-    TeamState *teams;
+    int32_t TW_G *team_of;
+    TeamPlan TW_G *plan;
+    TeamState TW_G *teams;
     // switch-phase scratch, per wave: [wave][64 * 2 * MAX_TX] arrival times, refs, drawn routes
-    int64_t *scr_ts;
-    uint32_t *scr_ref;
-    uint8_t *scr_up;
+    int64_t TW_G *scr_ts;
+    uint32_t TW_G *scr_ref;
+    uint8_t TW_G *scr_up;
     int32_t scr_waves;
     QTrace qt; // the queue trace (off unless asked for)
 };
