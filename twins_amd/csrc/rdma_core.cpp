@@ -914,7 +914,7 @@ extern "C" int trd_batch_run(trd_batch *b, trd_run_stats *st)
     // the launch shape: hosts per host item (fewer: less divergence among a wave's hosts, more
     // waves), and one team size, from the largest world's work items (about one per wave, at most 32
     // workgroups); as many teams as the GPU holds at once. Small batches take 8 hosts per item where
-    // every world's items still fit one per wave of a 32-workgroup team (config 3, HPCC + DCQCN:
+    // every world's items still fit one per wave of a 32-workgroup team (512-host leaf-spine, HPCC + DCQCN:
     // 1.19 s of kernel against 1.45 s with 16; with 6 the items outnumber the waves: 1.72 s), else 16.
     const int cap = b->max_teams;
     int hpw = b->hpw;
