@@ -129,8 +129,22 @@ def test_cli_txtrace_one_world_only(cpu_bin, tmp_path):
     assert r.returncode == 1 and "--txtrace traces one world" in r.stderr
 
 
+def test_cli_sent_file(cpu_bin, tmp_path):
+    """--sent: the reference records first SendCompletes only under the SimAI transmit rule, so for
+    these worlds its file is empty (checked against its CPU build where that is built)."""
+    sent = tmp_path / "x.sent"
+    r = run_cli(cpu_bin, "ft4.topo", "inc1.csv", ["CC_MODE=1", "BUFFER_SIZE=1", f"--sent={sent}"], tmp_path / "x.fct")
+    assert r.returncode == 0, r.stderr
+    assert sent.exists() and sent.read_text() == ""
+    ref = os.path.join(util.ROOT, "oracle", "_ref", "twins-rdma-cpu")
+    if os.path.exists(ref):
+        rs = tmp_path / "r.sent"
+        rr = run_cli(ref, "ft4.topo", "inc1.csv", ["CC_MODE=1", "BUFFER_SIZE=1", f"--sent={rs}"], tmp_path / "r.fct")
+        assert rr.returncode == 0 and rs.read_text() == sent.read_text()
+
+
 def test_cli_refuses_what_is_not_modelled(cpu_bin, tmp_path):
-    r = run_cli(cpu_bin, "ft4.topo", "inc1.csv", ["--sent=x.txt"], tmp_path / "x.fct")
+    r = run_cli(cpu_bin, "ft4.topo", "inc1.csv", ["--astra=x"], tmp_path / "x.fct")
     assert r.returncode == 1 and "not modelled" in r.stderr
     dag = tmp_path / "x.dag"
     dag.write_text("flow 0 0 1 1000\nroot 0 0\n")

@@ -3,7 +3,7 @@
 // 2178-2258, 2316-2373):
 //
 //   twins-rdma CONFIG --topo=F --flows=F --out=F [--rate=100Gbps --delay=1000] [--q3cap=N]
-//              [--tail=log2] [--txtrace=F --from=ns --until=ns] [KEY=VALUE ...]
+//              [--tail=log2] [--txtrace=F --from=ns --until=ns] [--sent=F] [KEY=VALUE ...]
 //   twins-rdma CONFIG --batch=JOBFILE --outdir=DIR [--path=DIR:DIR...] [options] [KEY=VALUE ...]
 //
 // out: "id start_ns fct_ns" per finished flow, by id. Exit status: 0 every flow done, 2 some
@@ -51,7 +51,7 @@ int main(int argc, char **argv)
     const string config = argv[1];
     string topo, flows, out = "fct.txt", batch, outdir = ".", dirs, rate_s = "100Gbps";
     int64_t delay = 1000, tx_from = 0, tx_until = INT64_MAX;
-    string txtrace;
+    string txtrace, sent;
     trd_pack_cfg pc = {};
     vector<string> kv;
     for (int i = 2; i < argc; i++)
@@ -85,7 +85,9 @@ int main(int argc, char **argv)
             tx_from = atoll(v.c_str());
         else if (k == "--until")
             tx_until = atoll(v.c_str());
-        else if (k == "--sent" || k == "--astra" || k.compare(0, 5, "--nvl") == 0 || k.compare(0, 5, "--isl") == 0)
+        else if (k == "--sent")
+            sent = v;
+        else if (k == "--astra" || k.compare(0, 5, "--nvl") == 0 || k.compare(0, 5, "--isl") == 0)
         {
             fprintf(stderr, "%s: not modelled in this build\n", k.c_str());
             return 1;
@@ -239,6 +241,24 @@ int main(int argc, char **argv)
         }
         if (seen > n)
             fprintf(stderr, "txtrace: %llu records, %llu kept\n", (unsigned long long)seen, (unsigned long long)n);
+    }
+    if (!sent.empty())
+    {
+        // "id ns" per flow at its first SendComplete: the reference records it only under the SimAI
+        // transmit rule (src/rdma/rstep.cuh:1062-1069), which this build's worlds never use (SimAI
+        // topologies are refused), so the file is empty as the reference's is for these worlds
+        // (src/rdma/rmain.cu:2202-2219; with --batch, OUTDIR/JOB.sent per world)
+        for (size_t wi = 0; wi < jobs.size(); wi++)
+        {
+            const string path = batch.empty() ? sent : outdir + "/" + jobs[wi].name + ".sent";
+            FILE *fs = fopen(path.c_str(), "w");
+            if (!fs)
+            {
+                fprintf(stderr, "cannot write %s\n", path.c_str());
+                return 1;
+            }
+            fclose(fs);
+        }
     }
     const bool bad = ev[4] || ev[5] || ev[6] || ev[7] || ev[12] || ev[32];
     if (bad)
