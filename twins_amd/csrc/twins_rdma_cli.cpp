@@ -9,7 +9,7 @@
 // out: "id start_ns fct_ns" per finished flow, by id. Exit status: 0 every flow done, 2 some
 // unfinished at the stop time, 3 an overflow or a late event (the results are not exact), 1 an
 // argument or input error. (The reference's launch-shape flags --serial, --spread, --win, --gwin,
-// --persist and --rank* are accepted and change nothing here.)
+// --persist, --rank* and --dcap are accepted and change nothing here.)
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -92,7 +92,8 @@ int main(int argc, char **argv)
             fprintf(stderr, "%s: not modelled in this build\n", k.c_str());
             return 1;
         }
-        else if (k == "--serial" || k == "--spread" || k == "--win" || k == "--gwin" || k == "--persist" || k.compare(0, 6, "--rank") == 0)
+        else if (k == "--serial" || k == "--spread" || k == "--win" || k == "--gwin" || k == "--persist" || k.compare(0, 6, "--rank") == 0 ||
+                 k == "--dcap") // (--dcap sizes DAG message buffers; DAG flow files are refused)
             continue;
         else if (k.compare(0, 2, "--") != 0 && eq != string::npos)
             kv.push_back(a);
