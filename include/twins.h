@@ -80,6 +80,10 @@ int tw_world_info_get(const tw_world *w, tw_world_info *info);
  * the connections whose hashes computed here (ns-3's Hash32 over the 5-tuple) equal F's, of all
  * connections -- the reference's --hash_check=1 (src/main.cu:723-768). */
 int tw_world_hash_check(const tw_world *w, int32_t *same, int32_t *conns);
+/* A topology-file world's routes in the format of the ns-3 scenario's DUMP_ROUTES: per switch and
+ * host, "route NODE HOST IF..." (ns-3 node number, the output interfaces of the equal-cost routes
+ * before any link failure, numbered from 1) -- the reference's --dump_routes=F (src/main.cu:466-486). */
+int tw_world_dump_routes(const tw_world *w, const char *path);
 int tw_world_destroy(tw_world *w);
 
 /* An explicit launch plan (the batched sweep's autotuner writes one). Team j has team_blocks[j]

@@ -15,7 +15,7 @@ static int usage()
 {
     fprintf(stderr, "usage: twins (--k=<4..32> | --topo=leafspine --leaves=L --spines=S --hpl=N | --topo=graph --topofile=F)\n"
                     "         (--flows=<csv> [--out=csv] [--tcp=dctcp|newreno|cubic] [--ecnK=30] [--queue=200] [--fin=0|1]\n"
-                    "          [--ecmp_from=csv [--hash_check=1]] [--spray=1 [--RngSeed=1 --RngRun=1]] [--chain_runs=9|25]\n"
+                    "          [--ecmp_from=csv [--hash_check=1]] [--dump_routes=file] [--spray=1 [--RngSeed=1 --RngRun=1]] [--chain_runs=9|25]\n"
                     "          [--qtrace=file --qtrace_from=ps --qtrace_until=ps] (every enqueue in [from, until))\n"
                     "          [--block_threads=256 --blocks=N --team_blocks=T --hosts_per_wave=H] (launch shape; 0: automatic)\n"
                     "          | --worlds=<file>: one world per line, flows=..,tcp=..,ecnK=..,queue=..,fin=..,ecmp_from=..,out=..)\n"
@@ -31,6 +31,7 @@ int main(int argc, char **argv)
     std::string ecn_k, queue, fin, qtrace;
     long long qtrace_from = 0, qtrace_until = 0;
     int hash_check = 0;
+    std::string dump_routes;
     for (int i = 1; i < argc; i++)
     {
         std::string a = argv[i];
@@ -68,6 +69,8 @@ int main(int argc, char **argv)
             qtrace_from = atoll(val("--qtrace_from=").c_str());
         else if (is("--qtrace_until="))
             qtrace_until = atoll(val("--qtrace_until=").c_str());
+        else if (is("--dump_routes="))
+            dump_routes = val("--dump_routes=");
         else if (is("--hash_check="))
             hash_check = atoi(val("--hash_check=").c_str());
         else if (a == "--general" || is("--graph_k=") || is("--host_lanes="))
@@ -168,6 +171,12 @@ int main(int argc, char **argv)
             return 1;
         }
         hw.push_back(x);
+    }
+    // --dump_routes=F: the first world's routes (src/main.cu:466-486)
+    if (!dump_routes.empty() && tw_world_dump_routes(hw[0], dump_routes.c_str()))
+    {
+        fprintf(stderr, "%s\n", tw_last_error());
+        return 1;
     }
     if (hash_check)
     {

@@ -536,6 +536,38 @@ extern "C" int tw_world_hash_check(const tw_world *w, int32_t *same, int32_t *co
     return TW_OK;
 }
 
+extern "C" int tw_world_dump_routes(const tw_world *w, const char *path)
+{
+    if (!w || !path)
+        return fail(TW_EINVAL, "tw_world_dump_routes: null argument");
+    if (w->kind != TOPO_GRAPH)
+        return fail(TW_EINVAL, "tw_world_dump_routes: a topology-file world only (--topo=graph)");
+    FILE *o = fopen(path, "w");
+    if (!o)
+        return fail(TW_EIO, "cannot write %s", path);
+    // per switch and host: the ns-3 node (hosts first), the host, the output interfaces (port + 1)
+    // of the equal-cost routes before any link failure (src/main.cu:466-486)
+    const int ns = w->S;
+    for (int x = 0; x < ns; x++)
+        for (int h = 0; h < w->H; h++)
+        {
+            fprintf(o, "route %d %d", w->H + x, h);
+            const int e = w->host_sw[h];
+            if (e == x)
+                fprintf(o, " %d", w->host_port[h] + 1);
+            else
+            {
+                const size_t r = (size_t)x * ns + e;
+                for (int i = 0; i < w->rt_n[r]; i++)
+                    fprintf(o, " %d", w->rt_ports[w->rt_off[r] + i] + 1);
+            }
+            fprintf(o, "\n");
+        }
+    if (fclose(o))
+        return fail(TW_EIO, "cannot write %s", path);
+    return TW_OK;
+}
+
 extern "C" int tw_world_destroy(tw_world *w)
 {
     delete w;
