@@ -235,7 +235,7 @@ def serve_leg(a):
                     start, fct, ev = rsrv.result(sid[i])
                     finish[i] = time.perf_counter() - t_start
                     events += ev[38]
-                    inexact += any(ev[j] for j in rdma.BAD)
+                    inexact += any(ev[j] for j in rdma.INEXACT)
                     if i % a.serve_check == 0 and worlds[i].hosts <= 128:  # (the reference's CPU build: up to k=8)
                         results[i] = "".join(f"{j} {start[j]} {fct[j]}\n" for j in range(len(fct)) if fct[j] >= 0)
             else:
@@ -345,7 +345,7 @@ def rdma_leg(a):
         kern += st.kernel_s
         events += b.counters()[38]
     ev = b.counters()
-    inexact = [rdma.EV_NAMES[i] for i in rdma.BAD if ev[i]]
+    inexact = [rdma.EV_NAMES[i] for i in rdma.INEXACT if ev[i]]
     per_step = events // a.steps
     # the reference's RDMA model built for the CPU: parity target and CPU baseline (both worlds at once)
     ref = os.path.join(ROOT, "oracle", "_ref", "twins-rdma-cpu")

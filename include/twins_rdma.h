@@ -37,10 +37,13 @@ void trd_world_destroy(trd_world *w);
 typedef struct trd_pack_cfg
 {
     int32_t q3cap;       /* frames a switch port's data queue holds (0: 4096; src/rdma/rmain.cu:1037) */
-    int32_t tail_log2;   /* a world's ring of chain tails holds at most 2^tail_log2 nodes (0: 24; a world of E entities 2^(log2 E + 14), at least 2^16) */
+    int32_t tail_log2;   /* tail_exact 0: a world's ring of chain tails holds at most 2^tail_log2 nodes (0: 24; a world of E entities
+                            2^(log2 E + 14), at least 2^16); tail_exact 1: every world's ring holds 2^tail_log2 nodes (the
+                            reference's --tail, src/rdma/rmain.cu:29-31, 1078-1085) */
     int32_t team_blocks; /* workgroups (of 256 threads) per world (0: from the largest world, at most 32, or 16 beyond 8 worlds) */
     int32_t hosts_per_wave; /* hosts side by side on a wave's lanes (0: batches of up to 8 worlds 8, or 16 where a world's
                                work items would outnumber a 32-workgroup team's waves; larger batches 64) */
+    int32_t tail_exact;  /* see tail_log2 */
 } trd_pack_cfg;
 
 /* Worlds side by side on the GPU (the reference's --batch): one slot (the shortest link delay of
@@ -75,6 +78,9 @@ int trd_txtrace_enable(trd_batch *b, int64_t from_ns, int64_t until_ns, uint64_t
  * is the queue and the rest 0), sorted by time, node, device. *records: lines written; *seen:
  * records the run made over all worlds (more than max_records: the rest were not kept). */
 int trd_txtrace_write(const trd_batch *b, int32_t world, const char *path, uint64_t *records, uint64_t *seen);
+/* the largest log2 of the batch's worlds' tail rings: what a run that lost tails (counter 9) had; a
+ * rerun asks for more (the reference's --tail-auto, src/rdma/rmain.cu:2326-2363) */
+int trd_batch_tail_log2(const trd_batch *b, int32_t *log2_nodes);
 /* a world's slots run and device time (its team's constant-clock time, s) */
 int trd_world_stats_get(const trd_batch *b, int32_t world, int64_t *slots, double *device_s);
 void trd_batch_destroy(trd_batch *b);
@@ -107,7 +113,10 @@ int trd_server_submit(trd_server *s, const trd_world *w, int32_t *id);
 /* finished launches' results, admissions, the next launch; *st (may be null): the requests' states */
 int trd_server_poll(trd_server *s, trd_server_stats *st);
 int trd_server_state(const trd_server *s, int32_t id, int32_t *state);
-/* a done request's flows (as trd_fct_get) and counters; its device memory returns to the arena */
+/* a done request's flows (as trd_fct_get) and counters; its device memory returns to the arena. A
+ * failed request's memory returns too, and the call returns an error. A request whose run met an
+ * overwritten chain tail (counter 9) is run again by the server with larger tail rings (2^24, then
+ * 2^26 nodes) before it is done. */
 int trd_server_result(trd_server *s, int32_t id, int64_t *start_ns, int64_t *fct_ns, uint64_t *ev40);
 void trd_server_destroy(trd_server *s);
 

@@ -35,7 +35,7 @@ CASES = [
     ("ft8f_a2a30", T3, "base"), ("ft8f_a2a30", T3, "spray"), ("jf32f_a2a50", ("dctcp",), "base"),
     ("torus6f_a2a30", ("newreno",), "base"),
     ("k16_paper", ("dctcp",), "base"), ("k16_a2a30", ("dctcp", "cubic"), "base"),
-    ("k16_incast16", ("newreno",), "base"),
+    ("k16_incast16", ("newreno",), "base"), ("k16_a2a50s", T3, "base"),
 ]
 TCP = {"dctcp": "TcpDctcp", "newreno": "TcpNewReno", "cubic": "TcpCubic"}
 

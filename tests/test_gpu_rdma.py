@@ -153,3 +153,20 @@ def test_gpu_rdma_serving_loop():
     assert len(done) == len(ids)
     bad = [n for n, _ in ids if done[n] != rdma_util.ref_fct(n)]
     assert not bad, bad
+
+
+def test_gpu_lost_tails_rerun():
+    """Rings of 2^10 tail nodes per world: ties of this PFC incast meet overwritten tail nodes
+    (counter 9); run_exact() runs the worlds again from the start on 2^26-node rings, as the
+    reference's twins-rdma reruns itself, and ends with the reference's completions and counters."""
+    from twins_amd import rdma
+    I = rdma_util.INPUTS
+    name = "inc1_b1_cc1"
+    job = next(j for j in rdma_util.jobs() if j[0] == name)
+    print(f"rdma lost tails: {name} on 2^10-node rings", flush=True)
+    b = rdma.Batch([rdma.World(rdma_util.CONFIG, f"{I}/{job[1]}", f"{I}/{job[2]}", job[3])], tail_log2=10, tail_exact=True)
+    st, reruns = b.run_exact()
+    assert reruns >= 1 and b.tail_log2() >= 26
+    assert b.fct_text(0) == rdma_util.ref_fct(name)
+    assert b.summary_line() == job[5]
+    b.close()

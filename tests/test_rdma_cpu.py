@@ -173,3 +173,32 @@ def test_world_build_on_host_and_no_cpu_fallback():
         pytest.skip("a GPU is present")
     with pytest.raises(twins_amd.TwError, match="no GPU"):
         rdma.Batch([w])
+
+
+def test_cli_lost_tails_rerun(cpu_bin, tmp_path):
+    """Lost tails (src/rdma/rmain.cu:2326-2363): with rings of 2^10 nodes (--tail-auto=10) a tie of
+    this PFC incast meets an overwritten tail node; the command prints the reference's warning,
+    runs the worlds again with 2^26-node rings in the same process and ends with the reference's
+    flow completions and counters."""
+    name = "inc1_b1_cc1"
+    out = tmp_path / "x.fct"
+    r = run_cli(cpu_bin, "ft4.topo", "inc1.csv", ["CC_MODE=1", "BUFFER_SIZE=1", "--tail-auto=10"], out)
+    assert r.returncode == 0, r.stderr
+    lines = [l for l in r.stdout.splitlines() if l.startswith("drops ")]
+    assert len(lines) == 2 and "on lost tails 0 " not in lines[0], r.stdout
+    assert "warning: ties the chains did not decide exactly" in r.stderr and "a larger --tail helps" in r.stderr
+    assert "lost tails: running again with --tail-auto=26" in r.stderr
+    assert lines[1] == rdma_util.jobs()[[j[0] for j in rdma_util.jobs()].index(name)][5]
+    assert out.read_text() == rdma_util.ref_fct(name)
+
+
+def test_cli_tail_set_never_grows(cpu_bin, tmp_path):
+    """--tail=N fixes the rings (the reference reruns only when --tail was not given): one run, the
+    lost tails reported, no rerun."""
+    r = run_cli(cpu_bin, "ft4.topo", "inc1.csv", ["CC_MODE=1", "BUFFER_SIZE=1", "--tail=10"], tmp_path / "x.fct")
+    assert r.returncode == 0, r.stderr
+    lines = [l for l in r.stdout.splitlines() if l.startswith("drops ")]
+    assert len(lines) == 1 and "on lost tails 0 " not in lines[0]
+    assert "running again" not in r.stderr and "a larger --tail helps" in r.stderr
+    r = run_cli(cpu_bin, "ft4.topo", "inc1.csv", ["CC_MODE=1", "--tail=9"], tmp_path / "y.fct")
+    assert r.returncode == 1 and "from 10 to 30" in r.stderr

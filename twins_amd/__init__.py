@@ -5,8 +5,10 @@ line makes): build worlds from `twins`-style arguments, pack them into one devic
 the GPU, read per-flow establishment / completion times and the counters. It never simulates
 anything itself: without the built library, or without a GPU, every call raises.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtwins_core.so")
@@ -64,6 +66,32 @@ REQ_WAITING, REQ_RESIDENT, REQ_DONE, REQ_FAILED, REQ_TAKEN = range(5)
 
 _lib = None
 
+# Live handles, released in order (servers, batches, worlds) by an exit hook while the interpreter
+# and the HIP runtime are still whole; after it, finalizers release nothing (a hipFree from a
+# finalizer run during interpreter shutdown can meet a runtime already torn down).
+_live = {rank: weakref.WeakSet() for rank in range(3)}
+_exiting = False
+
+
+def _track(obj, rank):
+    _live[rank].add(obj)
+
+
+@atexit.register
+def _close_all():
+    global _exiting
+    for rank in range(3):
+        for obj in list(_live[rank]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+    _exiting = True
+
+
+def _release_ok():
+    return _lib is not None and not _exiting
+
 
 def lib():
     """The loaded core library (raises if it has not been built)."""
@@ -112,11 +140,15 @@ class World:
         _check(L.tw_world_create_args(len(args), arr, ctypes.byref(self._h)))
         self.info = WorldInfo()
         _check(L.tw_world_info_get(self._h, ctypes.byref(self.info)))
+        _track(self, 2)
+
+    def close(self):
+        if getattr(self, "_h", None) and _release_ok():
+            _lib.tw_world_destroy(self._h)
+        self._h = None
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
-            _lib.tw_world_destroy(self._h)
-            self._h = None
+        self.close()
 
 
 def launch_capacity(chain_runs=9, block_threads=0):
@@ -154,6 +186,7 @@ class Batch:
                       ctypes.pointer(p) if p is not None else ctypes.POINTER(Plan)())
         self._h = ctypes.c_void_p()
         _check(L.tw_batch_create(hs, len(self.worlds), ctypes.byref(cfg), ctypes.byref(self._h)))
+        _track(self, 1)
 
     def run(self):
         st = RunStats()
@@ -186,10 +219,13 @@ class Batch:
         _check(lib().tw_counters_get(self._h, w, ev))
         return list(ev)
 
-    def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
+    def close(self):
+        if getattr(self, "_h", None) and _release_ok():
             _lib.tw_batch_destroy(self._h)
-            self._h = None
+        self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Server:
@@ -204,6 +240,7 @@ class Server:
         self._h = ctypes.c_void_p()
         self._worlds = {}
         _check(L.tw_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))
+        _track(self, 0)
 
     def submit(self, world):
         i = ctypes.c_int32()
@@ -228,9 +265,9 @@ class Server:
         return list(est), list(done), list(ev)
 
     def close(self):
-        if getattr(self, "_h", None) and _lib is not None:
+        if getattr(self, "_h", None) and _release_ok():
             _lib.tw_server_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self.close()

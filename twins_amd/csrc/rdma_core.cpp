@@ -490,6 +490,7 @@ struct trd_batch
     rd::TeamState *teams_d = nullptr;
     rd::LaunchCfg *cfg_d = nullptr;
     int max_teams = 0;
+    int tail_lg = 0; // the largest world's tail ring: 2^tail_lg nodes
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool ran = false;
@@ -824,16 +825,21 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
     }
     TRY(b.alloc(&s.pend, (size_t)pend_total));
     {
-        // a ring of tail nodes per world: 2^(log2(entities) + 14) nodes, from 2^16 to 2^tail_log2
+        // a ring of tail nodes per world: 2^(log2(entities) + 14) nodes, from 2^16 to 2^tail_log2;
+        // with tail_exact, 2^tail_log2 nodes each
         const int lgmax = cfg.tail_log2 > 0 ? std::min(cfg.tail_log2, 30) : 24;
+        if (cfg.tail_exact && (cfg.tail_log2 < 10 || cfg.tail_log2 > 30))
+            return fail(TW_EINVAL, "tail_log2 %d: from 10 to 30", cfg.tail_log2);
         vector<rd::TailRing> rings(W);
         uint64_t total = 0;
+        b.tail_lg = 0;
         for (int wi = 0; wi < W; wi++)
         {
             int lg = 0;
             while ((1 << lg) < wd[wi].nh + wd[wi].ns)
                 lg++;
-            lg = std::max(16, std::min(lgmax, lg + 14));
+            lg = cfg.tail_exact ? cfg.tail_log2 : std::max(16, std::min(lgmax, lg + 14));
+            b.tail_lg = std::max(b.tail_lg, lg);
             rings[wi].off = total;
             rings[wi].mask = ((uint64_t)1 << lg) - 1;
             total += (uint64_t)1 << lg;
@@ -1118,6 +1124,14 @@ extern "C" int trd_txtrace_write(const trd_batch *b, int32_t world, const char *
     return TW_OK;
 }
 
+extern "C" int trd_batch_tail_log2(const trd_batch *b, int32_t *log2_nodes)
+{
+    if (!b || !log2_nodes)
+        return fail(TW_EINVAL, "trd_batch_tail_log2: null argument");
+    *log2_nodes = b->tail_lg;
+    return TW_OK;
+}
+
 extern "C" int trd_world_stats_get(const trd_batch *b, int32_t world, int64_t *slots, double *device_s)
 {
     if (!b || world < 0 || world >= b->W || b->runs.empty())
@@ -1139,12 +1153,14 @@ extern "C" void trd_batch_destroy(trd_batch *b) { delete b; }
 // workgroups (by default a CU in eight, which a TCP serving loop can leave free: then both loops'
 // teams always fit on the GPU together). A launch ends after 1 s of device time; requests that
 // arrived meanwhile join its class's next launch.
+constexpr int RSRV_TAIL_MAX = 26; // the largest tail ring a rerun of a request gets (2^26 nodes)
 struct RSrvReq
 {
     std::unique_ptr<trd_world> w;
     trd_batch *b = nullptr;
     int state = TW_REQ_WAITING;
-    int items = 0; // work items per slot
+    int items = 0;   // work items per slot
+    int tail_lg = 0; // 0: the server's automatic tail rings; else 2^tail_lg nodes (a rerun after lost tails)
 };
 // one class's launches: its stream, its configuration in device memory and host copies (kept until
 // the launch ends)
@@ -1177,6 +1193,7 @@ struct trd_server
     vector<RSrvReq> reqs;
     vector<int> waiting;
     size_t wait_head = 0;
+    vector<int> redo; // requests to run again with larger tail rings (admitted before the waiting ones)
     ~trd_server()
     {
         for (auto &l : lane)
@@ -1266,6 +1283,12 @@ static int rsrv_collect(trd_server &s, RSrvLane &l)
     for (size_t i = 0; i < l.launched.size(); i++)
         if (hipMemcpyAsync(&runs[i], s.reqs[l.launched[i]].b->s.run, sizeof(rd::WorldRun), hipMemcpyDeviceToHost, l.ks) != hipSuccess)
             return fail(TW_EDEVICE, "hipMemcpy failed");
+    // (each request's lost-tail counter: one world per request batch)
+    vector<uint64_t> lost(l.launched.size(), 0);
+    for (size_t i = 0; i < l.launched.size(); i++)
+        if (hipMemcpyAsync(&lost[i], s.reqs[l.launched[i]].b->s.ev + rd::EV_LOST, sizeof(uint64_t), hipMemcpyDeviceToHost, l.ks) !=
+            hipSuccess)
+            return fail(TW_EDEVICE, "hipMemcpy failed");
     vector<rd::TeamState> ts(l.blocks);
     if (hipMemcpyAsync(ts.data(), l.teams_d, sizeof(rd::TeamState) * l.blocks, hipMemcpyDeviceToHost, l.ks) != hipSuccess ||
         hipStreamSynchronize(l.ks) != hipSuccess)
@@ -1278,6 +1301,16 @@ static int rsrv_collect(trd_server &s, RSrvLane &l)
         RSrvReq &r = s.reqs[l.launched[i]];
         if (err)
             r.state = TW_REQ_FAILED;
+        else if (runs[i].finished && lost[i] && r.tail_lg < RSRV_TAIL_MAX)
+        {
+            // a tie met an overwritten tail node: the request runs again from the start with larger
+            // rings (2^24, then 2^26 nodes), as the reference reruns itself (src/rdma/rmain.cu:2332-2363)
+            delete r.b; // (this class's launch has ended; no kernel uses it)
+            r.b = nullptr;
+            r.tail_lg = r.tail_lg < 24 ? 24 : RSRV_TAIL_MAX;
+            r.state = TW_REQ_WAITING;
+            s.redo.push_back(l.launched[i]);
+        }
         else if (runs[i].finished)
             r.state = TW_REQ_DONE;
     }
@@ -1351,17 +1384,19 @@ extern "C" int trd_server_poll(trd_server *s, trd_server_stats *st)
             else if (q != hipErrorNotReady)
                 return fail(TW_EDEVICE, "the rdma kernel failed: %s", hipGetErrorString(q));
         }
-    // admissions, in arrival order, while the arena has room
-    while (s->wait_head < s->waiting.size())
+    // admissions: reruns first, then in arrival order, while the arena has room
+    while (!s->redo.empty() || s->wait_head < s->waiting.size())
     {
-        RSrvReq &r = s->reqs[s->waiting[s->wait_head]];
+        const bool rerun = !s->redo.empty();
+        RSrvReq &r = s->reqs[rerun ? s->redo.front() : s->waiting[s->wait_head]];
         std::unique_ptr<trd_batch> b(new trd_batch);
         b->arena = &s->arena;
         b->st = s->up;
         b->own_stream = false;
         trd_pack_cfg pc = {};
         pc.q3cap = s->cfg.q3cap;
-        pc.tail_log2 = s->cfg.tail_log2 > 0 ? s->cfg.tail_log2 : 22;
+        pc.tail_log2 = r.tail_lg ? r.tail_lg : s->cfg.tail_log2 > 0 ? s->cfg.tail_log2 : 22;
+        pc.tail_exact = r.tail_lg ? 1 : 0;
         trd_world *wp = r.w.get();
         const int e = pack(*b, &wp, 1, pc);
         if (e == TW_ENOMEM)
@@ -1372,7 +1407,10 @@ extern "C" int trd_server_poll(trd_server *s, trd_server_stats *st)
                 break; // it waits for memory
             // (it does not fit even an empty arena: it never will)
         }
-        s->wait_head++;
+        if (rerun)
+            s->redo.erase(s->redo.begin());
+        else
+            s->wait_head++;
         if (e)
         {
             r.state = TW_REQ_FAILED;
@@ -1415,6 +1453,16 @@ extern "C" int trd_server_result(trd_server *s, int32_t id, int64_t *start_ns, i
     if (!s || id < 0 || id >= (int32_t)s->reqs.size())
         return fail(TW_EINVAL, "trd_server_result: bad request %d", id);
     RSrvReq &r = s->reqs[id];
+    if (r.state == TW_REQ_FAILED)
+    {
+        // a failed request's memory returns to the arena (its class's launch has ended: a request
+        // is marked failed only when the launch's end is collected, or before it is admitted)
+        delete r.b;
+        r.b = nullptr;
+        r.w.reset();
+        r.state = TW_REQ_TAKEN;
+        return fail(TW_EDEVICE, "trd_server_result: request %d failed (a team barrier timed out, or its world could not be packed)", id);
+    }
     if (r.state != TW_REQ_DONE)
         return fail(TW_ESTATE, "trd_server_result: request %d is not done (state %d)", id, r.state);
     int e = trd_fct_get(r.b, 0, start_ns, fct_ns);

@@ -6,16 +6,18 @@ call raises.
 """
 import ctypes
 
-from . import lib as _core_lib, TwError
+from . import lib as _core_lib, TwError, _track, _release_ok
 
 EV_NAMES = {0: "drops", 1: "marks", 2: "pauses", 3: "resumes", 4: "tx-ovf", 5: "queue-ovf", 6: "ackq-ovf", 7: "qps-ovf",
             8: "undecided-ties", 9: "lost-tails", 10: "acks-for-done", 11: "nacks", 12: "due-ovf", 32: "late"}
 BAD = (4, 5, 6, 7, 12, 32)  # counters that make a run inexact (the reference's exit status 3)
+EV_LOST = 9  # ties decided on an overwritten tail node: the run is redone with larger tail rings
+INEXACT = BAD + (EV_LOST,)  # a result that is not known to be exact
 
 
 class PackCfg(ctypes.Structure):
     _fields_ = [("q3cap", ctypes.c_int32), ("tail_log2", ctypes.c_int32), ("team_blocks", ctypes.c_int32),
-                ("hosts_per_wave", ctypes.c_int32)]
+                ("hosts_per_wave", ctypes.c_int32), ("tail_exact", ctypes.c_int32)]
 
 
 class RunStats(ctypes.Structure):
@@ -52,6 +54,7 @@ def lib():
         L.trd_fct_get.argtypes = [vp, i32, ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.trd_counters_get.argtypes = [vp, ctypes.POINTER(u64)]
         L.trd_world_stats_get.argtypes = [vp, i32, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_double)]
+        L.trd_batch_tail_log2.argtypes = [vp, ctypes.POINTER(i32)]
         L.trd_batch_destroy.argtypes = [vp]
         L.trd_txtrace_enable.argtypes = [vp, i64, i64, u64]
         L.trd_txtrace_write.argtypes = [vp, i32, ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(u64)]
@@ -84,11 +87,12 @@ class World:
         _check(L.trd_world_info(self._h, ctypes.byref(h), ctypes.byref(s), ctypes.byref(f), ctypes.byref(rtt), ctypes.byref(bdp)))
         self.hosts, self.switches, self.flows = h.value, s.value, f.value
         self.max_rtt, self.max_bdp = rtt.value, bdp.value
+        _track(self, 2)
 
     def close(self):
-        if self._h:
+        if getattr(self, "_h", None) and _release_ok():
             lib().trd_world_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -98,20 +102,48 @@ class World:
 
 
 class Batch:
-    """Worlds side by side on the GPU (the reference's --batch)."""
+    """Worlds side by side on the GPU (the reference's --batch). tail_log2 with tail_exact: every
+    world's ring of chain tails holds 2^tail_log2 nodes (the reference's --tail); without, it caps
+    the automatic size."""
 
-    def __init__(self, worlds, q3cap=0, tail_log2=0, team_blocks=0, hosts_per_wave=0):
-        L = lib()
+    TAIL_MAX = 27  # the largest ring run_exact() asks for (the reference's rerun limit)
+
+    def __init__(self, worlds, q3cap=0, tail_log2=0, team_blocks=0, hosts_per_wave=0, tail_exact=False):
         self.worlds = list(worlds)
-        arr = (ctypes.c_void_p * len(self.worlds))(*[w._h for w in self.worlds])
-        cfg = PackCfg(q3cap, tail_log2, team_blocks, hosts_per_wave)
+        self._cfg = (q3cap, tail_log2, team_blocks, hosts_per_wave, 1 if tail_exact else 0)
         self._h = ctypes.c_void_p()
-        _check(L.trd_batch_create(arr, len(self.worlds), ctypes.byref(cfg), ctypes.byref(self._h)))
+        self._create()
+        _track(self, 1)
+
+    def _create(self):
+        arr = (ctypes.c_void_p * len(self.worlds))(*[w._h for w in self.worlds])
+        _check(lib().trd_batch_create(arr, len(self.worlds), ctypes.byref(PackCfg(*self._cfg)), ctypes.byref(self._h)))
 
     def run(self):
         st = RunStats()
         _check(lib().trd_batch_run(self._h, ctypes.byref(st)))
         return st
+
+    def tail_log2(self):
+        v = ctypes.c_int32()
+        _check(lib().trd_batch_tail_log2(self._h, ctypes.byref(v)))
+        return v.value
+
+    def run_exact(self):
+        """run(); if a tie met an overwritten tail node, the worlds again from the start on larger
+        rings (2^26, then 2^27 nodes), as the reference's twins-rdma reruns itself
+        (src/rdma/rmain.cu:2332-2363). Returns (stats, reruns)."""
+        reruns = 0
+        while True:
+            st = self.run()
+            lg = self.tail_log2()
+            if not self.counters()[EV_LOST] or lg >= self.TAIL_MAX:
+                return st, reruns
+            self.close()
+            q3cap, _, team_blocks, hpw, _ = self._cfg
+            self._cfg = (q3cap, 26 if lg < 26 else self.TAIL_MAX, team_blocks, hpw, 1)
+            self._create()
+            reruns += 1
 
     def fct(self, w):
         """(start_ns, fct_ns) lists of world w's flows by id; fct -1: not finished."""
@@ -152,9 +184,9 @@ class Batch:
         return slots.value, dev.value
 
     def close(self):
-        if self._h:
+        if getattr(self, "_h", None) and _release_ok():
             lib().trd_batch_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -171,6 +203,7 @@ class Server:
         _check(lib().trd_server_create(ctypes.byref(ServerCfg(q3cap, tail_log2, hosts_per_wave, max_blocks, arena_bytes)),
                                        ctypes.byref(self._h)))
         self.flows = []
+        _track(self, 0)
 
     def submit(self, world):
         i = ctypes.c_int32()
@@ -196,9 +229,9 @@ class Server:
         return list(a)[:n], list(b)[:n], list(ev)
 
     def close(self):
-        if self._h:
+        if getattr(self, "_h", None) and _release_ok():
             lib().trd_server_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
