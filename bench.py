@@ -33,7 +33,11 @@ model built for the CPU (oracle/_ref, src/rdma compiled with TWINS_CPU), which i
 baseline (the two worlds on two cores at once, its event loop's wall time). roofline: 480 B per
 packet-event (a 176-byte link frame written and read, a 64-byte queue entry written and read).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--sweep 0|1] [--serve 0|1] [--rdma 0|1]
+Reference workloads (the "reference_workloads" object): the reference's own single-world speed
+table (data/twins_speed.csv, an RX 9060 XT) on the workloads whose ns-3 outputs are stored here,
+each world alone through the plain launch, the event loop timed, every flow checked against ns-3.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--sweep 0|1] [--serve 0|1] [--rdma 0|1] [--refspeed 0|1]
 """
 import argparse
 import concurrent.futures as cf
@@ -209,8 +213,28 @@ def serve_leg(a):
     cus = resident  # (one slot-kernel workgroup per CU)
     srv = twins_amd.Server(idle_s=2.0, small_teams=resident - cus // 8 - rblocks - 16 * 4 - 4 * 8 if n_rdma else 0)
     rsrv = rdma.Server(max_blocks=rblocks) if n_rdma else None
+    # every serve_check-th request against the oracle (TCP) or the reference's RDMA build, on the
+    # host's cores, each as soon as its result is in (the checks overlap the serving)
+    ref = os.path.join(ROOT, "oracle", "_ref", "twins-rdma-cpu")
+
+    def check(i, got):
+        if is_rdma[i]:
+            if not os.path.exists(ref):
+                return i, None, got
+            r = reqs[i][1]
+            out = os.path.join(a.sweep_dir, f"s{i:04d}.fct")
+            subprocess.run([ref, cfg, f"--topo={r['topo']}", f"--flows={r['flows']}", f"--out={out}", "--gwin=0"] + r["overrides"],
+                           capture_output=True, timeout=1800)
+            rows = open(out).read().splitlines()
+            os.remove(out)
+            return i, rows, got.splitlines()
+        out = os.path.join(a.sweep_dir, f"s{i:04d}.csv")
+        _, ev, rows = oracle_world(reqs[i][1], out)
+        return i, rows, got
+    pool = cf.ThreadPoolExecutor(max(1, a.cpu_cores - 2))
+    checks = []
     sid = {}
-    arrival, finish, results = {}, {}, {}
+    arrival, finish = {}, {}
     events = 0
     submitted = 0
     t_start = time.perf_counter()
@@ -236,8 +260,8 @@ def serve_leg(a):
                     finish[i] = time.perf_counter() - t_start
                     events += ev[38]
                     inexact += any(ev[j] for j in rdma.INEXACT)
-                    if i % a.serve_check == 0 and worlds[i].hosts <= 128:  # (the reference's CPU build: up to k=8)
-                        results[i] = "".join(f"{j} {start[j]} {fct[j]}\n" for j in range(len(fct)) if fct[j] >= 0)
+                    if i % a.serve_check == 0:
+                        checks.append(pool.submit(check, i, "".join(f"{j} {start[j]} {fct[j]}\n" for j in range(len(fct)) if fct[j] >= 0)))
             else:
                 s_ = srv.state(sid[i])
                 if s_ in (twins_amd.REQ_DONE, twins_amd.REQ_FAILED):
@@ -245,7 +269,7 @@ def serve_leg(a):
                     finish[i] = time.perf_counter() - t_start
                     events += ev[26]
                     if i % a.serve_check == 0:
-                        results[i] = list(zip(est, done))
+                        checks.append(pool.submit(check, i, list(zip(est, done))))
         if time.perf_counter() - last_log > 10:
             last_log = time.perf_counter()
             # (the TCP requests admitted and not yet taken, by their worlds' device footprint)
@@ -275,30 +299,17 @@ def serve_leg(a):
         xs = sorted(finish[r] - arrival[r] for r in finish if is_rdma[r] == kind)
         if xs:
             kind_lat["rdma" if kind else "tcp"] = {"requests": len(xs), "p50_s": round(pct(xs, 0.5), 4), "p99_s": round(pct(xs, 0.99), 4)}
-    # every serve_check-th request against the oracle (TCP) or the reference's RDMA build, on all host cores
-    ref = os.path.join(ROOT, "oracle", "_ref", "twins-rdma-cpu")
-
-    def check(i):
-        if is_rdma[i]:
-            if not os.path.exists(ref):
-                return i, None
-            r = reqs[i][1]
-            out = os.path.join(a.sweep_dir, f"s{i:04d}.fct")
-            subprocess.run([ref, cfg, f"--topo={r['topo']}", f"--flows={r['flows']}", f"--out={out}", "--gwin=0"] + r["overrides"],
-                           capture_output=True, timeout=1800)
-            return i, open(out).read().splitlines()
-        out = os.path.join(a.sweep_dir, f"s{i:04d}.csv")
-        _, ev, rows = oracle_world(reqs[i][1], out)
-        return i, rows
-    same = total = 0
-    with cf.ThreadPoolExecutor(a.cpu_cores) as ex:
-        for i, rows in ex.map(check, sorted(results)):
-            if rows is None:
-                continue
-            got = results[i].splitlines() if is_rdma[i] else results[i]
-            same += sum(x == y for x, y in zip(rows, got)) if len(rows) == len(got) else 0
-            total += len(rows)
-    log(f"serve: parity {same}/{total} flows of {len(results)} requests checked")
+    t_chk = time.time()
+    same = total = checked = 0
+    for f in checks:
+        i, rows, got = f.result()
+        if rows is None:
+            continue
+        checked += 1
+        same += sum(x == y for x, y in zip(rows, got)) if len(rows) == len(got) else 0
+        total += len(rows)
+    pool.shutdown()
+    log(f"serve: parity {same}/{total} flows of {checked} requests checked ({time.time() - t_chk:.1f} s after the last result)")
     return {
         "workload": f"config 5: {n} requests arriving Poisson at {a.serve_rate:g}/s (seed 5); TCP requests: a FatTree k in "
                     f"{{4,8,16,32}} world with U{{1000..64000}} Pareto(1.2) flows and a TCP variant drawn per request, a quarter with "
@@ -314,9 +325,54 @@ def serve_leg(a):
         "requests": n,
         "wall_s": round(wall, 3),
         "rdma_inexact_requests": inexact,
-        "parity": {"flows_identical_to_oracle": same, "flows": total, "requests_checked": len(results),
+        "parity": {"flows_identical_to_oracle": same, "flows": total, "requests_checked": checked,
                    "oracle": "TCP: oracle/tcp_oracle; RDMA: the reference's RDMA build (oracle/_ref)"},
     }
+
+
+# the reference's own speed table (data/twins_speed.csv: its event loop's wall time on an RX 9060 XT,
+# min of 3 runs, one world per process) for the workloads whose ns-3 outputs are stored here
+REF_SPEED = [("k8_paper", "dctcp", 8, 0.2238), ("k8_paper", "newreno", 8, 0.3648), ("k8_a2a30", "dctcp", 8, 0.3262),
+             ("k8_a2a30", "newreno", 8, 0.396), ("k8_a2a50s", "dctcp", 8, 0.357), ("k16_paper", "dctcp", 16, 0.434),
+             ("k16_a2a30", "dctcp", 16, 0.6371), ("k16_a2a50s", "dctcp", 16, 0.5073), ("k16_a2a50s", "newreno", 16, 1.0502)]
+TCP_FILE = {"dctcp": "TcpDctcp", "newreno": "TcpNewReno", "cubic": "TcpCubic"}
+
+
+def refspeed_leg(a):
+    """The reference's published single-world workloads: each world alone (the drop-in's plain
+    launch, automatic shape), its event loop timed (tw_batch_run, min of a.ref_runs), every flow
+    checked against the stored ns-3 output; returns the "reference_workloads" object."""
+    import twins_amd
+    g = os.path.join(ROOT, "tests", "golden")
+    out = {}
+    for name, tcp, k, ref_s in REF_SPEED:
+        w = twins_amd.World([f"--k={k}", f"--flows={g}/flows/{name}.csv", f"--tcp={tcp}"])
+        best, b, runs = None, None, 9
+        for _ in range(a.ref_runs):
+            if b is not None:
+                b.close()
+            b = twins_amd.Batch([w], chain_runs=runs)
+            st = b.run()
+            if b.counters(0)[25] and runs == 9:
+                runs, best = 25, None  # undecided ties: 25-run chains from here on (the reference's twins128)
+                b.close()
+                b = twins_amd.Batch([w], chain_runs=runs)
+                st = b.run()
+            best = st.wall_s if best is None else min(best, st.wall_s)
+        est, done = b.fct(0)
+        exp = []
+        with open(f"{g}/ns3/{name}_{TCP_FILE[tcp]}.csv") as f:
+            next(f)
+            exp = [tuple(int(x) for x in line.split(",")[4:6]) for line in f]
+        same = sum(e == x for e, x in zip(exp, zip(est, done)))
+        slots = b.counters(0)[31]
+        b.close()
+        out[f"{name}/{tcp}"] = {"event_loop_s": round(best, 4), "reference_rx9060xt_s": ref_s, "speedup_vs_reference": round(ref_s / best, 3),
+                                "slots": slots, "us_per_slot": round(1e6 * best / max(1, slots), 2), "chain_runs": runs,
+                                "flows_identical_to_ns3": same, "flows": len(exp)}
+        log(f"refspeed: {name} {tcp}: {best:.4f} s ({slots} slots, {1e6 * best / max(1, slots):.1f} us/slot), reference "
+            f"{ref_s} s on an RX 9060 XT; {same}/{len(exp)} flows identical to ns-3")
+    return out
 
 
 RDMA_BYTES_PER_EVENT = 480
@@ -412,10 +468,13 @@ def main():
     ap.add_argument("--serve", type=int, default=1, help="also run the config-5 serving loop (the \"serve\" object)")
     ap.add_argument("--serve_requests", type=int, default=1000)
     ap.add_argument("--serve_rate", type=float, default=100.0)
-    ap.add_argument("--serve_check", type=int, default=20, help="every n-th request is checked against the oracle")
+    ap.add_argument("--serve_check", type=int, default=1, help="every n-th request is checked against the oracle")
     ap.add_argument("--serve_rdma", type=float, default=0.1, help="the share of the serving loop's requests that are RDMA worlds")
     ap.add_argument("--serve_rdma_blocks", type=int, default=64, help="workgroups the RDMA serving loop's launches take")
     ap.add_argument("--rdma", type=int, default=1, help="also run config 3, the RDMA model (the \"rdma\" object)")
+    ap.add_argument("--refspeed", type=int, default=1, help="also time the reference's published single-world workloads")
+    ap.add_argument("--ref_runs", type=int, default=3, help="runs per reference workload (the best is reported)")
+    ap.add_argument("--serve_first", type=int, default=0, help="run the serving leg before the sweep")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU (the worlds of a sweep are independent and would be "
@@ -499,18 +558,24 @@ def main():
             "sample": "the oracle (sequential CPU restatement of the model) on the same 3 worlds, event loop only",
         },
     }
+    if a.serve and a.serve_first:
+        line["serve"] = serve_leg(a)
     if a.sweep:
         line["sweep"] = sweep_leg(a)
-    if a.serve:
+    if a.serve and not a.serve_first:
         line["serve"] = serve_leg(a)
     if a.rdma:
         line["rdma"] = rdma_leg(a)
+    if a.refspeed:
+        line["reference_workloads"] = refspeed_leg(a)
     print(json.dumps(line), flush=True)
     bad = [k for k in ("sweep", "serve") if k in line and line[k]["parity"]["flows_identical_to_oracle"] != line[k]["parity"]["flows"]]
     if "rdma" in line:
         r = line["rdma"]
         if r["parity"]["flows_identical_to_reference"] != r["parity"]["flows"] or r["config"]["inexact_counters"]:
             bad.append("rdma")
+    if "reference_workloads" in line:
+        bad += [k for k, v in line["reference_workloads"].items() if v["flows_identical_to_ns3"] != v["flows"]]
     if same != total or bad:
         sys.exit(1)
 

@@ -156,17 +156,27 @@ def test_gpu_rdma_serving_loop():
 
 
 def test_gpu_lost_tails_rerun():
-    """Rings of 2^10 tail nodes per world: ties of this PFC incast meet overwritten tail nodes
-    (counter 9); run_exact() runs the worlds again from the start on 2^26-node rings, as the
-    reference's twins-rdma reruns itself, and ends with the reference's completions and counters."""
+    """Rings of 2^4 tail nodes per world (one reservation chunk, so every chunk an entity takes
+    overwrites the nodes before it): ties of these incasts meet overwritten tail nodes (counter 9;
+    the CPU build of the same code loses 13, 316 and 105 of them). run_exact() runs the worlds
+    again from the start on 2^26-node rings, as the reference's twins-rdma reruns itself, and
+    every world ends with the reference's completions and counters."""
     from twins_amd import rdma
     I = rdma_util.INPUTS
-    name = "inc1_b1_cc1"
-    job = next(j for j in rdma_util.jobs() if j[0] == name)
-    print(f"rdma lost tails: {name} on 2^10-node rings", flush=True)
-    b = rdma.Batch([rdma.World(rdma_util.CONFIG, f"{I}/{job[1]}", f"{I}/{job[2]}", job[3])], tail_log2=10, tail_exact=True)
+    names = ["inc1_b1_cc1", "jf32_incast_b1_cc1", "k8_perm_lowk_cc8"]
+    by = {j[0]: j for j in rdma_util.jobs()}
+    print(f"rdma lost tails: {names} on 2^4-node rings", flush=True)
+    ws = [rdma.World(rdma_util.CONFIG, f"{I}/{by[n][1]}", f"{I}/{by[n][2]}", by[n][3]) for n in names]
+    b = rdma.Batch(ws, tail_log2=4, tail_exact=True)
+    first = b.run()
+    lost = b.counters()[rdma.EV_LOST]
+    assert lost > 0, "no tie met an overwritten node on 16-node rings"
+    b.close()
+    b = rdma.Batch(ws, tail_log2=4, tail_exact=True)
     st, reruns = b.run_exact()
     assert reruns >= 1 and b.tail_log2() >= 26
-    assert b.fct_text(0) == rdma_util.ref_fct(name)
-    assert b.summary_line() == job[5]
+    assert b.counters()[rdma.EV_LOST] == 0
+    for i, n in enumerate(names):
+        assert b.fct_text(i) == rdma_util.ref_fct(n), n
+    assert rdma_util.sum_counters([b.summary_line()]) == rdma_util.sum_counters([by[n][5] for n in names])
     b.close()

@@ -828,8 +828,8 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
         // a ring of tail nodes per world: 2^(log2(entities) + 14) nodes, from 2^16 to 2^tail_log2;
         // with tail_exact, 2^tail_log2 nodes each
         const int lgmax = cfg.tail_log2 > 0 ? std::min(cfg.tail_log2, 30) : 24;
-        if (cfg.tail_exact && (cfg.tail_log2 < 10 || cfg.tail_log2 > 30))
-            return fail(TW_EINVAL, "tail_log2 %d: from 10 to 30", cfg.tail_log2);
+        if (cfg.tail_exact && (cfg.tail_log2 < 4 || cfg.tail_log2 > 30))
+            return fail(TW_EINVAL, "tail_log2 %d: from 4 to 30", cfg.tail_log2);
         vector<rd::TailRing> rings(W);
         uint64_t total = 0;
         b.tail_lg = 0;
@@ -1252,7 +1252,8 @@ extern "C" int trd_server_create(const trd_server_cfg *cfg, trd_server **out)
         return fail(TW_EDEVICE, "hipStreamCreate failed");
     for (auto &l : s->lane)
     {
-        if (hipStreamCreateWithFlags(&l.ks, hipStreamNonBlocking) != hipSuccess || hipEventCreateWithFlags(&l.done_ev, hipEventDisableTiming) != hipSuccess)
+        // (each class's launches on a hardware queue of their own, which the uploads never share)
+        if (serving_stream_create(&l.ks) != hipSuccess || hipEventCreateWithFlags(&l.done_ev, hipEventDisableTiming) != hipSuccess)
             return fail(TW_EDEVICE, "hipStreamCreate failed");
         if (hipMalloc((void **)&l.queue_d, sizeof(int) * RSRV_CAP) != hipSuccess || hipMalloc((void **)&l.qnext_d, sizeof(unsigned)) != hipSuccess ||
             hipMalloc((void **)&l.teams_d, sizeof(rd::TeamState) * l.blocks) != hipSuccess ||

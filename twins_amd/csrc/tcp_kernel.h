@@ -5,6 +5,9 @@
 
 namespace tw
 {
+// the longest a serving-loop team holds a claim on a request not yet published (seconds); an idle
+// team gives its claim back sooner (the server's idle_s, which must be less)
+constexpr double SERVER_WAIT_S = 600.0;
 // a launch of the persistent world-queue kernel: `blocks` workgroups of `threads` (their teams,
 // sizes and queues are the batch's TeamPlan); `teams` when a team has several workgroups (the
 // grid must then fit the GPU at once); each world runs at most slot_budget slots per launch

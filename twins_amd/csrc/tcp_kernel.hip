@@ -24,7 +24,7 @@ namespace tw
 
 constexpr int LDS_PER_WAVE = 64 * 33 + 64 * 3 + 64 + 64; // switch phase: rows, in_win, base, cnt
 constexpr unsigned long long WAIT_TICKS = 2000000000ull;  // a team-barrier wait gives up after 20 s (100 MHz clock)
-constexpr unsigned long long SERVER_MAX_WAIT = 60000000000ull; // a server team waits 10 min at most for a request
+constexpr unsigned long long SERVER_MAX_WAIT = (unsigned long long)(SERVER_WAIT_S * 100e6); // (100 MHz clock)
 
 __device__ __forceinline__ unsigned ld_acquire(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
 

@@ -1619,6 +1619,7 @@ struct tw_server
     std::vector<int> waiting; // ids, in arrival order
     size_t wait_head = 0;
     std::vector<void *> dev, host;
+    std::vector<std::unique_ptr<tw_batch>> graveyard; // failed requests' batches, freed once the kernel has left
     bool launched = false;
     uint64_t relaunches = 0;
     ~tw_server()
@@ -1627,6 +1628,7 @@ struct tw_server
             __atomic_store_n(shutdown, 1, __ATOMIC_RELEASE);
         if (ks)
             (void)hipStreamSynchronize(ks);
+        graveyard.clear();
         reqs.clear();
         for (void *p : dev)
             (void)hipFree(p);
@@ -1686,6 +1688,9 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
     if (s->threads % 64 || s->threads > 256)
         return fail(TW_EINVAL, "block_threads: a multiple of 64 up to 256");
     s->cap = c.max_requests > 0 ? c.max_requests : 65536;
+    // an idle team gives its claim back after idle_s; a claim is held at most SERVER_WAIT_S in all
+    if (c.idle_s >= SERVER_WAIT_S)
+        return fail(TW_EINVAL, "idle_s %.1f: less than %.0f s", c.idle_s, SERVER_WAIT_S);
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     HIPCHK(tcp_max_blocks(s->runs, s->threads, &per_cu));
@@ -1720,7 +1725,9 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
     s->arena.base = (char *)ab;
     s->arena.size = want;
     s->arena.free_[0] = want;
-    HIPCHK(hipStreamCreateWithFlags(&s->ks, hipStreamNonBlocking));
+    // the persistent kernel's stream on a hardware queue of its own: the uploads the host waits
+    // for never queue behind it
+    HIPCHK(serving_stream_create(&s->ks));
     HIPCHK(hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking));
     int rc;
     if ((rc = srv_pinned(*s, &s->req, (size_t)s->nq * s->cap)) || (rc = srv_pinned(*s, &s->tail, s->nq)) ||
@@ -1816,11 +1823,28 @@ static int srv_service(tw_server &s)
             }
         s.relaunches++;
     }
-    // each queue restarts at its first request not yet run
-    std::vector<unsigned> first(s.published);
-    for (auto &r : s.reqs)
-        if (r.state == TW_REQ_RESIDENT)
-            first[r.queue] = std::min(first[r.queue], r.index);
+    // no kernel runs now: failed requests' memory may go back to the arena
+    s.graveyard.clear();
+    // each queue restarts after its last published index, with the requests not yet run published
+    // again there (a request the old kernel finished is never claimed twice, and its memory may
+    // already be back in the arena)
+    std::vector<unsigned> first(s.nq, 0); // (a first launch: every published request is still to run)
+    if (s.launched)
+    {
+        first = s.published;
+        for (auto &r : s.reqs)
+            if (r.state == TW_REQ_RESIDENT)
+            {
+                const unsigned i = s.published[r.queue];
+                if ((int)i >= s.cap)
+                    return fail(TW_ESTATE, "queue %d is full (%d requests)", r.queue, s.cap);
+                s.req[(size_t)r.queue * s.cap + i] = s.req[(size_t)r.queue * s.cap + r.index];
+                r.index = i;
+                s.published[r.queue] = i + 1;
+            }
+    }
+    for (int q = 0; q < s.nq; q++)
+        __atomic_store_n(&s.tail[q], s.published[q], __ATOMIC_RELEASE);
     std::vector<TeamState> zero(s.nteams);
     memset(zero.data(), 0, sizeof(TeamState) * s.nteams);
     HIPCHK(hipMemcpyAsync(s.next_d, first.data(), 4 * s.nq, hipMemcpyHostToDevice, s.up));
@@ -1898,6 +1922,8 @@ extern "C" int tw_server_result(tw_server *s, int32_t id, int64_t *est_ns, int64
     if (ev)
         ev[31] = b.run_host[0].slot;
     const bool failed = r.state == TW_REQ_FAILED;
+    if (failed && s->launched && hipStreamQuery(s->ks) == hipErrorNotReady)
+        s->graveyard.push_back(std::move(r.b)); // (the team's other workgroups may still write into it)
     r.b.reset(); // its memory goes back to the arena
     r.state = TW_REQ_TAKEN;
     return failed ? fail(TW_EDEVICE, "request %d failed on the device (a team wait timed out)", id) : TW_OK;

@@ -91,9 +91,9 @@ int main(int argc, char **argv)
             pc.tail_log2 = atoi(v.c_str());
             pc.tail_exact = 1;
             tail_set = k == "--tail";
-            if (pc.tail_log2 < 10 || pc.tail_log2 > 30)
+            if (pc.tail_log2 < 4 || pc.tail_log2 > 30)
             {
-                fprintf(stderr, "%s: log2 of the tail ring from 10 to 30\n", k.c_str());
+                fprintf(stderr, "%s: log2 of the tail ring from 4 to 30\n", k.c_str());
                 return 1;
             }
         }
