@@ -59,7 +59,9 @@ static void run_cpu(BatchT<RUNS> &B, int64_t budget)
             for (int hl = 0; hl < W.H; hl++)
             {
                 int64_t x = INT64_MAX;
-                act += Md::host_step(B, W, R, hl, t, &x);
+                SndState ls;
+                RcvState lr;
+                act += Md::host_step(B, W, R, hl, t, &x, &ls, &lr);
                 wk = std::min(wk, (unsigned long long)x);
             }
             for (int sw = 0; sw < W.S; sw++)

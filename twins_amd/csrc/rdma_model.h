@@ -974,8 +974,13 @@ RD_HD double rand_u01(double *x)
     return (p1 > p2) ? (p1 - p2) * norm : (p1 - p2 + m1) * norm;
 }
 
-// ------------------------------------------------------------------ host: the NIC (src/rdma/rstep.cuh:986-1647)
-struct HCtx
+// ------------------------------------------------------------------ host: the NIC
+// A host's slot works on a NicSlot: the host record in registers, its out link's open bucket and
+// the chain of the event that runs. The NIC's behaviour is the HPCC simulator's QbbNetDevice +
+// RdmaHw (src/rdma/rstep.cuh:986-1647 restate it); here it is organised by concern: the egress
+// pump (what the NIC sends next and when), the receive side (ACK / NACK replies), the queue pairs'
+// lifecycle, and one rate controller per CC mode, each a small set of functions over the QP.
+struct NicSlot
 {
     const Sim *s;
     const Params *P;
@@ -992,29 +997,34 @@ struct HCtx
     OutBucket ob;
 };
 
-RD_HD uint64_t bytes_left(const SndQP &q) { return q.size >= q.snd_nxt ? q.size - q.snd_nxt : 0; }
-RD_HD int64_t qp_tx(SndQP &q, uint64_t rate, uint32_t bytes)
+// the chain of an event the running one schedules `lead` ns ahead (each takes the next ordinal)
+RD_HD void nic_schedule(NicSlot &n, Chain *dst, int64_t lead) { cpush(*n.s, n.st.ta, dst, ccode(lead, n.ord++), n.cur); }
+
+RD_HD uint64_t qp_unsent(const SndQP &q) { return q.size >= q.snd_nxt ? q.size - q.snd_nxt : 0; }
+RD_HD bool qp_finished(const SndQP &q) { return q.snd_una >= q.size; }
+// a frame's serialization time at `rate`, the last (rate, size) pair's kept with the QP
+RD_HD int64_t qp_serial(SndQP &q, uint64_t rate, uint32_t bytes)
 {
-    if (rate != q.tx_rate || bytes != q.tx_size)
+    if (q.tx_rate != rate || q.tx_size != bytes)
     {
+        q.tx_val = tx_ns(rate, bytes);
         q.tx_rate = rate;
         q.tx_size = bytes;
-        q.tx_val = tx_ns(rate, bytes);
     }
     return q.tx_val;
 }
-RD_HD bool win_bound(const Params &P, const SndQP &q)
+// the window: bytes in flight may not reach it (VAR_WIN scales it with the rate)
+RD_HD bool qp_window_full(const Params &P, const SndQP &q)
 {
-    if (q.win == 0)
+    if (!q.win)
         return false;
-    uint64_t w = q.win;
+    uint64_t lim = q.win;
     if (P.var_win)
     {
-        w = (uint64_t)q.win * q.rate / q.max_rate; // (exact: q.win * rate stays below 2^64)
-        if (w == 0)
-            w = 1;
+        lim = (uint64_t)q.win * q.rate / q.max_rate; // (q.win * rate stays below 2^64)
+        lim = lim ? lim : 1;
     }
-    return q.snd_nxt - q.snd_una >= w;
+    return q.snd_nxt - q.snd_una >= lim;
 }
 
 // a transmission-trace record (src/rdma/rstep.cuh:915-942): out of line, off the path when tracing is off
@@ -1034,455 +1044,465 @@ RD_COLD void trace_tx(const Sim &s, int64_t now, int node, int dev, const Pkt &p
     r.size = p.size;
     r.ecn = p.ecn;
     r.pad = 0;
-    if (p.kind >= K_PAUSE)
-    {
-        r.a = p.q;
-        r.b = r.sport = r.seq = 0;
-    }
-    else
-    {
-        const bool data = p.kind == K_DATA;
-        r.a = data ? p.src : p.dst;
-        r.b = data ? p.dst : p.src;
-        r.sport = p.sport;
-        r.seq = p.seq;
-    }
+    const bool ctrl = p.kind >= K_PAUSE, fwd = p.kind == K_DATA;
+    r.a = ctrl ? p.q : fwd ? p.src : p.dst;
+    r.b = ctrl ? 0 : fwd ? p.dst : p.src;
+    r.sport = ctrl ? 0 : p.sport;
+    r.seq = ctrl ? 0 : p.seq;
     t.rec[i] = r;
 }
 
-// QbbNetDevice::TransmitStart (src/rdma/rstep.cuh:1056-1095)
-// (the packet by reference: one in a ring is copied straight into the frame, not through a stack copy)
-RD_HD void h_transmit(HCtx &c, const Pkt &p)
+// a frame onto the out link (TransmitStart): filed in its arrival bucket, the NIC busy until its
+// last bit is out (the frame is written before the chain work, so a packet in a ring is copied
+// straight into it)
+RD_HD void nic_emit(NicSlot &n, const Pkt &p)
 {
-    const Sim &s = *c.s;
-    int64_t tx = link_txo(s, c.ob, p.size);
-    c.st.tc_at = c.now + tx;
-    // the frame first (the packet is not kept across the chain work), then the chains
-    const int ord = c.ord;
-    c.ord += 2;
-    c.nout++;
-    int64_t ts = c.now + tx + c.ldelay;
-    LFrame *f = ob_push(s, c.pend, c.pend_n, ts, c.ob);
-    if (f)
+    const Sim &s = *n.s;
+    const int64_t busy = link_txo(s, n.ob, p.size), lands = n.now + busy + n.ldelay;
+    n.st.tc_at = n.now + busy;
+    const int done_ord = n.ord; // the TransmitComplete's ordinal; the arrival's is the next
+    n.ord += 2;
+    n.nout++;
+    LFrame *fr = ob_push(s, n.pend, n.pend_n, lands, n.ob);
+    if (fr)
     {
-        f->ts = ts;
-        f->p = p;
-        f->p.ord = (uint8_t)(ord + 1 < 255 ? ord + 1 : 255);
+        fr->ts = lands;
+        fr->p = p;
+        fr->p.ord = (uint8_t)(done_ord < 254 ? done_ord + 1 : 255);
     }
-    cpush(s, c.st.ta, &s.h_tc[c.h], ccode(tx, ord), c.cur);
-    if (!f)
-        return;
-    f->c = c.cur;
-    if (s.txt_rec.rec)
-        trace_tx(s, c.now, c.h, 1, f->p);
-}
-RD_HD void h_schedule_ns(HCtx &c, int64_t at)
-{
-    c.st.ns_at = at;
-    cpush(*c.s, c.st.ta, &c.s->h_ns[c.h], ccode(at - c.now, c.ord++), c.cur);
-}
-// RdmaEgressQueue::GetNextQindex: -1 the ACK queue, -1024 none, else the QP's place in the group;
-// finished queue pairs leave the group here (src/rdma/rstep.cuh:1106-1144)
-RD_HD int next_qindex(HCtx &c)
-{
-    const Sim &s = *c.s;
-    int ackq = c.P->ack_high_prio ? 0 : PG;
-    if (!(c.st.paused >> ackq & 1) && c.st.ackq_n > 0)
-        return -1;
-    int *list = s.qps + (size_t)c.h * s.qps_cap;
-    int res = -1024, fcount = c.st.qp_n, min_finish = INT_MAX;
-    int idx = fcount ? c.st.rr % fcount : 0;
-    for (int qi = 1; qi <= fcount; qi++)
+    cpush(s, n.st.ta, &s.h_tc[n.h], ccode(busy, done_ord), n.cur);
+    if (fr)
     {
-        idx = idx + 1 == fcount ? 0 : idx + 1;
-        const SndQP &q = s.qp[list[idx]];
-        if (!(c.st.paused >> PG & 1) && bytes_left(q) > 0 && !win_bound(*c.P, q))
+        fr->c = n.cur;
+        if (s.txt_rec.rec)
+            trace_tx(s, n.now, n.h, 1, fr->p);
+    }
+}
+
+// The egress pump (DequeueAndTransmit + RdmaEgressQueue): when the link is free, the ACK queue
+// first (unless its class is paused), else the queue pairs round-robin from the last one served.
+// A QP is eligible with bytes left, its window open, the data class not paused and its pacing
+// time reached. Finished QPs seen on the way leave the list (ns-3's lazy removal, done in the same
+// pass that picks).
+enum
+{
+    PICK_ACK = -1,
+    PICK_NONE = -1024
+};
+// removes the finished QPs from list[from ..) (from: the first finished one); returns the new
+// place of list index `keep` (PICK_NONE stays)
+RD_HD int qp_list_prune(NicSlot &n, int *list, int from, int keep)
+{
+    const Sim &s = *n.s;
+    int w = from;
+    for (int i = from + 1; i < n.st.qp_n; i++)
+    {
+        if (qp_finished(s.qp[list[i]]))
+            continue;
+        if (i == keep)
+            keep = w;
+        list[w++] = list[i];
+    }
+    n.st.qp_n = w;
+    return keep;
+}
+RD_HD int nic_pick(NicSlot &n)
+{
+    const Sim &s = *n.s;
+    const int ack_class = n.P->ack_high_prio ? 0 : PG;
+    if (n.st.ackq_n > 0 && !(n.st.paused >> ack_class & 1))
+        return PICK_ACK;
+    int *list = s.qps + (size_t)n.h * s.qps_cap;
+    const int cnt = n.st.qp_n;
+    const bool data_paused = n.st.paused >> PG & 1;
+    int pick = PICK_NONE, first_done = -1;
+    for (int k = 1, i = cnt ? n.st.rr % cnt : 0; k <= cnt; k++)
+    {
+        i = i + 1 < cnt ? i + 1 : 0;
+        const SndQP &q = s.qp[list[i]];
+        const bool ready = !data_paused && qp_unsent(q) > 0 && !qp_window_full(*n.P, q);
+        if (!ready)
         {
-            if (q.next_avail > c.now)
-                continue;
-            res = idx;
+            if (qp_finished(q) && (first_done < 0 || i < first_done))
+                first_done = i;
+            continue;
+        }
+        if (q.next_avail <= n.now)
+        {
+            pick = i;
             break;
         }
-        else if (q.snd_una >= q.size)
-            min_finish = idx < min_finish ? idx : min_finish;
     }
-    if (min_finish < INT_MAX)
-    {
-        int nxt = min_finish;
-        for (int i = min_finish + 1; i < fcount; i++)
-            if (!(s.qp[list[i]].snd_una >= s.qp[list[i]].size))
-            {
-                if (i == res)
-                    res = nxt;
-                list[nxt++] = list[i];
-            }
-        c.st.qp_n = nxt;
-    }
-    return res;
+    return first_done >= 0 ? qp_list_prune(n, list, first_done, pick) : pick;
 }
-// QbbNetDevice::DequeueAndTransmit on a host (src/rdma/rstep.cuh:1146-1196)
-RD_HD void h_dequeue(HCtx &c)
+// the next data segment of QP f (GetNxtPacket), then its pacing (PktSent, UpdateNextAvail)
+RD_HD void nic_send_segment(NicSlot &n, int f)
 {
-    const Sim &s = *c.s;
-    if (c.st.tc_at >= 0)
-        return;
-    int idx = next_qindex(c);
-    if (idx == -1)
-    {
-        const Pkt &p = s.ackq[(size_t)c.h * ACKQ_CAP + c.st.ackq_head]; // (not reused before the copy into the frame)
-        c.st.ackq_head = ring_add(c.st.ackq_head, 1, ACKQ_CAP);
-        c.st.ackq_n--;
-        h_transmit(c, p);
-        return;
-    }
-    int *list = s.qps + (size_t)c.h * s.qps_cap;
-    if (idx >= 0)
-    {
-        int f = list[idx];
-        c.st.rr = idx;
-        SndQP &q = s.qp[f];
-        // RdmaHw::GetNxtPacket
-        uint64_t left = bytes_left(q);
-        uint32_t payload = left < c.P->mtu ? (uint32_t)left : c.P->mtu;
-        Pkt p = {};
-        p.kind = K_DATA;
-        p.flow = (uint32_t)f;
-        p.src = (uint16_t)s.fi[f].src;
-        p.dst = (uint16_t)s.fi[f].dst;
-        p.sport = (uint16_t)s.fi[f].sport;
-        p.seq = (uint32_t)q.snd_nxt;
-        p.size = (uint16_t)(payload + data_hdr(*c.P));
-        q.snd_nxt += payload;
-        h_transmit(c, p);
-        // RdmaHw::PktSent, UpdateNextAvail
-        q.last_pkt = p.size;
-        q.next_avail = c.now + qp_tx(q, c.P->rate_bound ? q.rate : q.max_rate, p.size);
-        return;
-    }
-    int64_t t = INT64_MAX;
-    for (int i = 0; i < c.st.qp_n; i++)
+    const Sim &s = *n.s;
+    const Params &P = *n.P;
+    SndQP &q = s.qp[f];
+    const FlowInfo &fi = s.fi[f];
+    const uint64_t left = qp_unsent(q);
+    const uint32_t body = left < P.mtu ? (uint32_t)left : P.mtu;
+    Pkt p = {};
+    p.kind = K_DATA;
+    p.flow = (uint32_t)f;
+    p.src = (uint16_t)fi.src;
+    p.dst = (uint16_t)fi.dst;
+    p.sport = (uint16_t)fi.sport;
+    p.seq = (uint32_t)q.snd_nxt;
+    p.size = (uint16_t)(body + data_hdr(P));
+    q.snd_nxt += body;
+    nic_emit(n, p);
+    q.last_pkt = p.size;
+    q.next_avail = n.now + qp_serial(q, P.rate_bound ? q.rate : q.max_rate, p.size);
+}
+// nothing to send now: m_nextSend at the earliest pacing time of a QP with bytes left
+RD_HD void nic_arm_next_send(NicSlot &n)
+{
+    const Sim &s = *n.s;
+    const int *list = s.qps + (size_t)n.h * s.qps_cap;
+    int64_t at = INT64_MAX;
+    for (int i = 0; i < n.st.qp_n; i++)
     {
         const SndQP &q = s.qp[list[i]];
-        if (bytes_left(q) == 0)
-            continue;
-        t = q.next_avail < t ? q.next_avail : t;
+        if (qp_unsent(q) > 0 && q.next_avail < at)
+            at = q.next_avail;
     }
-    if (c.st.ns_at < 0 && t < INT64_MAX && t > c.now)
-        h_schedule_ns(c, t);
-}
-// RdmaHw::ChangeRate (src/rdma/rstep.cuh:1198-1207)
-RD_HD void change_rate(HCtx &c, SndQP &q, uint64_t new_rate)
-{
-    int64_t st = qp_tx(q, q.rate, q.last_pkt), nt = qp_tx(q, new_rate, q.last_pkt);
-    q.next_avail = q.next_avail + nt - st;
-    if (c.st.ns_at >= 0 && q.next_avail < c.st.ns_at)
-        h_schedule_ns(c, q.next_avail < c.now ? c.now : q.next_avail);
-    q.rate = new_rate;
-}
-// RdmaHw::UpdateRateHp with one rate for the path (src/rdma/rstep.cuh:1235-1312)
-RD_HD void hpcc_update(HCtx &c, SndQP &q, const Pkt &p, bool fast)
-{
-    const Params &P = *c.P;
-    uint32_t next_seq = (uint32_t)q.snd_nxt;
-    // (loops of MAX_HOP trips, so that the frame's hops are indexed statically)
-    if (q.last_update_seq == 0)
+    if (n.st.ns_at < 0 && at != INT64_MAX && at > n.now)
     {
-        q.last_update_seq = next_seq;
+        n.st.ns_at = at;
+        nic_schedule(n, &s.h_ns[n.h], at - n.now);
+    }
+}
+RD_HD void nic_pump(NicSlot &n)
+{
+    if (n.st.tc_at >= 0)
+        return; // a transmission is in progress
+    const int pick = nic_pick(n);
+    const Sim &s = *n.s;
+    if (pick == PICK_ACK)
+    {
+        const Pkt &a = s.ackq[(size_t)n.h * ACKQ_CAP + n.st.ackq_head]; // (the slot outlives the copy into the frame)
+        n.st.ackq_head = ring_add(n.st.ackq_head, 1, ACKQ_CAP);
+        n.st.ackq_n--;
+        nic_emit(n, a);
+    }
+    else if (pick != PICK_NONE)
+    {
+        n.st.rr = pick;
+        nic_send_segment(n, s.qps[(size_t)n.h * s.qps_cap + pick]);
+    }
+    else
+        nic_arm_next_send(n);
+}
+
+// a QP's sending rate changes (ChangeRate): its pacing time moves by the difference of the last
+// frame's serialization at the two rates, and an m_nextSend set later than it comes forward
+RD_HD void qp_set_rate(NicSlot &n, SndQP &q, uint64_t rate)
+{
+    const int64_t was = qp_serial(q, q.rate, q.last_pkt), becomes = qp_serial(q, rate, q.last_pkt);
+    q.next_avail += becomes - was;
+    if (n.st.ns_at >= 0 && q.next_avail < n.st.ns_at)
+    {
+        const int64_t at = q.next_avail > n.now ? q.next_avail : n.now;
+        n.st.ns_at = at;
+        nic_schedule(n, &n.s->h_ns[n.h], at - n.now);
+    }
+    q.rate = rate;
+}
+
+// ---- HPCC (UpdateRateHp, one rate for the path): the path's utilization from the INT hops of
+// the ACK against those of the last update, then a multiplicative or additive step
+// the largest normalized utilization over the hops the ACK reports, and its hop's time delta; the
+// QP's stored hops become the ACK's. false: no hop took part (all skipped by sample feedback).
+RD_HD bool hpcc_path_load(const Params &P, SndQP &q, const Pkt &p, bool fast, double *U, uint64_t *tau_max)
+{
+    bool any = false;
+    *U = 0;
+    *tau_max = 0;
+    // (MAX_HOP trips with static indices: the frame's hops stay in registers)
+    for (int i = 0; i < MAX_HOP; i++)
+    {
+        if (i >= p.nhop)
+            break;
+        const uint64_t now_h = p.hop[i], was_h = q.hop[i];
+        if (fast && P.sample_feedback && hop_qlen(now_h) == 0)
+            continue;
+        any = true;
+        const uint64_t tau = hop_time_delta(now_h, was_h);
+        const double bits = (double)(hop_bytes_delta(now_h, was_h, P.int_multi) * 8);
+        const double link = (double)hop_rate(now_h);
+        const uint32_t qn = hop_qlen(now_h) * 80 * P.int_multi, qw = hop_qlen(was_h) * 80 * P.int_multi;
+        const double util = bits / ((double)tau * 1e-9) / link + (double)(qn < qw ? qn : qw) * (double)q.max_rate / link / (double)q.win;
+        if (util > *U)
+        {
+            *U = util;
+            *tau_max = tau;
+        }
+        q.hop[i] = now_h;
+    }
+    return any;
+}
+RD_HD void hpcc_react(NicSlot &n, SndQP &q, double U, uint64_t tau, bool fast)
+{
+    const Params &P = *n.P;
+    if (tau > q.base_rtt)
+        tau = q.base_rtt;
+    // the EWMA of U over a base RTT (the fork's build contracts u (B - tau) + U tau into one fma)
+    q.u = fma(q.u, (double)(q.base_rtt - tau), U * (double)tau) / (double)q.base_rtt;
+    const double over = q.u / P.u_target;
+    const bool mult = over >= 1 || (uint32_t)q.inc_stage >= P.mi_thresh;
+    uint64_t rate = mult ? rate_of((double)q.cur_rate / over) + P.rai : q.cur_rate + P.rai;
+    rate = rate < P.min_rate ? P.min_rate : rate > q.max_rate ? q.max_rate : rate;
+    qp_set_rate(n, q, rate);
+    if (!fast)
+    {
+        q.cur_rate = rate;
+        q.inc_stage = mult ? 0 : q.inc_stage + 1;
+    }
+}
+RD_HD void hpcc_on_ack(NicSlot &n, SndQP &q, const Pkt &p, bool fast)
+{
+    const uint32_t sent_to = (uint32_t)q.snd_nxt;
+    if (!q.last_update_seq)
+    {
+        // the first ACK only records the path
+        q.last_update_seq = sent_to;
         for (int i = 0; i < MAX_HOP; i++)
             if (i < p.nhop)
                 q.hop[i] = p.hop[i];
         return;
     }
-    if (p.nhop <= MAX_HOP)
-    {
-        double U = 0;
-        uint64_t dt = 0;
-        bool any = false;
-        for (int i = 0; i < MAX_HOP; i++)
-        {
-            if (i >= p.nhop)
-                break;
-            if (P.sample_feedback && hop_qlen(p.hop[i]) == 0 && fast)
-                continue;
-            any = true;
-            uint64_t tau = hop_time_delta(p.hop[i], q.hop[i]);
-            double duration = (double)tau * 1e-9;
-            double tx_rate = (double)(hop_bytes_delta(p.hop[i], q.hop[i], P.int_multi) * 8) / duration;
-            uint32_t qa = hop_qlen(p.hop[i]) * 80 * P.int_multi, qb = hop_qlen(q.hop[i]) * 80 * P.int_multi;
-            double lr = (double)hop_rate(p.hop[i]);
-            double u = tx_rate / lr + (double)(qa < qb ? qa : qb) * (double)q.max_rate / lr / (double)q.win;
-            if (u > U)
-            {
-                U = u;
-                dt = tau;
-            }
-            q.hop[i] = p.hop[i];
-        }
-        if (any)
-        {
-            uint64_t new_rate;
-            int32_t new_inc;
-            if (dt > q.base_rtt)
-                dt = q.base_rtt;
-            // the fork's build contracts u (B - dt) + U dt into one fma
-            q.u = fma(q.u, (double)(q.base_rtt - dt), U * (double)dt) / (double)q.base_rtt;
-            double max_c = q.u / P.u_target;
-            if (max_c >= 1 || (uint32_t)q.inc_stage >= P.mi_thresh)
-            {
-                new_rate = rate_of((double)q.cur_rate / max_c) + P.rai;
-                new_inc = 0;
-            }
-            else
-            {
-                new_rate = q.cur_rate + P.rai;
-                new_inc = q.inc_stage + 1;
-            }
-            if (new_rate < P.min_rate)
-                new_rate = P.min_rate;
-            if (new_rate > q.max_rate)
-                new_rate = q.max_rate;
-            change_rate(c, q, new_rate);
-            if (!fast)
-            {
-                q.cur_rate = new_rate;
-                q.inc_stage = new_inc;
-            }
-        }
-    }
-    if (!fast && next_seq > q.last_update_seq)
-        q.last_update_seq = next_seq;
+    double U;
+    uint64_t tau;
+    if (p.nhop <= MAX_HOP && hpcc_path_load(*n.P, q, p, fast, &U, &tau))
+        hpcc_react(n, q, U, tau, fast);
+    if (!fast && sent_to > q.last_update_seq)
+        q.last_update_seq = sent_to;
 }
-RD_HD void due_note(HCtx &c, int64_t t)
+
+// ---- DCQCN (Mellanox): a CNP arms the alpha and rate-decrease timers on the QP's first one; the
+// three timers re-arm themselves (their deadlines feed the host's earliest-timer bound)
+RD_HD void dcqcn_arm(NicSlot &n, int64_t *deadline, Chain *chain, int64_t lead)
 {
-    if (t < c.st.due_min)
-        c.st.due_min = t;
+    *deadline = n.now + lead;
+    if (*deadline < n.st.due_min)
+        n.st.due_min = *deadline;
+    nic_schedule(n, chain, lead);
 }
-// RdmaHw::cnp_received_mlx (src/rdma/rstep.cuh:1324-1345)
-RD_HD void dcqcn_cnp(HCtx &c, int f, SndQP &q)
+RD_HD uint64_t dcqcn_cut(const Params &P, const SndQP &q)
 {
-    const Sim &s = *c.s;
-    q.alpha_cnp = 1;
-    q.dec_cnp = 1;
-    if (q.first_cnp)
-    {
-        q.alpha = 1;
-        q.alpha_cnp = 0;
-        q.t_alpha = c.now + c.P->alpha_int;
-        due_note(c, q.t_alpha);
-        cpush(s, c.st.ta, &s.qc[f].alpha, ccode(c.P->alpha_int, c.ord++), c.cur);
-        q.t_dec = c.now + c.P->dec_int + 1;
-        due_note(c, q.t_dec);
-        cpush(s, c.st.ta, &s.qc[f].dec, ccode(c.P->dec_int + 1, c.ord++), c.cur);
-        q.target = q.rate = rate_of((double)q.rate * c.P->rate_first_cnp);
-        q.first_cnp = 0;
-    }
+    const uint64_t r = rate_of((double)q.rate * (1 - q.alpha / 2));
+    return r > P.min_rate ? r : P.min_rate;
 }
-// RdmaHw::HandleAckDctcp (src/rdma/rstep.cuh:1347-1385)
-RD_HD void dctcp_ack(HCtx &c, SndQP &q, const Pkt &p)
+RD_HD void dcqcn_on_cnp(NicSlot &n, int f, SndQP &q)
 {
-    const Params &P = *c.P;
-    bool new_batch = false;
-    q.ecn_cnt += p.cnp ? 1 : 0;
-    if (p.seq > q.last_update_seq)
+    const Params &P = *n.P;
+    q.alpha_cnp = q.dec_cnp = 1;
+    if (!q.first_cnp)
+        return;
+    q.first_cnp = 0;
+    q.alpha = 1;
+    q.alpha_cnp = 0;
+    dcqcn_arm(n, &q.t_alpha, &n.s->qc[f].alpha, P.alpha_int);
+    dcqcn_arm(n, &q.t_dec, &n.s->qc[f].dec, P.dec_int + 1);
+    q.rate = rate_of((double)q.rate * P.rate_first_cnp);
+    q.target = q.rate;
+}
+RD_HD void dcqcn_alpha_tick(NicSlot &n, int f, SndQP &q)
+{
+    const Params &P = *n.P;
+    q.alpha = q.alpha_cnp ? fma(1 - P.g, q.alpha, P.g) : (1 - P.g) * q.alpha;
+    q.alpha_cnp = 0;
+    dcqcn_arm(n, &q.t_alpha, &n.s->qc[f].alpha, P.alpha_int);
+}
+RD_HD void dcqcn_decrease_tick(NicSlot &n, int f, SndQP &q)
+{
+    const Params &P = *n.P;
+    dcqcn_arm(n, &q.t_dec, &n.s->qc[f].dec, P.dec_int);
+    if (!q.dec_cnp)
+        return;
+    if (q.rp_stage || P.clamp)
+        q.target = q.rate;
+    q.rate = dcqcn_cut(P, q);
+    q.rp_stage = 0;
+    q.dec_cnp = 0;
+    dcqcn_arm(n, &q.t_rp, &n.s->qc[f].rp, P.rp_int);
+}
+RD_HD void dcqcn_increase_tick(NicSlot &n, int f, SndQP &q)
+{
+    const Params &P = *n.P;
+    dcqcn_arm(n, &q.t_rp, &n.s->qc[f].rp, P.rp_int);
+    if (q.rp_stage >= P.rpg_threshold)
     {
-        new_batch = true;
-        if (q.last_update_seq == 0)
-        {
-            q.last_update_seq = (uint32_t)q.snd_nxt;
-            q.batch = (uint32_t)(q.snd_nxt / P.mtu + 1);
-        }
-        else
-        {
-            const double frac = (double)q.ecn_cnt / (double)q.batch;
-            q.alpha = fma(1 - P.g, q.alpha, P.g * (frac < 1.0 ? frac : 1.0));
-            q.last_update_seq = (uint32_t)q.snd_nxt;
-            q.ecn_cnt = 0;
-            q.batch = (uint32_t)((q.snd_nxt - (uint64_t)p.seq) / P.mtu + 1);
-        }
+        const uint64_t t = q.target + (q.rp_stage == P.rpg_threshold ? P.rai : P.rhai);
+        q.target = t > q.max_rate ? q.max_rate : t;
     }
+    q.rate = rate_of((double)q.rate / 2) + rate_of((double)q.target / 2);
+    q.rp_stage++;
+}
+
+// ---- DCTCP over RDMA (HandleAckDctcp): alpha per window of segments, one cut per window with a
+// mark, an additive step per window without
+// closes the window of ACKs when one passes its end: alpha from its marked share; true if so
+RD_HD bool dctcp_window(const Params &P, SndQP &q, const Pkt &p)
+{
+    if (p.seq <= q.last_update_seq)
+        return false;
+    if (q.last_update_seq)
+    {
+        const double marked = (double)q.ecn_cnt / (double)q.batch;
+        q.alpha = fma(1 - P.g, q.alpha, P.g * (marked < 1.0 ? marked : 1.0));
+        q.ecn_cnt = 0;
+        q.batch = (uint32_t)((q.snd_nxt - (uint64_t)p.seq) / P.mtu + 1);
+    }
+    else
+        q.batch = (uint32_t)(q.snd_nxt / P.mtu + 1);
+    q.last_update_seq = (uint32_t)q.snd_nxt;
+    return true;
+}
+RD_HD void dctcp_on_ack(const Params &P, SndQP &q, const Pkt &p)
+{
+    if (p.cnp)
+        q.ecn_cnt++;
+    const bool window_end = dctcp_window(P, q, p);
     if (q.ca_state == 1 && p.seq > q.high_seq)
-        q.ca_state = 0;
-    if (p.cnp && q.ca_state == 0)
+        q.ca_state = 0; // the reduced window is acknowledged
+    if (q.ca_state == 0 && p.cnp)
     {
-        const uint64_t r = rate_of((double)q.rate * (1 - q.alpha / 2));
-        q.rate = r > P.min_rate ? r : P.min_rate;
+        q.rate = dcqcn_cut(P, q); // (the same rate cut: rate (1 - alpha/2), at least the minimum)
         q.ca_state = 1;
         q.high_seq = (uint32_t)q.snd_nxt;
     }
-    if (q.ca_state == 0 && new_batch)
+    if (q.ca_state == 0 && window_end)
     {
         const uint64_t r = q.rate + P.dctcp_rai;
-        q.rate = r < q.max_rate ? r : q.max_rate;
+        q.rate = r > q.max_rate ? q.max_rate : r;
     }
 }
-// the DCQCN timers: 0 UpdateAlphaMlx, 1 CheckRateDecreaseMlx, 2 RateIncEventTimerMlx
-// (src/rdma/rstep.cuh:1387-1437)
-RD_HD void dcqcn_timer(HCtx &c, int f, int kind)
+
+// ---- the queue pairs' lifecycle
+// an ACK or NACK (ReceiveAck): the QP's acknowledged point moves; the last byte completes it (its
+// flow completion time is recorded, its timers end); a NACK rewinds sending to it (go-back-N);
+// then the CC mode's reaction and the pump
+RD_HD void nic_on_ack(NicSlot &n, const Pkt &p, unsigned *flows_done)
 {
-    const Sim &s = *c.s;
-    const Params &P = *c.P;
-    SndQP &q = s.qp[f];
-    if (kind == 0)
-    {
-        if (q.alpha_cnp)
-            q.alpha = fma(1 - P.g, q.alpha, P.g);
-        else
-            q.alpha = (1 - P.g) * q.alpha;
-        q.alpha_cnp = 0;
-        q.t_alpha = c.now + P.alpha_int;
-        due_note(c, q.t_alpha);
-        cpush(s, c.st.ta, &s.qc[f].alpha, ccode(P.alpha_int, c.ord++), c.cur);
-    }
-    else if (kind == 1)
-    {
-        q.t_dec = c.now + P.dec_int;
-        due_note(c, q.t_dec);
-        cpush(s, c.st.ta, &s.qc[f].dec, ccode(P.dec_int, c.ord++), c.cur);
-        if (q.dec_cnp)
-        {
-            if (P.clamp || q.rp_stage != 0)
-                q.target = q.rate;
-            uint64_t r = rate_of((double)q.rate * (1 - q.alpha / 2));
-            q.rate = r > P.min_rate ? r : P.min_rate;
-            q.rp_stage = 0;
-            q.dec_cnp = 0;
-            q.t_rp = c.now + P.rp_int;
-            due_note(c, q.t_rp);
-            cpush(s, c.st.ta, &s.qc[f].rp, ccode(P.rp_int, c.ord++), c.cur);
-        }
-    }
-    else
-    {
-        q.t_rp = c.now + P.rp_int;
-        due_note(c, q.t_rp);
-        cpush(s, c.st.ta, &s.qc[f].rp, ccode(P.rp_int, c.ord++), c.cur);
-        if (q.rp_stage >= P.rpg_threshold)
-        {
-            q.target += q.rp_stage == P.rpg_threshold ? P.rai : P.rhai;
-            if (q.target > q.max_rate)
-                q.target = q.max_rate;
-        }
-        q.rate = rate_of((double)q.rate / 2) + rate_of((double)q.target / 2);
-        q.rp_stage++;
-    }
-}
-// RdmaHw::ReceiveAck, ACK and NACK (src/rdma/rstep.cuh:1440-1488)
-RD_HD void h_ack(HCtx &c, const Pkt &p, unsigned *flows_done)
-{
-    const Sim &s = *c.s;
-    int f = (int)p.flow;
+    const Sim &s = *n.s;
+    const Params &P = *n.P;
+    const int f = (int)p.flow;
     SndQP &q = s.qp[f];
     if (q.done)
     {
         count(s, EV_ACK_DONE);
         return;
     }
-    if (p.seq > q.snd_una)
-        q.snd_una = p.seq;
-    if (q.snd_una >= q.size)
+    q.snd_una = p.seq > q.snd_una ? p.seq : q.snd_una;
+    const bool completes = qp_finished(q);
+    if (completes)
     {
-        // QpComplete
-        q.t_alpha = q.t_dec = q.t_rp = -1;
         q.done = 1;
-        s.fct[f] = c.now - s.fi[f].start;
+        q.t_alpha = q.t_dec = q.t_rp = -1;
+        s.fct[f] = n.now - s.fi[f].start;
         at_add32(flows_done, 1u);
     }
     if (p.kind == K_NACK)
         q.snd_nxt = q.snd_una;
-    if (p.cnp && c.P->cc == 1 && !q.done)
-        dcqcn_cnp(c, f, q);
-    if (c.P->cc == 3)
+    switch (P.cc)
     {
-        if (p.seq > q.last_update_seq)
-            hpcc_update(c, q, p, false);
-        else if (c.P->fast_react)
-            hpcc_update(c, q, p, true);
+    case 1:
+        if (p.cnp && !completes)
+            dcqcn_on_cnp(n, f, q);
+        break;
+    case 3:
+        if (p.seq > q.last_update_seq || P.fast_react)
+            hpcc_on_ack(n, q, p, p.seq <= q.last_update_seq);
+        break;
+    case 8:
+        if (!completes)
+            dctcp_on_ack(P, q, p);
+        break;
     }
-    if (c.P->cc == 8 && !q.done)
-        dctcp_ack(c, q, p);
-    h_dequeue(c);
+    nic_pump(n);
 }
-// RdmaHw::ReceiveUdp and ReceiverCheckSeq (src/rdma/rstep.cuh:1490-1546)
-RD_HD void h_data(HCtx &c, const Pkt &p)
+// a data frame at the receiver (ReceiveUdp, ReceiverCheckSeq): in order, it advances the
+// expected byte and is acknowledged at the ACK interval or a chunk's end; ahead of it, a NACK
+// unless one for the same point is still within its interval; old data is ignored
+enum
 {
-    const Sim &s = *c.s;
+    RX_ACK,
+    RX_NACK,
+    RX_QUIET
+};
+RD_HD int rx_check(const Params &P, RcvQP &r, const Pkt &p, int64_t now)
+{
+    const uint32_t body = p.size - data_hdr(P);
+    if (p.seq == r.expected)
+    {
+        r.expected += body;
+        return r.expected >= P.ack_interval || (P.chunk && r.expected % P.chunk == 0) ? RX_ACK : RX_QUIET;
+    }
+    if (p.seq < r.expected || (now < r.nack_timer && r.last_nack == r.expected))
+        return RX_QUIET;
+    r.nack_timer = now + P.nack_interval;
+    r.last_nack = r.expected;
+    return RX_NACK;
+}
+RD_HD void nic_on_data(NicSlot &n, const Pkt &p)
+{
+    const Sim &s = *n.s;
     RcvQP &r = s.rx[p.flow];
-    uint32_t payload = p.size - data_hdr(*c.P);
     if (r.expected >= s.fi[p.flow].size)
         count(s, EV_DATA_DONE);
-    int x;
-    uint32_t expected = r.expected;
-    if (p.seq == expected)
-    {
-        r.expected = expected + payload;
-        if (r.expected >= c.P->ack_interval)
-            x = 1;
-        else if (c.P->chunk && r.expected % c.P->chunk == 0)
-            x = 1;
-        else
-            x = 5;
-    }
-    else if (p.seq > expected)
-    {
-        if (c.now >= r.nack_timer || r.last_nack != expected)
-        {
-            r.nack_timer = c.now + c.P->nack_interval;
-            r.last_nack = expected;
-            x = 2;
-        }
-        else
-            x = 4;
-    }
-    else
-        x = 3;
-    if (x != 1 && x != 2)
+    const int v = rx_check(*n.P, r, p, n.now);
+    if (v == RX_QUIET)
         return;
-    if (x == 2)
+    if (v == RX_NACK)
         count(s, EV_NACK);
-    Pkt a = p;
-    a.kind = x == 1 ? K_ACK : K_NACK;
-    a.seq = r.expected;
-    a.cnp = p.ecn ? 1 : 0;
-    a.ecn = 0;
-    a.size = (uint16_t)ack_size(*c.P);
-    if (c.st.ackq_n >= ACKQ_CAP)
+    if (n.st.ackq_n >= ACKQ_CAP)
     {
         count(s, EV_ACKQ_OVF);
         return;
     }
-    s.ackq[(size_t)c.h * ACKQ_CAP + ring_add(c.st.ackq_head, c.st.ackq_n, ACKQ_CAP)] = a;
-    c.st.ackq_n++;
-    h_dequeue(c);
+    // the reply carries the data frame's addresses and hops, the expected byte and the CE mark as CNP
+    Pkt &a = s.ackq[(size_t)n.h * ACKQ_CAP + ring_add(n.st.ackq_head, n.st.ackq_n, ACKQ_CAP)];
+    a = p;
+    a.kind = v == RX_ACK ? K_ACK : K_NACK;
+    a.seq = r.expected;
+    a.cnp = p.ecn ? 1 : 0;
+    a.ecn = 0;
+    a.size = (uint16_t)ack_size(*n.P);
+    n.st.ackq_n++;
+    nic_pump(n);
 }
-// RdmaClient::StartApplication -> RdmaHw::AddQueuePair (src/rdma/rstep.cuh:1548-1587)
-RD_HD void h_start(HCtx &c, int f)
+// a flow starts (StartApplication, AddQueuePair): a fresh QP at line rate joins the host's list
+RD_HD void nic_add_qp(NicSlot &n, int f)
 {
-    const Sim &s = *c.s;
+    const Sim &s = *n.s;
     const FlowInfo &fi = s.fi[f];
-    SndQP &q = s.qp[f];
-    q = SndQP{};
+    const uint64_t line = s.li[n.link].bps;
+    SndQP q = {};
     q.size = fi.size;
-    const uint64_t nic = s.li[c.link].bps;
-    q.rate = q.max_rate = nic;
     q.win = fi.win;
     q.base_rtt = fi.base_rtt;
+    q.rate = q.max_rate = q.target = line;
     q.alpha = 1;
     q.first_cnp = 1;
-    q.target = nic;
     q.t_alpha = q.t_dec = q.t_rp = -1;
-    if (c.P->cc == 3)
+    q.next_avail = n.now;
+    if (n.P->cc == 3)
     {
-        q.cur_rate = nic;
+        q.cur_rate = line;
         q.u = 1;
     }
-    q.next_avail = c.now;
-    if (c.st.qp_n >= s.qps_cap)
+    s.qp[f] = q;
+    if (n.st.qp_n >= s.qps_cap)
     {
         count(s, EV_QPS_OVF);
         return;
     }
-    s.qps[(size_t)c.h * s.qps_cap + c.st.qp_n++] = f;
-    h_dequeue(c);
+    s.qps[(size_t)n.h * s.qps_cap + n.st.qp_n++] = f;
+    nic_pump(n);
 }
 
 RD_HD const Chain &due_chain(const Sim &s, const Due &d)
@@ -1525,7 +1545,7 @@ RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, ui
         *wake = s.hs[h].wake;
         return false;
     }
-    HCtx c;
+    NicSlot c;
     c.s = &s;
     c.P = &P;
     c.pend = pend;
@@ -1641,36 +1661,43 @@ RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, ui
             const Pkt p = af->p;
             ai++;
             if (p.kind == K_DATA)
-                h_data(c, p);
+                nic_on_data(c, p);
             else if (p.kind == K_ACK || p.kind == K_NACK)
-                h_ack(c, p, flows_done);
+                nic_on_ack(c, p, flows_done);
             else if (p.kind == K_PAUSE)
                 c.st.paused |= 1u << p.q;
             else
             {
                 c.st.paused &= ~(1u << p.q);
-                h_dequeue(c);
+                nic_pump(c);
             }
         }
         else if (bk == 1)
         {
             c.st.tc_at = -1;
-            h_dequeue(c);
+            nic_pump(c);
         }
         else if (bk == 2)
         {
             c.st.ns_at = -1;
-            h_dequeue(c);
+            nic_pump(c);
         }
         else if (bk == 3)
         {
+            // a DCQCN timer: 0 alpha update, 1 rate-decrease check, 2 rate increase
             const Due x = due[di++];
-            dcqcn_timer(c, x.flow, x.kind);
+            SndQP &q = s.qp[x.flow];
+            if (x.kind == 0)
+                dcqcn_alpha_tick(c, x.flow, q);
+            else if (x.kind == 1)
+                dcqcn_decrease_tick(c, x.flow, q);
+            else
+                dcqcn_increase_tick(c, x.flow, q);
         }
         else
         {
             c.st.next_start++;
-            h_start(c, sf);
+            nic_add_qp(c, sf);
         }
     }
     bool act = c.nout > 0;

@@ -22,7 +22,15 @@
 namespace tw
 {
 
-constexpr int LDS_PER_WAVE = 64 * 33 + 64 * 3 + 64 + 64; // switch phase: rows, in_win, base, cnt
+// per wave, for the item it runs: a switch item's rows, in_win, base, cnt; a host item's slots for
+// the connection records of an event, one SndState and one RcvState per lane. A wave runs its
+// items one after another, so the two share one region.
+constexpr int LDS_SWITCH = 64 * 33 + 64 * 3 + 64 + 64;
+constexpr int LDS_SND = 0, LDS_RCV = LDS_SND + 64 * (int)(sizeof(SndState) / 4);
+constexpr int LDS_RECORDS = LDS_RCV + 64 * (int)(sizeof(RcvState) / 4);
+constexpr int LDS_PER_WAVE = LDS_RECORDS > LDS_SWITCH ? LDS_RECORDS : LDS_SWITCH;
+static_assert(sizeof(SndState) % 128 == 0 && sizeof(RcvState) % 128 == 0, "LDS record alignment");
+static_assert(4 * (size_t)LDS_PER_WAVE * 4 + 1024 <= 160 * 1024, "a 256-thread workgroup's LDS");
 constexpr unsigned long long WAIT_TICKS = 2000000000ull;  // a team-barrier wait gives up after 20 s (100 MHz clock)
 constexpr unsigned long long SERVER_MAX_WAIT = (unsigned long long)(SERVER_WAIT_S * 100e6); // (100 MHz clock)
 
@@ -57,7 +65,7 @@ template <int RUNS, bool SRV>
 __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, const ServerCtl *__restrict__ Sp, int64_t slot_budget)
 {
     typedef Model<RUNS> Md;
-    extern __shared__ int lds[];
+    extern __shared__ __attribute__((aligned(128))) int lds[];
     __shared__ int s_w;
     __shared__ int64_t s_t;
     __shared__ int s_done, s_err;
@@ -237,7 +245,7 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
                     if (lane < hpw && hl < W.H)
                     {
                         int64_t x = INT64_MAX;
-                        act += Md::host_step(B, W, R, hl, t, &x);
+                        act += Md::host_step(B, W, R, hl, t, &x, (SndState *)(wl + LDS_SND) + lane, (RcvState *)(wl + LDS_RCV) + lane);
                         wk = (unsigned long long)x < wk ? (unsigned long long)x : wk;
                     }
 #ifdef TW_PROF

@@ -273,7 +273,35 @@ struct Model
         Due *due;
         int nd, id;
         int64_t slot_end;
+        // the records of the connection side an event works on, copied into this lane's LDS
+        // slots for the event (snd_open / rcv_open) and written back once at its end
+        SndState *ls;
+        RcvState *lr;
     };
+    // an event on connection f's sending (receiving) side: its record from device memory into
+    // the lane's LDS slot, and back when the event ends (a record is 16 (8) independent 16-byte
+    // loads, issued together, instead of a dependent load per field use)
+    // (dst and src: one record type, in LDS and in device memory, either way round)
+    template <typename D, typename S>
+    static __device__ __forceinline__ void rec_move(D *dst, const S *src)
+    {
+        static_assert(sizeof(D) == sizeof(S) && sizeof(D) % 16 == 0, "records move in 16-byte pieces");
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(D) / 16); i++)
+            reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(src)[i];
+    }
+    static __device__ __forceinline__ SndState &snd_open(Host &x, int f)
+    {
+        rec_move(x.ls, &x.B->snd[f]);
+        return *x.ls;
+    }
+    static __device__ __forceinline__ void snd_close(Host &x, int f) { rec_move(&x.B->snd[f], x.ls); }
+    static __device__ __forceinline__ RcvState &rcv_open(Host &x, int f)
+    {
+        rec_move(x.lr, &x.B->rcv[f]);
+        return *x.lr;
+    }
+    static __device__ __forceinline__ void rcv_close(Host &x, int f) { rec_move(&x.B->rcv[f], x.lr); }
 
     // Active lists: a host's sending and receiving connections. A connection leaves its list at
     // the end of a slot in which it is finished (sender: closed; receiver: the sender closed and
@@ -364,7 +392,7 @@ struct Model
     }
     static __device__ void send_ack(Host &x, int f, uint32_t ack, bool force_ece, bool no_ece, int64_t now)
     {
-        RcvState &r = x.B->rcv[f];
+        RcvState &r = *x.lr;
         bool ece = false;
         if (force_ece)
             ece = true;
@@ -386,7 +414,7 @@ struct Model
         uint32_t *st;
         uint8_t *fl;
         uint32_t cap;
-        __device__ Tx(const Batch &B, int f) : s(B.snd[f])
+        __device__ Tx(const Batch &B, SndState &rec, int f) : s(rec)
         {
             const ConnStatic &c = B.cs[f];
             st = B.it_start + c.it_off;
@@ -698,9 +726,11 @@ struct Model
         uint32_t v = s.h_head + j;
         return v >= cap ? v - cap : v;
     }
-    static __device__ void rtt_history(const Batch &B, WorldRun *R, int f, uint32_t seq, uint32_t sz, bool retx, int64_t now)
+    static __device__ void rtt_history(Host &x, int f, uint32_t seq, uint32_t sz, bool retx, int64_t now)
     {
-        SndState &s = B.snd[f];
+        const Batch &B = *x.B;
+        WorldRun *R = x.R;
+        SndState &s = *x.ls;
         const ConnStatic &c = B.cs[f];
         uint32_t *hs = B.h_seq + c.it_off, *hc = B.h_cnt + c.it_off;
         if (!retx)
@@ -749,9 +779,10 @@ struct Model
         int64_t r = est + (v > CLOCK_GRAN_PS ? v : CLOCK_GRAN_PS);
         s.rto = r > MIN_RTO_PS ? r : MIN_RTO_PS;
     }
-    static __device__ void rtt_ack(const Batch &B, int f, uint32_t ack, int64_t now)
+    static __device__ void rtt_ack(Host &x, int f, uint32_t ack, int64_t now)
     {
-        SndState &s = B.snd[f];
+        const Batch &B = *x.B;
+        SndState &s = *x.ls;
         if (s.h_n == 0)
             return;
         const ConnStatic &c = B.cs[f];
@@ -998,8 +1029,8 @@ struct Model
     static __device__ uint32_t send_segment(Host &x, int f, uint32_t seq, uint32_t max_size, int64_t now)
     {
         const Batch &B = *x.B;
-        SndState &s = B.snd[f];
-        Tx t(B, f);
+        SndState &s = *x.ls;
+        Tx t(B, *x.ls, f);
         bool retx = false;
         uint32_t len = tb_copy(t, x.R, max_size, seq, &retx);
         if (len == 0)
@@ -1028,7 +1059,7 @@ struct Model
             timer_chain(x, B.rchain, f, s.rto);
         }
         s.cwnd_limited = s.cwnd < inflight(s) + MSS;
-        rtt_history(B, x.R, f, seq, len, retx, now);
+        rtt_history(x, f, seq, len, retx, now);
         if (seq + len > s.high)
             s.high = seq + len;
         atomicMax(&x.R->ev[EV_MAX_INFLIGHT], (unsigned long long)s.it_n);
@@ -1037,10 +1068,10 @@ struct Model
     static __device__ void send_pending(Host &x, int f, int64_t now)
     {
         const Batch &B = *x.B;
-        SndState &s = B.snd[f];
+        SndState &s = *x.ls;
         if (s.st != S_EST)
             return;
-        Tx t(B, f);
+        Tx t(B, *x.ls, f);
         for (int guard = 0; guard < 4096; guard++)
         {
             if (s.cong == C_OPEN && (s.fin & (FIN_SENT | FIN_ACKED)) == FIN_SENT)
@@ -1064,8 +1095,8 @@ struct Model
     }
     static __device__ void do_retransmit(Host &x, int f, int64_t now)
     {
-        SndState &s = x.B->snd[f];
-        Tx t(*x.B, f);
+        SndState &s = *x.ls;
+        Tx t(*x.B, *x.ls, f);
         uint32_t seq;
         if (!tb_next_seg(t, &seq))
             seq = s.una;
@@ -1077,7 +1108,7 @@ struct Model
     // NewAck (src/step.cuh:1457-1479)
     static __device__ void new_ack(Host &x, int f, uint32_t ack, bool reset_rto, int64_t now)
     {
-        SndState &s = x.B->snd[f];
+        SndState &s = *x.ls;
         if (reset_rto)
         {
             int64_t est = s.srtt >= 0 ? s.srtt : INIT_RTT_PS, v = 4 * s.rttvar;
@@ -1093,8 +1124,8 @@ struct Model
     }
     static __device__ __forceinline__ void enter_recovery(Host &x, int f, int64_t now)
     {
-        SndState &s = x.B->snd[f];
-        Tx t(*x.B, f);
+        SndState &s = *x.ls;
+        Tx t(*x.B, *x.ls, f);
         tb_add_reno_sack(t);
         tb_mark_head_lost(t);
         s.recover = s.high;
@@ -1107,7 +1138,7 @@ struct Model
     static __device__ void messages_acked(Host &x, int f, int64_t now)
     {
         const Batch &B = *x.B;
-        SndState &s = B.snd[f];
+        SndState &s = *x.ls;
         const ConnStatic &c = B.cs[f];
         while (s.m_dn < s.m_wr && s.una >= B.mg_end[c.m_first + s.m_dn])
         {
@@ -1124,14 +1155,14 @@ struct Model
     // ReceivedAck + ProcessAck (src/step.cuh:1515-1692)
     static __device__ void on_ack(Host &x, int f, uint32_t ack, bool ece, int64_t now)
     {
-        SndState &s = x.B->snd[f];
+        SndState &s = *x.ls;
         if (s.st != S_EST)
             return;
         uint32_t old = s.una;
         if (ack < old)
             return;
-        Tx t(*x.B, f);
-        rtt_ack(*x.B, f, ack, now);
+        Tx t(*x.B, *x.ls, f);
+        rtt_ack(x, f, ack, now);
         bool advancing = ack > old;
         uint32_t delivered = 0;
         if (advancing)
@@ -1276,7 +1307,7 @@ struct Model
     }
     static __device__ __forceinline__ void on_rto(Host &x, int f, int64_t now)
     {
-        SndState &s = x.B->snd[f];
+        SndState &s = *x.ls;
         s.rto_at = -1;
         if (s.st != S_EST)
             return;
@@ -1289,7 +1320,7 @@ struct Model
         count(x.R, EV_RTO);
         uint32_t before = inflight(s);
         s.dupacks = 0;
-        Tx t(*x.B, f);
+        Tx t(*x.B, *x.ls, f);
         tb_all_lost(t);
         s.recover = s.high;
         s.recover_active = 1;
@@ -1308,7 +1339,7 @@ struct Model
     // (src/step.cuh:1727-2106)
     static __device__ void send_synack(Host &x, int f, int64_t now)
     {
-        RcvState &r = x.B->rcv[f];
+        RcvState &r = *x.lr;
         int64_t rto = CONN_TIMEOUT_PS << r.srv_syn_n;
         r.srv_syn_n++;
         if (r.r_synack_t < 0)
@@ -1330,7 +1361,7 @@ struct Model
     }
     static __device__ void ce_state(Host &x, int f, bool ce_now, int64_t now)
     {
-        RcvState &r = x.B->rcv[f];
+        RcvState &r = *x.lr;
         if (ce_now)
         {
             if (!r.ce_state && r.reserved && r.prior_flag)
@@ -1353,7 +1384,7 @@ struct Model
     {
         if (!x.dctcp)
             return;
-        RcvState &r = x.B->rcv[f];
+        RcvState &r = *x.lr;
         bool ce = p.flags & F_CE;
         if (ce && r.ecn_ce_seq < (int64_t)p.sa)
         {
@@ -1381,9 +1412,11 @@ struct Model
         }
     }
     // TcpRxBuffer::Add for [seq, end): the out-of-order blocks stay disjoint and sorted
-    static __device__ bool rx_add(const Batch &B, WorldRun *R, int f, uint32_t seq, uint32_t end)
+    static __device__ bool rx_add(Host &x, int f, uint32_t seq, uint32_t end)
     {
-        RcvState &r = B.rcv[f];
+        const Batch &B = *x.B;
+        WorldRun *R = x.R;
+        RcvState &r = *x.lr;
         const ConnStatic &c = B.cs[f];
         uint32_t head = seq > r.rcv_nxt ? seq : r.rcv_nxt;
         if (head >= end)
@@ -1462,7 +1495,7 @@ struct Model
     static __device__ __forceinline__ bool rx_finished(const RcvState &r) { return r.r_fin >= 0 && (uint32_t)r.r_fin < r.rcv_nxt; }
     static __device__ void send_fin_ack(Host &x, int f, int64_t now)
     {
-        RcvState &r = x.B->rcv[f];
+        RcvState &r = *x.lr;
         r.delack_at = -1;
         r.delack_count = 0;
         emit(x, f, F_ACK | F_FIN, 0, r.rcv_nxt, CTRL_FRAME, true, now);
@@ -1482,7 +1515,7 @@ struct Model
     }
     static __device__ void do_peer_close(Host &x, int f, int64_t now)
     {
-        RcvState &r = x.B->rcv[f];
+        RcvState &r = *x.lr;
         r.r_st = R_LAST_ACK;
         send_fin_ack(x, f, now);
         r.r_last_n = DATA_RETRIES;
@@ -1499,9 +1532,9 @@ struct Model
     }
     static __device__ void received_data(Host &x, int f, uint32_t seq, uint32_t len, bool fin, int64_t now)
     {
-        RcvState &r = x.B->rcv[f];
+        RcvState &r = *x.lr;
         uint32_t expected = r.rcv_nxt;
-        if (!rx_add(*x.B, x.R, f, seq, seq + len))
+        if (!rx_add(x, f, seq, seq + len))
         {
             send_ack(x, f, r.rcv_nxt, false, false, now);
             return;
@@ -1537,7 +1570,7 @@ struct Model
     }
     static __device__ void on_data(Host &x, int f, const Frame &p, int64_t now)
     {
-        RcvState &r = x.B->rcv[f];
+        RcvState &r = *x.lr;
         if (!r.r_up || r.r_st == R_CLOSED)
             return;
         uint32_t seq = p.sa, len = (p.flags & F_DATA) ? p.len - 42u : 0u, end = seq + len;
@@ -1597,7 +1630,7 @@ struct Model
     static __device__ void app_write(Host &x, int f, int64_t now)
     {
         const Batch &B = *x.B;
-        SndState &s = B.snd[f];
+        SndState &s = *x.ls;
         const ConnStatic &c = B.cs[f];
         if (s.m_wr >= s.m_due)
             return;
@@ -1620,14 +1653,11 @@ struct Model
             due_insert(x, now + 1, f, 2);
         }
     }
-    // a frame at its destination host (src/step.cuh:2147-2236)
-    static __device__ void on_frame(Host &x, const Frame &p, int64_t now)
+    // a frame at its destination host (src/step.cuh:2147-2236): the sending side's part
+    static __device__ void on_frame_snd(Host &x, int f, const Frame &p, int64_t now)
     {
-        const Batch &B = *x.B;
-        int f = (int)p.flow;
-        if (p.to_sender)
         {
-            SndState &s = B.snd[f];
+            SndState &s = *x.ls;
             if (p.flags & F_SYN)
             {
                 if (s.st == S_SYN_SENT)
@@ -1658,7 +1688,11 @@ struct Model
             }
             return;
         }
-        RcvState &r = B.rcv[f];
+    }
+    // ... and the receiving side's
+    static __device__ void on_frame_rcv(Host &x, int f, const Frame &p, int64_t now)
+    {
+        RcvState &r = *x.lr;
         if (!(p.flags & F_SYN) && r.r_up && r.r_srtt < 0 && r.r_synack_t >= 0)
         {
             int64_t m = now - r.r_synack_t;
@@ -1689,6 +1723,23 @@ struct Model
                 rcv_closed(r);
         }
     }
+    // one event per frame: the side's record in LDS for its duration
+    static __device__ void on_frame(Host &x, const Frame &p, int64_t now)
+    {
+        const int f = (int)p.flow;
+        if (p.to_sender)
+        {
+            snd_open(x, f);
+            on_frame_snd(x, f, p, now);
+            snd_close(x, f);
+        }
+        else
+        {
+            rcv_open(x, f);
+            on_frame_rcv(x, f, p, now);
+            rcv_close(x, f);
+        }
+    }
     static __device__ __forceinline__ const uint32_t *due_chain(const Batch &B, const Due &d)
     {
         const uint32_t *base = d.kind == 1 || d.kind == 3 || d.kind == 4 ? B.dchain
@@ -1711,7 +1762,8 @@ struct Model
 
     // -------------------------------------------------------------- one slot of a host
     // (src/step.cuh:2263-2528): returns 1 if it sent or holds frames; *wake: its next slot
-    static __device__ int host_step(const Batch &B, const WorldDesc &W, WorldRun *R, int hl, int64_t t, int64_t *wake)
+    static __device__ int host_step(const Batch &B, const WorldDesc &W, WorldRun *R, int hl, int64_t t, int64_t *wake, SndState *ls,
+                                    RcvState *lr)
     {
         const int64_t S0 = t * SLOT_PS, E0 = S0 + SLOT_PS;
         const int h = W.host0 + hl, lid = W.link0 + hl;
@@ -1752,6 +1804,8 @@ struct Model
         x.cubic = B.host_cc[h] == CC_CUBIC;
         x.due = B.due + (size_t)h * MAX_DUE;
         x.slot_end = E0;
+        x.ls = ls;
+        x.lr = lr;
         Due *due = x.due;
         // timers due in this slot, by time. A list is scanned only when one of its timers falls in
         // the slot: tmin_s / tmin_r are its earliest, exact since the end of the host's last slot
@@ -1826,16 +1880,18 @@ struct Model
                 pn++;
                 int f = B.mg_conn[pm];
                 chain_root_msg(x.echain, pm - W.msg0);
-                B.snd[f].m_due++;
-                if (B.snd[f].st == S_EST)
+                SndState &s = snd_open(x, f);
+                s.m_due++;
+                if (s.st == S_EST)
                     app_write(x, f, now);
+                snd_close(x, f);
             }
             else if (now == tp)
             {
                 // a connection's application starts: SYN
                 pn++;
                 int f = B.mg_conn[pm];
-                SndState &s = B.snd[f];
+                SndState &s = snd_open(x, f);
                 chain_root(x.echain, hl, f - W.conn0);
                 s.m_due = 1;
                 add_active(x, true, f);
@@ -1847,6 +1903,7 @@ struct Model
                 timer_chain(x, B.rchain, f, CONN_TIMEOUT_PS);
                 s.timing_time = now;
                 emit(x, f, F_SYN, 0, 0, SYN_FRAME, false, now);
+                snd_close(x, f);
             }
             else if (now == td && (now != tr || timer_before_arrival(x, due[x.id], i2 < n2 ? m2[i2] : m1[i1])))
             {
@@ -1856,11 +1913,15 @@ struct Model
                 if (d.kind == 0)
                 {
                     if (B.snd[f].rto_at == d.at)
+                    {
+                        snd_open(x, f);
                         on_rto(x, f, now);
+                        snd_close(x, f);
+                    }
                 }
                 else if (d.kind == 2)
                 {
-                    SndState &s = B.snd[f];
+                    SndState &s = snd_open(x, f);
                     if (s.syn_at == d.at && s.st == S_EST)
                     {
                         s.syn_at = -1; // the deferred SendPendingData
@@ -1874,10 +1935,11 @@ struct Model
                         timer_chain(x, B.rchain, f, s.rto);
                         emit(x, f, F_SYN, 0, 0, SYN_FRAME, false, now);
                     }
+                    snd_close(x, f);
                 }
                 else
                 {
-                    RcvState &r = B.rcv[f];
+                    RcvState &r = rcv_open(x, f);
                     if (d.kind == 1 && r.delack_at == d.at)
                     {
                         r.delack_at = -1;
@@ -1908,6 +1970,7 @@ struct Model
                         r.srv_syn_at = -1;
                         send_synack(x, f, now);
                     }
+                    rcv_close(x, f);
                 }
             }
             else

@@ -116,6 +116,7 @@ class Batch:
         _track(self, 1)
 
     def _create(self):
+        self._h = ctypes.c_void_p()  # (a fresh handle: close() left None)
         arr = (ctypes.c_void_p * len(self.worlds))(*[w._h for w in self.worlds])
         _check(lib().trd_batch_create(arr, len(self.worlds), ctypes.byref(PackCfg(*self._cfg)), ctypes.byref(self._h)))
 
