@@ -1727,8 +1727,13 @@ RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, ui
     return act;
 }
 
-// ------------------------------------------------------------------ switch (src/rdma/rstep.cuh:2117-2467)
-struct SCtx
+// ------------------------------------------------------------------ switch: ports and the MMU
+// A switch's slot works on a SwSlot: its ports (in device memory, or LDS in a safe slot), the
+// ports' per-slot state, the switch-wide MMU counters in registers and the running event's chain.
+// The behaviour is the HPCC simulator's SwitchNode + SwitchMmu (src/rdma/rstep.cuh:2117-2467
+// restate it); here it is organised as the MMU's accounting (admission, release, PFC decisions),
+// the egress port's queues and transmitter, and the ingress path that ties them together.
+struct SwSlot
 {
     const Sim *s;
     const Params *P;
@@ -1750,171 +1755,196 @@ struct SCtx
     int ep;    // the running event's port (its tail positions: one writer per port and slot)
     int ord;
 };
-RD_HD const Chain &sc_cur(SCtx &c)
+// the running event's chain, pushed the first time an event it schedules needs it
+RD_HD const Chain &sw_event_chain(SwSlot &c)
 {
     if (!c.cur_ok)
     {
-        if (c.cur_code)
-            cpush(*c.s, c.pt[c.ep].ta, &c.cur, c.cur_code, *c.cur_src);
-        else
-            c.cur = *c.cur_src;
         c.cur_ok = true;
+        if (!c.cur_code)
+            c.cur = *c.cur_src;
+        else
+            cpush(*c.s, c.pt[c.ep].ta, &c.cur, c.cur_code, *c.cur_src);
     }
     return c.cur;
 }
-RD_HD uint32_t pfc_thresh(const SCtx &c, int port)
-{
-    return (c.P->buffer_size - c.sw.total_hdrm - c.sw.total_rsrv - c.sw.shared_used) >> c.pc[port].shift;
-}
-RD_HD uint32_t shared_used(const Port &p) { return p.ing > RESERVE ? p.ing - RESERVE : 0; }
 
-RD_HD void s_transmit(SCtx &c, int port, const Pkt &p)
+// ---- the MMU (shared buffer with per-port reserve, headroom and dynamic PFC thresholds)
+// a port's use of the shared pool: its ingress bytes above the reserve
+RD_HD uint32_t mmu_shared_of(const Port &p) { return p.ing > RESERVE ? p.ing - RESERVE : 0; }
+// the PFC threshold of an ingress port: the free shared buffer, shifted by the port's factor
+RD_HD uint32_t mmu_pfc_limit(const SwSlot &c, int port)
+{
+    const uint32_t free_shared = c.P->buffer_size - c.sw.total_hdrm - c.sw.total_rsrv - c.sw.shared_used;
+    return free_shared >> c.pc[port].shift;
+}
+// a data frame entering through `inport` (CheckIngressAdmission, UpdateIngressAdmission): false
+// if it is dropped; else its bytes go to the reserve, then the shared pool, then headroom
+RD_HD bool mmu_admit(SwSlot &c, int inport, uint32_t size)
+{
+    Port &in = c.pt[inport];
+    const uint32_t lim = mmu_pfc_limit(c, inport);
+    if (size + in.hdrm > c.pc[inport].hdrm_cap && size + mmu_shared_of(in) > lim)
+        return false;
+    const uint32_t after = in.ing + size;
+    if (after > RESERVE && after - RESERVE > lim)
+    {
+        in.hdrm += size;
+        return true;
+    }
+    in.ing = after;
+    if (after > RESERVE)
+        c.sw.shared_used += size < after - RESERVE ? size : after - RESERVE;
+    return true;
+}
+// a data frame leaving (RemoveFromIngressAdmission): headroom first, then the shared pool and
+// the reserve
+RD_HD void mmu_release(SwSlot &c, int inport, uint32_t size)
+{
+    Port &in = c.pt[inport];
+    const uint32_t hd = in.hdrm < size ? in.hdrm : size, rest = size - hd, sh = mmu_shared_of(in);
+    in.hdrm -= hd;
+    in.ing -= rest;
+    c.sw.shared_used -= rest < sh ? rest : sh;
+}
+// ECN at dequeue (ShouldSendCN): above Kmax always, between Kmin and Kmax with a probability
+// rising to Pmax, drawn from the port's own stream
+RD_HD bool mmu_ecn_mark(SwSlot &c, int port)
+{
+    const PortCfg &pc = c.pc[port];
+    const uint32_t q = c.pt[port].egr;
+    if (q > pc.kmax)
+        return true;
+    if (q <= pc.kmin)
+        return false;
+    return rand_u01(c.pt[port].rng) < pc.pmax * (double)(q - pc.kmin) / (double)(pc.kmax - pc.kmin);
+}
+
+// ---- an egress port: the control queue (PFC frames; ACKs under ack_high_prio), the data queue
+// and the transmitter
+// a frame onto the port's link (TransmitStart): as a NIC's, the frame first, then the chains
+RD_HD void port_emit(SwSlot &c, int port, const Pkt &p)
 {
     const Sim &s = *c.s;
-    int64_t tx = link_txo(s, c.w[port].ob, p.size);
-    c.pt[port].tc_at = c.now + tx;
-    // the frame first (the packet is not kept across the chain work), then the chains
-    const int ord = c.ord;
+    PortRun &r = c.w[port];
+    const int64_t busy = link_txo(s, r.ob, p.size), lands = c.now + busy + r.ob.delay;
+    c.pt[port].tc_at = c.now + busy;
+    const int done_ord = c.ord;
     c.ord += 2;
-    c.w[port].nout++;
-    int64_t ts = c.now + tx + c.w[port].ob.delay;
-    LFrame *f = ob_push(s, c.pend, c.pend_n, ts, c.w[port].ob);
-    if (f)
+    r.nout++;
+    LFrame *fr = ob_push(s, c.pend, c.pend_n, lands, r.ob);
+    if (fr)
     {
-        f->ts = ts;
-        f->p = p;
-        f->p.ord = (uint8_t)(ord + 1 < 255 ? ord + 1 : 255);
+        fr->ts = lands;
+        fr->p = p;
+        fr->p.ord = (uint8_t)(done_ord < 254 ? done_ord + 1 : 255);
     }
-    const Chain &cur = sc_cur(c);
-    cpush(s, c.pt[c.ep].ta, &c.tc[port], ccode(tx, ord), cur);
-    if (!f)
-        return;
-    f->c = cur;
-    if (s.txt_rec.rec)
-        trace_tx(s, c.now, c.node, port + 1, f->p);
+    const Chain &ev = sw_event_chain(c);
+    cpush(s, c.pt[c.ep].ta, &c.tc[port], ccode(busy, done_ord), ev);
+    if (fr)
+    {
+        fr->c = ev;
+        if (s.txt_rec.rec)
+            trace_tx(s, c.now, c.node, port + 1, fr->p);
+    }
 }
-RD_HD void s_enqueue(SCtx &c, int port, int q, const Pkt &p)
+// a frame into one of the port's queues (0: control, else data); the data queue's head is kept
+// in the port's slot state
+RD_HD void port_push(SwSlot &c, int port, int queue, const Pkt &p)
 {
     const Sim &s = *c.s;
     Port &pt = c.pt[port];
     const size_t pi = (size_t)c.pbase + port;
-    if (q == 0)
+    const bool ctrl = queue == 0;
+    const int n = ctrl ? pt.q0_n : pt.q3_n, cap = ctrl ? s.q0_cap : s.q3_cap;
+    if (n >= cap)
     {
-        if (pt.q0_n >= s.q0_cap)
-        {
-            count(s, EV_QUEUE_OVF);
-            return;
-        }
-        s.q0[pi * s.q0_cap + ring_add(pt.q0_head, pt.q0_n, s.q0_cap)] = p;
-        pt.q0_n++;
+        count(s, EV_QUEUE_OVF);
+        return;
+    }
+    if (ctrl)
+    {
+        s.q0[pi * s.q0_cap + ring_add(pt.q0_head, n, s.q0_cap)] = p;
+        pt.q0_n = n + 1;
     }
     else
     {
-        if (pt.q3_n >= s.q3_cap)
-        {
-            count(s, EV_QUEUE_OVF);
-            return;
-        }
-        s.q3[pi * s.q3_cap + ring_add(pt.q3_head, pt.q3_n, s.q3_cap)] = p;
-        if (pt.q3_n == 0)
+        s.q3[pi * s.q3_cap + ring_add(pt.q3_head, n, s.q3_cap)] = p;
+        if (!n)
             c.w[port].qhead = p;
-        pt.q3_n++;
+        pt.q3_n = n + 1;
     }
     pt.bytes_total += p.size;
 }
-RD_HD void s_start_q0(SCtx &c, int port)
+// the control queue's head onto an idle link
+RD_HD void port_send_ctrl(SwSlot &c, int port)
 {
     const Sim &s = *c.s;
     Port &pt = c.pt[port];
     if (pt.tc_at >= 0)
         return;
-    const size_t pi = (size_t)c.pbase + port;
-    const Pkt &p = s.q0[pi * s.q0_cap + pt.q0_head]; // (its slot is not reused before the copy into the frame)
+    const Pkt &p = s.q0[((size_t)c.pbase + port) * s.q0_cap + pt.q0_head]; // (the slot outlives the copy into the frame)
     pt.q0_head = ring_add(pt.q0_head, 1, s.q0_cap);
     pt.q0_n--;
     pt.bytes_total -= p.size;
     pt.tx_bytes += p.size;
-    s_transmit(c, port, p);
+    port_emit(c, port, p);
 }
-// QbbNetDevice::SendPfc
-RD_HD void s_send_pfc(SCtx &c, int port, int type)
+// a PAUSE or RESUME for the data class out of `port` (SendPfc), ahead of everything queued there
+RD_HD void port_pfc(SwSlot &c, int port, bool pause)
 {
-    Pkt p = {};
-    p.kind = type == 0 ? K_PAUSE : K_RESUME;
-    p.q = PG;
-    p.size = PFC_SIZE;
-    count(*c.s, type == 0 ? EV_PAUSE : EV_RESUME);
-    s_enqueue(c, port, 0, p);
+    count(*c.s, pause ? EV_PAUSE : EV_RESUME);
+    Pkt f = {};
+    f.kind = pause ? K_PAUSE : K_RESUME;
+    f.q = PG;
+    f.size = PFC_SIZE;
+    port_push(c, port, 0, f);
     if (c.pt[port].q0_n > 0)
-        s_start_q0(c, port);
+        port_send_ctrl(c, port);
 }
-// SwitchNotifyDequeue, ECN marking, CheckAndSendResume, INT, TransmitStart for a data-queue frame
-// (src/rdma/rstep.cuh:2322-2395)
-RD_HD void s_start_q3(SCtx &c, int port, Pkt p)
+// a data-queue frame leaves through `port`: the MMU lets go of it (in a safe slot only its ingress
+// counter moves), it may be ECN-marked, its ingress port may RESUME, an HPCC frame gets this hop's
+// INT record, then it is sent
+RD_HD void port_send_data(SwSlot &c, int port, Pkt p)
 {
     const Sim &s = *c.s;
     const Params &P = *c.P;
     Port &pt = c.pt[port];
-    Port &in = c.pt[p.inport];
     if (c.safe)
-    {
-        // no headroom anywhere, no port paused: RemoveFromIngressAdmission takes it from ing
-        at_sub32(&in.ing, (uint32_t)p.size);
-        pt.egr -= p.size;
-    }
+        at_sub32(&c.pt[p.inport].ing, (uint32_t)p.size); // (no headroom, no pause anywhere in a safe slot)
     else
+        mmu_release(c, p.inport, p.size);
+    pt.egr -= p.size;
+    if (P.ecn && mmu_ecn_mark(c, port))
     {
-        // RemoveFromIngressAdmission, RemoveFromEgressAdmission
-        uint32_t from_hdrm = in.hdrm < p.size ? in.hdrm : p.size;
-        uint32_t sh = in.ing > RESERVE ? in.ing - RESERVE : 0;
-        uint32_t from_shared = p.size - from_hdrm < sh ? p.size - from_hdrm : sh;
-        in.hdrm -= from_hdrm;
-        in.ing -= p.size - from_hdrm;
-        c.sw.shared_used -= from_shared;
-        pt.egr -= p.size;
+        p.ecn = 3;
+        count(s, EV_MARK);
     }
-    if (P.ecn)
+    Port &in = c.pt[p.inport];
+    if (!c.safe && in.mmu_paused && in.hdrm == 0)
     {
-        // SwitchMmu::ShouldSendCN
-        bool mark = false;
-        const uint32_t kmin = c.pc[port].kmin, kmax = c.pc[port].kmax;
-        if (pt.egr > kmax)
-            mark = true;
-        else if (pt.egr > kmin)
+        const uint32_t sh = mmu_shared_of(in);
+        if (sh == 0 || sh + RESUME_OFFSET <= mmu_pfc_limit(c, p.inport))
         {
-            double pr = c.pc[port].pmax * (double)(pt.egr - kmin) / (double)(kmax - kmin);
-            mark = rand_u01(pt.rng) < pr;
-        }
-        if (mark)
-        {
-            p.ecn = 3;
-            count(s, EV_MARK);
-        }
-    }
-    if (!c.safe && in.mmu_paused)
-    {
-        uint32_t su = shared_used(in);
-        if (in.hdrm == 0 && (su == 0 || su + RESUME_OFFSET <= pfc_thresh(c, p.inport)))
-        {
-            s_send_pfc(c, p.inport, 1);
+            port_pfc(c, p.inport, false);
             in.mmu_paused = 0;
         }
     }
     if (p.kind == K_DATA && P.cc == 3 && p.nhop < 0xFFFF)
     {
-        const uint64_t hv = hop_make((uint64_t)c.now, pt.tx_bytes, pt.bytes_total, c.pc[port].bps, P.int_multi);
+        const uint64_t rec = hop_make((uint64_t)c.now, pt.tx_bytes, pt.bytes_total, c.pc[port].bps, P.int_multi);
+        const int slot_i = p.nhop % MAX_HOP;
         // (static indices only: a frame indexed at run time would leave the registers for scratch memory)
-        const int hi = p.nhop % MAX_HOP;
         for (int k = 0; k < MAX_HOP; k++)
-            if (k == hi)
-                p.hop[k] = hv;
+            p.hop[k] = k == slot_i ? rec : p.hop[k];
         p.nhop++;
     }
     pt.tx_bytes += p.size;
-    s_transmit(c, port, p);
+    port_emit(c, port, p);
 }
-// QbbNetDevice::DequeueAndTransmit on a switch (src/rdma/rstep.cuh:2299-2320)
-RD_HD void s_dequeue(SCtx &c, int port)
+// the port's link is free (DequeueAndTransmit): control frames first, then the data queue unless
+// the data class is paused
+RD_HD void port_pump(SwSlot &c, int port)
 {
     const Sim &s = *c.s;
     Port &pt = c.pt[port];
@@ -1922,85 +1952,73 @@ RD_HD void s_dequeue(SCtx &c, int port)
         return;
     if (pt.q0_n > 0)
     {
-        s_start_q0(c, port);
+        port_send_ctrl(c, port);
         return;
     }
-    if ((pt.paused >> PG & 1) || pt.q3_n == 0)
+    if (pt.q3_n == 0 || (pt.paused >> PG & 1))
         return;
-    const Pkt p = c.w[port].qhead;
+    PortRun &r = c.w[port];
+    const Pkt p = r.qhead;
     pt.q3_head = ring_add(pt.q3_head, 1, s.q3_cap);
     pt.q3_n--;
     pt.bytes_total -= p.size;
     if (pt.q3_n > 0) // (the next head's load overlaps this frame's transmission)
-        c.w[port].qhead = s.q3[((size_t)c.pbase + port) * s.q3_cap + pt.q3_head];
-    s_start_q3(c, port, p);
+        r.qhead = s.q3[((size_t)c.pbase + port) * s.q3_cap + pt.q3_head];
+    port_send_data(c, port, p);
 }
-// SwitchNode's route: the egress port of a data frame or ACK (-1: none) and its queue
-// (src/rdma/rstep.cuh:2397-2409)
-RD_HD int s_route(const Sim &s, const Params &P, int x, const Pkt &p, int *q)
+
+// ---- the ingress path
+// the egress port of a data frame or an ACK (-1: no route) and its queue: equal-cost routes to
+// the destination host, chosen by the 5-tuple's hash with the switch's seed
+RD_HD int sw_route(const Sim &s, const Params &P, int x, const Pkt &p, int *q)
 {
     const SwInfo &si = s.si[x];
-    const bool data = p.kind == K_DATA;
-    const int src = (data ? p.src : p.dst) - si.hbase, dst = (data ? p.dst : p.src) - si.hbase;
-    const Route rt = s.rt[si.rt_off + dst];
-    *q = (P.ack_high_prio && !data) ? 0 : PG;
-    if (rt.n == 0)
+    const bool fwd = p.kind == K_DATA;
+    const int a = (fwd ? p.src : p.dst) - si.hbase, b = (fwd ? p.dst : p.src) - si.hbase;
+    *q = !fwd && P.ack_high_prio ? 0 : PG;
+    const Route rt = s.rt[si.rt_off + b];
+    if (!rt.n)
         return -1;
-    const uint32_t w2 = data ? (p.sport | 100u << 16) : (100u | (uint32_t)p.sport << 16);
-    const uint32_t hsh = ecmp_hash(node_ip(src), node_ip(dst), w2, si.seed);
-    return s.rtp[rt.off + hsh % rt.n];
+    const uint32_t ports_word = fwd ? (p.sport | 100u << 16) : (100u | (uint32_t)p.sport << 16);
+    return s.rtp[rt.off + ecmp_hash(node_ip(a), node_ip(b), ports_word, si.seed) % rt.n];
 }
-// ingress admission, PFC PAUSE, enqueue or cut-through (src/rdma/rstep.cuh:2411-2467)
-RD_HD void s_forward(SCtx &c, int inport, Pkt p, int out, int q)
+// a frame through `inport` bound for `out` (queue q): the MMU's admission (a drop, or a PAUSE when
+// the ingress port crosses its limit), then cut-through to an idle port with nothing queued, or
+// the queue and the pump
+RD_HD void sw_ingress(SwSlot &c, int inport, Pkt p, int out, int q)
 {
-    const Sim &s = *c.s;
     if (out < 0)
         return;
-    if (q != 0 && c.safe)
+    if (q && c.safe)
     {
-        // below every threshold for the whole slot: no drop, no headroom, no PAUSE; only ing
-        // changes (shared_used follows from it at the slot's end)
-        at_add32(&c.pt[inport].ing, (uint32_t)p.size);
+        at_add32(&c.pt[inport].ing, (uint32_t)p.size); // (below every limit all slot: shared_used follows at its end)
         c.pt[out].egr += p.size;
     }
-    else if (q != 0)
+    else if (q)
     {
-        Port &in = c.pt[inport];
-        const uint32_t th = pfc_thresh(c, inport);
-        // CheckIngressAdmission
-        if (p.size + in.hdrm > c.pc[inport].hdrm_cap && p.size + shared_used(in) > th)
+        if (!mmu_admit(c, inport, p.size))
         {
-            count(s, EV_DROP);
+            count(*c.s, EV_DROP);
             return;
         }
-        // UpdateIngressAdmission, UpdateEgressAdmission
-        const uint32_t nb = in.ing + p.size;
-        if (nb <= RESERVE)
-            in.ing += p.size;
-        else if (nb - RESERVE > th)
-            in.hdrm += p.size;
-        else
-        {
-            in.ing += p.size;
-            c.sw.shared_used += p.size < nb - RESERVE ? p.size : nb - RESERVE;
-        }
         c.pt[out].egr += p.size;
-        // CheckAndSendPfc
-        if (!in.mmu_paused && (in.hdrm > 0 || shared_used(in) >= pfc_thresh(c, inport)))
+        Port &in = c.pt[inport];
+        if (!in.mmu_paused && (in.hdrm > 0 || mmu_shared_of(in) >= mmu_pfc_limit(c, inport)))
         {
-            s_send_pfc(c, inport, 0);
+            port_pfc(c, inport, true);
             in.mmu_paused = 1;
         }
     }
     p.inport = (uint8_t)inport;
-    Port &o = c.pt[out];
-    if (q != 0 && o.tc_at < 0 && o.q0_n == 0 && o.q3_n == 0 && !(o.paused >> PG & 1))
+    const Port &o = c.pt[out];
+    const bool idle = o.tc_at < 0 && o.q0_n == 0 && o.q3_n == 0 && !(o.paused >> PG & 1);
+    if (q && idle)
     {
-        s_start_q3(c, out, p); // an idle port with empty queues hands it straight on
+        port_send_data(c, out, p);
         return;
     }
-    s_enqueue(c, out, q, p);
-    s_dequeue(c, out);
+    port_push(c, out, q, p);
+    port_pump(c, out);
 }
 
 // a frame's tx time on a port's in-link
@@ -2009,7 +2027,7 @@ RD_HD int64_t port_in_tx(const Sim &s, const PortRun &r, uint32_t bytes)
     return bytes == r.isz[0] ? r.itx[0] : bytes == r.isz[1] ? r.itx[1] : rc_tx(s, r.irc, r.ibps, bytes);
 }
 // a port's next arrival of the slot
-RD_HD void sw_peek(const Sim &s, const SCtx &c, int p)
+RD_HD void sw_peek(const Sim &s, const SwSlot &c, int p)
 {
     PortRun &w = c.w[p];
     w.ats = INT64_MAX;
@@ -2021,7 +2039,7 @@ RD_HD void sw_peek(const Sim &s, const SCtx &c, int p)
     }
 }
 
-RD_HD void sw_ctx(SCtx &c, const Sim &s, int x, int64_t slot, uint32_t *pend, int pend_n, Port *pt, PortRun *w)
+RD_HD void sw_ctx(SwSlot &c, const Sim &s, int x, int64_t slot, uint32_t *pend, int pend_n, Port *pt, PortRun *w)
 {
     const SwInfo &si = s.si[x];
     c.s = &s;
@@ -2043,7 +2061,7 @@ RD_HD void sw_ctx(SCtx &c, const Sim &s, int x, int64_t slot, uint32_t *pend, in
     c.now = slot * s.D;
 }
 // a port's slot set-up: its arrivals' cursor and its out bucket
-RD_HD void sw_port_open(const Sim &s, SCtx &c, int p, int64_t slot)
+RD_HD void sw_port_open(const Sim &s, SwSlot &c, int p, int64_t slot)
 {
     PortRun &r = c.w[p];
     r.ai = 0;
@@ -2064,7 +2082,7 @@ RD_HD void sw_port_open(const Sim &s, SCtx &c, int p, int64_t slot)
     ob_open(s, r.ob, c.pc[p].link_out, c.pt[p].ob_last);
 }
 // a port's slot end: its bucket's count out; busy (sent, or holds frames)?
-RD_HD bool sw_port_close(const Sim &s, SCtx &c, int p, unsigned long long *nout)
+RD_HD bool sw_port_close(const Sim &s, SwSlot &c, int p, unsigned long long *nout)
 {
     ob_flush(s, c.w[p].ob);
     c.pt[p].ob_last = c.w[p].ob.last;
@@ -2075,7 +2093,7 @@ RD_HD bool sw_port_close(const Sim &s, SCtx &c, int p, unsigned long long *nout)
 // All of a switch's events of the slot in order on one lane (the reference's serial path,
 // src/rdma/rstep.cuh:3032-3189): candidates are each port's next arrival and its TransmitComplete,
 // in port order, arrival first; ties by the chains; undecided ties keep the earlier candidate.
-RD_HD void sw_serial_events(const Sim &s, SCtx &c, int deg, int64_t t1)
+RD_HD void sw_serial_events(const Sim &s, SwSlot &c, int deg, int64_t t1)
 {
     for (int p = 0; p < deg; p++)
         sw_peek(s, c, p);
@@ -2114,9 +2132,9 @@ RD_HD void sw_serial_events(const Sim &s, SCtx &c, int deg, int64_t t1)
         {
             c.cur_src = &c.tc[p];
             c.cur_code = 0;
-            sc_cur(c); // (the TransmitComplete's chain is overwritten by the next transmission)
+            sw_event_chain(c); // (the TransmitComplete's chain is overwritten by the next transmission)
             c.pt[p].tc_at = -1;
-            s_dequeue(c, p);
+            port_pump(c, p);
             continue;
         }
         const LFrame &f = s.msg[c.w[p].abase + c.w[p].ai];
@@ -2132,14 +2150,14 @@ RD_HD void sw_serial_events(const Sim &s, SCtx &c, int deg, int64_t t1)
             else
             {
                 c.pt[p].paused &= ~(1u << pk.q);
-                s_dequeue(c, p);
+                port_pump(c, p);
             }
         }
         else
         {
             int q = PG;
-            const int out = s_route(s, *c.P, c.x, pk, &q);
-            s_forward(c, p, pk, out, q);
+            const int out = sw_route(s, *c.P, c.x, pk, &q);
+            sw_ingress(c, p, pk, out, q);
         }
     }
 }
@@ -2149,7 +2167,7 @@ RD_HD void sw_serial_events(const Sim &s, SCtx &c, int deg, int64_t t1)
 RD_HD bool switch_step(const Sim &s, int x, int64_t slot, uint32_t *pend, int pend_n)
 {
     const SwInfo &si = s.si[x];
-    SCtx c;
+    SwSlot c;
     sw_ctx(c, s, x, slot, pend, pend_n, s.ps + si.pbase, s.pr + si.pbase);
     for (int p = 0; p < si.deg; p++)
         sw_port_open(s, c, p, slot);
@@ -2210,7 +2228,7 @@ struct SwWork
 
 // the arrival code and chain of arrival j (of ingress p) in the workspace
 RD_HD const LFrame &sw_arr(const Sim &s, const SwWork &k, int p, int j) { return s.msg[k.w[p].abase + (uint64_t)(j - k.first[p])]; }
-RD_HD uint32_t sw_acode(const Sim &s, const SCtx &c, int p, const LFrame &f)
+RD_HD uint32_t sw_acode(const Sim &s, const SwSlot &c, int p, const LFrame &f)
 {
     const PortRun &r = c.w[p];
     return ccode(port_in_tx(s, r, f.p.size) + r.idelay, f.p.ord);
@@ -2230,7 +2248,7 @@ RD_HD uint32_t sw_acode(const Sim &s, const SCtx &c, int p, const LFrame &f)
 #define RD_PROF_T(v)
 #define RD_PROF_ADD(i, d)
 #endif
-RD_HD void sw_egress_events(const Sim &s, SCtx &c, SwWork &k, int o, int deg, int64_t t1)
+RD_HD void sw_egress_events(const Sim &s, SwSlot &c, SwWork &k, int o, int deg, int64_t t1)
 {
     // the ingress ports with arrivals left for o (bit p: port p)
     uint32_t live = 0;
@@ -2285,9 +2303,9 @@ RD_HD void sw_egress_events(const Sim &s, SCtx &c, SwWork &k, int o, int deg, in
             c.now = tc;
             c.cur_src = &c.tc[o];
             c.cur_code = 0;
-            sc_cur(c);
+            sw_event_chain(c);
             k.pt[o].tc_at = -1;
-            s_dequeue(c, o);
+            port_pump(c, o);
             RD_PROF_T(q2);
             RD_PROF_ADD(30, q2 - q1);
             continue;
@@ -2306,11 +2324,11 @@ RD_HD void sw_egress_events(const Sim &s, SCtx &c, SwWork &k, int o, int deg, in
             else
             {
                 k.pt[o].paused &= ~(1u << pk.q);
-                s_dequeue(c, o);
+                port_pump(c, o);
             }
         }
         else
-            s_forward(c, bp, pk, o, k.q[kb]);
+            sw_ingress(c, bp, pk, o, k.q[kb]);
         RD_PROF_T(q3);
         RD_PROF_ADD(31, q3 - q1);
     }
@@ -2323,7 +2341,7 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
     const SwInfo &si = s.si[x];
     const int deg = si.deg;
     const int64_t t1 = (slot + 1) * s.D;
-    SCtx c;
+    SwSlot c;
     sw_ctx(c, s, x, slot, pend, pend_n, k.pt, k.w);
     c.pc = k.pc;
 #if defined(RD_PROF) && defined(__HIP_DEVICE_COMPILE__)
@@ -2369,7 +2387,7 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
                     k.own[j] = (int8_t)p;
                 else
                 {
-                    const int out = s_route(s, *c.P, x, f.p, &q);
+                    const int out = sw_route(s, *c.P, x, f.p, &q);
                     k.own[j] = (int8_t)out;
                     if (out >= 0 && q != 0)
                         k.abytes[p] += f.p.size;
