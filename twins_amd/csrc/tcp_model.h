@@ -125,8 +125,12 @@ struct Model
     }
     static __device__ __forceinline__ void count(WorldRun *R, int i, unsigned long long n = 1) { atomicAdd(&R->ev[i], n); }
 
-    // -------------------------------------------------------------- egress FIFO
-    // (src/step.cuh:167-317) frames not yet started; starts go to this slot's link message
+    // -------------------------------------------------------------- egress queue
+    // One sending queue per link (a host's NIC, a switch port): frames wait in a ring until the
+    // link is free; a frame whose transmission starts in this slot goes to the link's message for
+    // the slot with its arrival time at the far end. The queue's state lives in registers for the
+    // entity's slot (load / store). Admission is RED with MinTh = MaxTh = K and QW = 1 on the
+    // queue-disc length ns-3 would see (src/step.cuh:167-317 restates the same rules).
     struct Fifo
     {
         const Batch *B;
@@ -157,7 +161,7 @@ struct Model
             sent = 0;
             out = b.msg + ((size_t)ring_t * b.nlinks + link) * MAX_TX;
         }
-        // the slot's message and the queue state back to memory
+        // the slot's message count and the queue back to memory
         __device__ __forceinline__ void store(int64_t t, int ring_t)
         {
             B->msg_cnt[(size_t)ring_t * B->nlinks + lid] = (long long)t << 16 | sent;
@@ -169,38 +173,44 @@ struct Model
             if (sent)
                 count(R, EV_PKT_EVENTS, (unsigned long long)sent);
         }
-        // start every frame whose transmission begins at or before `until` within the slot
+        // the event that starts a frame: the previous frame's TransmitComplete when the frame had
+        // to wait for the link (its chain: the last transmission's, one more run), else the
+        // frame's own enqueue (a link that never sent counts as idle)
+        __device__ __forceinline__ void start_event(Frame &o, int64_t start)
+        {
+            const bool waited = o.ts <= link_free && link_free > last_start;
+            if (waited)
+            {
+                chain_child(last, (uint32_t)((link_free - last_start) / PS_PER_BYTE), last);
+                chain_store(o.chain, last);
+            }
+            else
+            {
+                chain_load(last, o.chain);
+                idle_start = start;
+            }
+            last_start = start;
+        }
+        // starts, in order, the frames whose transmission begins by `until` within the slot
         __device__ void advance(int64_t until)
         {
-            while (size > 0 && sent < MAX_TX)
+            for (; size > 0; size--)
             {
-                const Frame &f = ring[head];
-                int64_t start = f.ts > link_free ? f.ts : link_free;
-                if (start > until || start >= slot_end)
+                if (sent >= MAX_TX)
+                {
+                    count(R, EV_TXCAP);
                     return;
-                int64_t end = start + (int64_t)f.len * PS_PER_BYTE;
-                Frame o = f;
-                // the start event: the previous frame's TransmitComplete if this one waited,
-                // else its own enqueue (a link that never sent is idle)
-                if (f.ts <= link_free && link_free > last_start)
-                {
-                    chain_child(last, (uint32_t)((link_free - last_start) / PS_PER_BYTE), last);
-                    chain_store(o.chain, last);
                 }
-                else
-                {
-                    idle_start = start;
-                    chain_load(last, o.chain);
-                }
-                last_start = start;
-                o.ts = end + PROP_PS;
+                Frame o = ring[head];
+                const int64_t start = o.ts > link_free ? o.ts : link_free;
+                if (start >= slot_end || start > until)
+                    return;
+                start_event(o, start);
+                link_free = start + (int64_t)o.len * PS_PER_BYTE;
+                o.ts = link_free + PROP_PS; // its arrival at the far end
                 out[sent++] = o;
-                link_free = end;
-                head = head + 1 == cap ? 0 : head + 1;
-                size--;
+                head = head + 1 < cap ? head + 1 : 0;
             }
-            if (size > 0 && sent >= MAX_TX)
-                count(R, EV_TXCAP);
         }
         // a queue-trace record (src/step.cuh:266-279), stored field by field (no record on the stack)
         __device__ __forceinline__ void trace(int64_t now, int qlen, uint32_t flow, uint32_t sa, uint16_t len, uint8_t flags, uint8_t to_sender)
@@ -219,42 +229,48 @@ struct Model
                 r->to_sender = to_sender;
             }
         }
-        // RED (MinTh = MaxTh = K, QW = 1) on the queue-disc length: the frame in ns-3's
-        // one-packet device queue is left out unless the Wake that refills it has yet to run
+        // The queue-disc length ns-3's RED sees at `now`. The first queued frame sits in the
+        // device's one-frame queue (outside the queue disc) once the device has pulled it: when a
+        // transmission started before `now`, or when the device's Wake ran at this very instant
+        // on an idle link; until then it still counts.
+        __device__ __forceinline__ int qdisc_len(int64_t now) const
+        {
+            const bool pulled = last_start < now || (woke == now && idle_start == now);
+            return size > 0 && link_free > now && pulled ? size - 1 : size;
+        }
+        // a frame offered at `now`: RED marks (ECN-capable) or drops it at K, the queue limit
+        // drops it; false if dropped
         __device__ bool enqueue(const Frame &f, int64_t now)
         {
             advance(now);
-            bool refilled = last_start < now || (woke == now && idle_start == now);
-            int nq = (size > 0 && link_free > now && refilled) ? size - 1 : size;
+            const int len = qdisc_len(now);
             Frame g = f;
             g.ts = now;
-            if (nq >= red_k && nq > 1)
+            const bool over_k = len >= red_k && len > 1;
+            if (over_k && !(g.flags & F_ECT))
             {
-                if (g.flags & F_ECT)
-                {
-                    g.flags |= F_CE;
-                    count(R, EV_MARK);
-                }
-                else
-                {
-                    count(R, EV_DROP);
-                    return false;
-                }
+                count(R, EV_DROP);
+                return false;
             }
-            if (nq >= limit)
+            if (over_k)
+            {
+                g.flags |= F_CE;
+                count(R, EV_MARK);
+            }
+            if (len >= limit)
             {
                 count(R, EV_DROP);
                 return false;
             }
             if (B->qt.rec && now >= B->qt.from && now < B->qt.until)
-                trace(now, nq + 1, g.flow, g.sa, g.len, g.flags, g.to_sender);
+                trace(now, len + 1, g.flow, g.sa, g.len, g.flags, g.to_sender);
             if (size >= cap)
             {
                 count(R, EV_RING_OVF);
                 return false;
             }
-            int at = head + size;
-            ring[at >= cap ? at - cap : at] = g;
+            const int at = head + size < cap ? head + size : head + size - cap;
+            ring[at] = g;
             size++;
             return true;
         }
@@ -390,538 +406,499 @@ struct Model
         chain_store(fr.chain, x.echain);
         x.nic.enqueue(fr, now);
     }
-    static __device__ void send_ack(Host &x, int f, uint32_t ack, bool force_ece, bool no_ece, int64_t now)
-    {
-        RcvState &r = *x.lr;
-        bool ece = false;
-        if (force_ece)
-            ece = true;
-        else if (!no_ece && (r.ecn_r == E_CE_RCVD || r.ecn_r == E_SENDING_ECE))
-        {
-            ece = true;
-            r.ecn_r = E_SENDING_ECE;
-        }
-        r.delack_at = -1;
-        r.delack_count = 0;
-        emit(x, f, F_ACK | (ece ? F_ECE : 0), 0, ack, CTRL_FRAME, true, now);
-    }
 
-    // -------------------------------------------------------------- the sent list
-    // (src/step.cuh:409-975): TcpTxBuffer items (start, flags) in a ring of it_cap entries
-    struct Tx
+    // -------------------------------------------------------------- the sent ledger
+    // ns-3's TcpTxBuffer sent list as this model uses it (no SACK blocks; Reno's emulated SACK on
+    // duplicate ACKs): one entry per run of transmitted bytes - its first byte and its marks
+    // (IT_SACKED, IT_LOST, IT_RETX) - in a ring per connection, oldest first; an entry ends where
+    // the next begins (the last at s.high). The connection's record keeps what decisions need
+    // without walking the ring: bytes per mark, the sacked entries, and the retransmission
+    // candidates (lost, neither retransmitted nor sacked) with a lower bound of the first.
+    struct Ledger
     {
         SndState &s;
-        uint32_t *st;
-        uint8_t *fl;
+        uint32_t *first;
+        uint8_t *marks;
         uint32_t cap;
-        __device__ Tx(const Batch &B, SndState &rec, int f) : s(rec)
+        __device__ Ledger(const Batch &B, SndState &rec, int f) : s(rec)
         {
             const ConnStatic &c = B.cs[f];
-            st = B.it_start + c.it_off;
-            fl = B.it_fl + c.it_off;
+            first = B.it_start + c.it_off;
+            marks = B.it_fl + c.it_off;
             cap = c.it_cap;
         }
-        __device__ uint32_t at(uint32_t i) const
-        {
-            uint32_t v = s.it_head + i;
-            return v >= cap ? v - cap : v;
-        }
-        __device__ uint32_t beg(uint32_t i) const { return st[at(i)]; }
-        __device__ uint32_t end(uint32_t i) const { return i + 1 < s.it_n ? st[at(i + 1)] : s.high; }
-        __device__ uint8_t flags(uint32_t i) const { return fl[at(i)]; }
+        __device__ uint32_t pos(uint32_t i) const { return s.it_head + i < cap ? s.it_head + i : s.it_head + i - cap; }
+        __device__ uint32_t lo(uint32_t i) const { return first[pos(i)]; }
+        __device__ uint32_t hi(uint32_t i) const { return i + 1 < s.it_n ? first[pos(i + 1)] : s.high; }
+        __device__ uint8_t mk(uint32_t i) const { return marks[pos(i)]; }
     };
-    static __device__ __forceinline__ bool cand_of(uint8_t f) { return (f & (IT_LOST | IT_RETX | IT_SACKED)) == IT_LOST; }
-    static __device__ void it_set(Tx &t, uint32_t i, uint8_t nf)
+    static __device__ __forceinline__ bool lg_candidate(uint8_t m) { return (m & (IT_LOST | IT_RETX | IT_SACKED)) == IT_LOST; }
+    // adds (sign 1) or removes (sign -1) an entry of `len` bytes with marks m from the totals
+    static __device__ __forceinline__ void lg_count(SndState &s, uint8_t m, uint32_t len, int sign)
     {
-        SndState &s = t.s;
-        uint32_t k = t.at(i);
-        uint8_t of = t.fl[k];
-        if (of == nf)
-            return;
-        uint32_t len = t.end(i) - t.st[k];
-        uint8_t d = of ^ nf;
-        if (d & IT_SACKED)
+        const uint32_t d = sign > 0 ? len : 0u - len, one = sign > 0 ? 1u : 0u - 1u;
+        if (m & IT_SACKED)
         {
-            s.sack_b += (nf & IT_SACKED) ? len : 0u - len;
-            s.sacked_n += (nf & IT_SACKED) ? 1u : 0u - 1u;
+            s.sack_b += d;
+            s.sacked_n += one;
         }
-        if (d & IT_LOST)
-            s.lost_b += (nf & IT_LOST) ? len : 0u - len;
-        if (d & IT_RETX)
-            s.rtx_b += (nf & IT_RETX) ? len : 0u - len;
-        if (cand_of(of) != cand_of(nf))
-        {
-            if (cand_of(nf))
-            {
-                s.cand_n++;
-                if (i < s.cand)
-                    s.cand = i;
-            }
-            else
-                s.cand_n--;
-        }
-        t.fl[k] = nf;
+        if (m & IT_LOST)
+            s.lost_b += d;
+        if (m & IT_RETX)
+            s.rtx_b += d;
+        if (lg_candidate(m))
+            s.cand_n += one;
     }
-    static __device__ bool it_insert(Tx &t, WorldRun *R, uint32_t i, uint32_t start, uint8_t f)
+    // entry i's marks become m
+    static __device__ void lg_remark(Ledger &L, uint32_t i, uint8_t m)
     {
-        SndState &s = t.s;
-        if (s.it_n >= t.cap)
+        const uint32_t k = L.pos(i);
+        const uint8_t was = L.marks[k];
+        if (was == m)
+            return;
+        const uint32_t len = L.hi(i) - L.first[k];
+        lg_count(L.s, was, len, -1);
+        lg_count(L.s, m, len, 1);
+        if (lg_candidate(m) && !lg_candidate(was) && i < L.s.cand)
+            L.s.cand = i;
+        L.marks[k] = m;
+    }
+    // a new entry at place i (the entries from i on move up one); its bytes were counted before
+    // (a split) or are new (unmarked); false if the ring is full
+    static __device__ bool lg_insert(Ledger &L, WorldRun *R, uint32_t i, uint32_t start, uint8_t m)
+    {
+        SndState &s = L.s;
+        if (s.it_n >= L.cap)
         {
             count(R, EV_RTT_OVF);
             return false;
         }
         for (uint32_t j = s.it_n; j > i; j--)
         {
-            uint32_t a = t.at(j), b = t.at(j - 1);
-            t.st[a] = t.st[b];
-            t.fl[a] = t.fl[b];
+            const uint32_t to = L.pos(j), from = L.pos(j - 1);
+            L.first[to] = L.first[from];
+            L.marks[to] = L.marks[from];
         }
-        uint32_t k = t.at(i);
-        t.st[k] = start;
-        t.fl[k] = f;
+        L.first[L.pos(i)] = start;
+        L.marks[L.pos(i)] = m;
         s.it_n++;
-        if (s.cand >= i && s.cand > 0)
+        if (s.cand > 0 && s.cand >= i)
             s.cand++;
-        if (cand_of(f))
+        if (lg_candidate(m))
         {
             s.cand_n++;
-            if (i < s.cand)
-                s.cand = i;
+            s.cand = i < s.cand ? i : s.cand;
         }
         return true;
     }
-    static __device__ void it_remove_start(Tx &t, uint32_t i)
+    // entry i leaves (the ones above move down); the caller has settled its counts
+    static __device__ void lg_erase(Ledger &L, uint32_t i)
     {
-        SndState &s = t.s;
-        for (uint32_t j = i; j + 1 < s.it_n; j++)
+        SndState &s = L.s;
+        for (uint32_t j = i + 1; j < s.it_n; j++)
         {
-            uint32_t a = t.at(j), b = t.at(j + 1);
-            t.st[a] = t.st[b];
-            t.fl[a] = t.fl[b];
+            const uint32_t to = L.pos(j - 1), from = L.pos(j);
+            L.first[to] = L.first[from];
+            L.marks[to] = L.marks[from];
         }
         s.it_n--;
         if (s.cand > i)
             s.cand--;
     }
-    static __device__ uint32_t it_lower(const Tx &t, uint32_t seq)
+    // the first entry starting at or after seq
+    static __device__ uint32_t lg_find(const Ledger &L, uint32_t seq)
     {
-        uint32_t lo = 0, hi = t.s.it_n;
-        while (lo < hi)
+        uint32_t a = 0, b = L.s.it_n;
+        while (a < b)
         {
-            uint32_t m = (lo + hi) / 2;
-            if (t.beg(m) < seq)
-                lo = m + 1;
+            const uint32_t m = (a + b) / 2;
+            if (L.lo(m) < seq)
+                a = m + 1;
             else
-                hi = m;
+                b = m;
         }
-        return lo;
+        return a;
     }
-    static __device__ __forceinline__ uint32_t tb_sent(const SndState &s) { return s.it_n ? s.high - s.una : 0u; }
-    static __device__ __forceinline__ uint32_t inflight(const SndState &s) { return tb_sent(s) - (s.sack_b + s.lost_b) + s.rtx_b; }
-    static __device__ void tb_add_reno_sack(Tx &t)
+    static __device__ __forceinline__ uint32_t lg_sent(const SndState &s) { return s.it_n ? s.high - s.una : 0u; }
+    // bytes in flight (TcpTxBuffer::BytesInFlight without SACK blocks)
+    static __device__ __forceinline__ uint32_t inflight(const SndState &s) { return lg_sent(s) - (s.sack_b + s.lost_b) + s.rtx_b; }
+    // a duplicate ACK in recovery: the next unsacked entry after the ones already sacked (Reno's
+    // emulation of a SACK block)
+    static __device__ void lg_dup_sack(Ledger &L)
     {
-        SndState &s = t.s;
-        uint32_t i = 1 + s.sacked_n;
-        while (i < s.it_n && (t.flags(i) & IT_SACKED))
-            i++;
-        if (i < s.it_n)
-        {
-            it_set(t, i, t.flags(i) | IT_SACKED);
-            s.sack_seen = 1;
-            s.hisack = t.beg(i);
-        }
+        SndState &s = L.s;
+        uint32_t i = s.sacked_n + 1;
+        for (; i < s.it_n; i++)
+            if (!(L.mk(i) & IT_SACKED))
+            {
+                lg_remark(L, i, L.mk(i) | IT_SACKED);
+                s.sack_seen = 1;
+                s.hisack = L.lo(i);
+                return;
+            }
     }
-    static __device__ void tb_mark_head_lost(Tx &t)
+    // the oldest entry lost (and no longer sacked or retransmitted)
+    static __device__ void lg_head_lost(Ledger &L)
     {
-        if (t.s.it_n == 0)
-            return;
-        it_set(t, 0, (uint8_t)((t.flags(0) & ~(IT_SACKED | IT_RETX)) | IT_LOST));
+        if (L.s.it_n)
+            lg_remark(L, 0, (uint8_t)(IT_LOST | (L.mk(0) & ~(IT_SACKED | IT_RETX))));
     }
-    static __device__ void tb_all_lost(Tx &t)
+    // a retransmission timeout: every entry lost, nothing sacked or retransmitted
+    static __device__ void lg_all_lost(Ledger &L)
     {
-        SndState &s = t.s;
+        SndState &s = L.s;
         for (uint32_t i = 0; i < s.it_n; i++)
-            t.fl[t.at(i)] = IT_LOST;
-        s.sack_b = 0;
+            L.marks[L.pos(i)] = IT_LOST;
+        s.lost_b = lg_sent(s);
+        s.sack_b = s.rtx_b = 0;
         s.sacked_n = 0;
-        s.rtx_b = 0;
-        s.lost_b = tb_sent(s);
         s.cand_n = s.it_n;
         s.cand = 0;
         s.sack_seen = 0;
         s.hisack = 0;
     }
-    // DiscardUpTo(ack): the bytes of the items removed whole
-    static __device__ uint32_t tb_discard(Tx &t, uint32_t ack)
+    // the bytes up to ack are acknowledged (DiscardUpTo): entries wholly below it leave, one cut by
+    // it starts there; returns the bytes of the entries that left
+    static __device__ uint32_t lg_release(Ledger &L, uint32_t ack)
     {
-        SndState &s = t.s;
-        uint32_t delivered = 0, removed = 0;
-        while (s.it_n > 0)
+        SndState &s = L.s;
+        uint32_t freed = 0, gone = 0;
+        while (s.it_n)
         {
-            uint32_t k = t.at(0), b = t.st[k], e = t.end(0);
-            if (ack >= e)
+            const uint32_t k = L.pos(0), a = L.first[k], b = L.hi(0);
+            if (b > ack)
             {
-                if (t.fl[k])
-                    it_set(t, 0, 0);
-                s.it_head = t.at(1);
-                s.it_n--;
-                removed++;
-                delivered += e - b;
-            }
-            else
-            {
-                if (ack > b)
+                if (ack > a)
                 {
-                    uint8_t fl = t.fl[k];
-                    uint32_t off = ack - b;
-                    if (fl & IT_SACKED)
-                        s.sack_b -= off;
-                    if (fl & IT_LOST)
-                        s.lost_b -= off;
-                    if (fl & IT_RETX)
-                        s.rtx_b -= off;
-                    t.st[k] = ack;
+                    // the entry's acknowledged front leaves its mark totals
+                    const uint8_t m = L.marks[k];
+                    const uint32_t cut = ack - a;
+                    s.sack_b -= m & IT_SACKED ? cut : 0u;
+                    s.lost_b -= m & IT_LOST ? cut : 0u;
+                    s.rtx_b -= m & IT_RETX ? cut : 0u;
+                    L.first[k] = ack;
                 }
                 break;
             }
+            if (L.marks[k])
+                lg_remark(L, 0, 0);
+            s.it_head = L.pos(1);
+            s.it_n--;
+            gone++;
+            freed += b - a;
         }
-        if (removed)
-            s.cand = s.cand > removed ? s.cand - removed : 0;
+        if (gone)
+            s.cand = s.cand > gone ? s.cand - gone : 0;
         s.una = ack;
-        if (s.it_n > 0 && (t.flags(0) & IT_SACKED))
+        if (s.it_n && (L.mk(0) & IT_SACKED))
         {
-            it_set(t, 0, (uint8_t)(t.flags(0) & ~IT_SACKED));
-            tb_add_reno_sack(t);
-            tb_mark_head_lost(t);
+            lg_remark(L, 0, (uint8_t)(L.mk(0) & ~IT_SACKED));
+            lg_dup_sack(L);
+            lg_head_lost(L);
         }
         if (s.hisack <= ack)
-        {
-            s.sack_seen = 0;
-            s.hisack = 0;
-        }
-        return delivered;
+            s.sack_seen = 0, s.hisack = 0;
+        return freed;
     }
-    static __device__ bool tb_next_seg(Tx &t, uint32_t *seq)
+    // where the next segment starts: the first retransmission candidate below the highest sack,
+    // else new data; false: nothing to send
+    static __device__ bool lg_next(Ledger &L, uint32_t *seq)
     {
-        SndState &s = t.s;
-        if (s.cand_n > 0)
+        SndState &s = L.s;
+        for (uint32_t i = s.cand; s.cand_n && i < s.it_n; i++)
         {
-            for (uint32_t i = s.cand; i < s.it_n; i++)
-            {
-                if (!cand_of(t.flags(i)))
-                    continue;
-                s.cand = i;
-                uint32_t b = t.beg(i);
-                if (!s.sack_seen || b < s.hisack)
-                {
-                    *seq = b;
-                    return true;
-                }
+            if (!lg_candidate(L.mk(i)))
+                continue;
+            s.cand = i;
+            if (s.sack_seen && L.lo(i) >= s.hisack)
                 break;
-            }
-        }
-        if (s.tail > s.high)
-        {
-            *seq = s.high;
+            *seq = L.lo(i);
             return true;
         }
-        return false;
+        if (s.high >= s.tail)
+            return false;
+        *seq = s.high;
+        return true;
     }
-    static __device__ bool tb_head_lost(Tx &t)
+    // is the oldest entry lost below the highest sack (a fast retransmit due)?
+    static __device__ bool lg_head_is_lost(const Ledger &L)
     {
-        const SndState &s = t.s;
-        if (!s.sack_seen || s.una >= s.hisack || s.it_n == 0)
-            return false;
-        return t.flags(0) & IT_LOST;
+        const SndState &s = L.s;
+        return s.sack_seen && s.una < s.hisack && s.it_n && (L.mk(0) & IT_LOST);
     }
-    static __device__ bool tb_retx_ends_at(Tx &t, uint32_t x)
+    // does a retransmitted, unsacked entry end exactly at x?
+    static __device__ bool lg_retx_ends_at(const Ledger &L, uint32_t x)
     {
-        const SndState &s = t.s;
-        if (s.it_n == 0 || x <= s.una || x > s.high)
+        const SndState &s = L.s;
+        if (!s.it_n || x <= s.una || x > s.high)
             return false;
-        uint32_t lo = it_lower(t, x);
-        if (lo == 0)
+        const uint32_t n = lg_find(L, x);
+        if (!n || L.hi(n - 1) != x)
             return false;
-        uint32_t i = lo - 1;
-        if (t.end(i) != x)
-            return false;
-        uint8_t fl = t.flags(i);
-        return !(fl & IT_SACKED) && (fl & IT_RETX);
+        const uint8_t m = L.mk(n - 1);
+        return (m & IT_RETX) && !(m & IT_SACKED);
     }
-    // GetTransmittedSegment for item i
-    static __device__ uint32_t tb_retransmit(Tx &t, WorldRun *R, uint32_t i, uint32_t num)
+    // num bytes sent again from entry i (GetTransmittedSegment): it merges with the next entry
+    // when that one is unsacked and equally lost (their retransmitted marks settled first), or is
+    // split at num; the result is marked retransmitted. Returns the bytes.
+    static __device__ uint32_t lg_resend(Ledger &L, WorldRun *R, uint32_t i, uint32_t num)
     {
-        SndState &s = t.s;
-        uint32_t len = t.end(i) - t.beg(i), sz = num;
-        if (i + 1 < s.it_n)
-        {
-            uint8_t a = t.flags(i), b = t.flags(i + 1);
-            uint32_t nlen = t.end(i + 1) - t.beg(i + 1);
-            if (!(b & IT_SACKED) && (a & IT_LOST) == (b & IT_LOST))
-                sz = sz < len + nlen ? sz : len + nlen;
-            else
-                sz = sz < len ? sz : len;
-        }
-        else
-            sz = sz < len ? sz : len;
+        SndState &s = L.s;
+        uint32_t len = L.hi(i) - L.lo(i);
+        const bool has_next = i + 1 < s.it_n;
+        const uint8_t a = L.mk(i), b = has_next ? L.mk(i + 1) : 0;
+        const bool mergeable = has_next && !(b & IT_SACKED) && (a & IT_LOST) == (b & IT_LOST);
+        const uint32_t most = mergeable ? len + (L.hi(i + 1) - L.lo(i + 1)) : len;
+        const uint32_t sz = num < most ? num : most;
         if (sz > len)
         {
-            uint8_t a = t.flags(i), b = t.flags(i + 1);
+            // entry i + 1 joins entry i: only one of them may stay retransmitted-marked
+            uint8_t nb = b;
             if ((a ^ b) & IT_RETX)
             {
                 if (a & IT_RETX)
-                    it_set(t, i, a &= (uint8_t)~IT_RETX);
+                    lg_remark(L, i, (uint8_t)(a & ~IT_RETX));
                 else
-                    it_set(t, i + 1, b &= (uint8_t)~IT_RETX);
+                    lg_remark(L, i + 1, nb = (uint8_t)(b & ~IT_RETX));
             }
-            if (cand_of(b))
+            if (lg_candidate(nb))
                 s.cand_n--;
-            it_remove_start(t, i + 1);
-            len = t.end(i) - t.beg(i);
+            lg_erase(L, i + 1);
+            len = L.hi(i) - L.lo(i);
         }
         if (sz < len)
-            it_insert(t, R, i + 1, t.beg(i) + sz, t.flags(i));
-        uint8_t fl = t.flags(i);
-        if (!(fl & IT_RETX))
-            it_set(t, i, fl | IT_RETX);
+            lg_insert(L, R, i + 1, L.lo(i) + sz, L.mk(i));
+        if (!(L.mk(i) & IT_RETX))
+            lg_remark(L, i, L.mk(i) | IT_RETX);
         return sz;
     }
-    // CopyFromSequence(num, seq)
-    static __device__ uint32_t tb_copy(Tx &t, WorldRun *R, uint32_t num, uint32_t seq, bool *retx)
+    // the segment of up to num bytes at seq (CopyFromSequence): new data appends an entry, data
+    // sent before is resent from its entry (*retx); a segment across s.high sends only the part
+    // below it. 0: nothing (or the ledger cannot hold it)
+    static __device__ uint32_t lg_take(Ledger &L, WorldRun *R, uint32_t num, uint32_t seq, bool *retx)
     {
-        SndState &s = t.s;
+        SndState &s = L.s;
         for (;;)
         {
-            uint32_t avail = s.tail > seq ? s.tail - seq : 0;
-            uint32_t sz = num < avail ? num : avail;
-            if (sz == 0)
+            const uint32_t sz = s.tail <= seq ? 0u : num < s.tail - seq ? num : s.tail - seq;
+            if (!sz)
                 return 0;
-            uint32_t sent_end = s.high;
-            if (sent_end >= seq + sz)
+            if (s.high <= seq)
             {
-                uint32_t i = it_lower(t, seq);
-                if (i >= s.it_n || t.beg(i) != seq)
-                {
-                    count(R, EV_RTT_OVF);
-                    return 0;
-                }
-                *retx = true;
-                return tb_retransmit(t, R, i, sz);
-            }
-            if (sent_end <= seq)
-            {
-                if (!it_insert(t, R, s.it_n, seq, 0))
+                if (!lg_insert(L, R, s.it_n, seq, 0))
                     return 0;
                 s.high = seq + sz;
                 *retx = false;
                 return sz;
             }
-            num = sent_end - seq;
+            if (s.high >= seq + sz)
+            {
+                const uint32_t i = lg_find(L, seq);
+                if (i >= s.it_n || L.lo(i) != seq)
+                {
+                    count(R, EV_RTT_OVF);
+                    return 0;
+                }
+                *retx = true;
+                return lg_resend(L, R, i, sz);
+            }
+            num = s.high - seq;
         }
     }
 
-    // -------------------------------------------------------------- RTT (src/step.cuh:977-1126)
-    static __device__ __forceinline__ uint32_t hat(const SndState &s, uint32_t cap, uint32_t j)
+    // -------------------------------------------------------------- RTT
+    // Each transmission is noted (first byte, length; the length's top bit once any of it is
+    // resent, which removes it from sampling: Karn) in a ring beside the ledger; the oldest
+    // unretransmitted record an ACK covers gives a sample (ns-3's RttMeanDeviation, in integer
+    // picoseconds with its shifts), and the records it covers leave.
+    static __device__ __forceinline__ uint32_t rtt_pos(const SndState &s, uint32_t cap, uint32_t j)
     {
-        uint32_t v = s.h_head + j;
-        return v >= cap ? v - cap : v;
+        return s.h_head + j < cap ? s.h_head + j : s.h_head + j - cap;
     }
-    static __device__ void rtt_history(Host &x, int f, uint32_t seq, uint32_t sz, bool retx, int64_t now)
+    static __device__ void rtt_note(Host &x, int f, uint32_t seq, uint32_t sz, bool retx, int64_t now)
     {
         const Batch &B = *x.B;
-        WorldRun *R = x.R;
         SndState &s = *x.ls;
         const ConnStatic &c = B.cs[f];
         uint32_t *hs = B.h_seq + c.it_off, *hc = B.h_cnt + c.it_off;
-        if (!retx)
+        if (retx)
         {
-            if (s.h_n >= c.it_cap)
+            // the record holding seq: resent up to seq + sz
+            for (uint32_t j = 0; j < s.h_n; j++)
             {
-                count(R, EV_RTT_OVF);
-                return;
+                const uint32_t k = rtt_pos(s, c.it_cap, j);
+                if (seq >= hs[k] && seq - hs[k] < (hc[k] & 0x7FFFFFFFu))
+                {
+                    hc[k] = 0x80000000u | (seq + sz - hs[k]);
+                    return;
+                }
             }
-            uint32_t k = hat(s, c.it_cap, s.h_n);
-            hs[k] = seq;
-            hc[k] = sz;
-            B.h_t[c.it_off + k] = now;
-            s.h_n++;
             return;
         }
-        for (uint32_t j = 0; j < s.h_n; j++)
+        if (s.h_n >= c.it_cap)
         {
-            uint32_t k = hat(s, c.it_cap, j), n = hc[k] & 0x7FFFFFFFu;
-            if (seq >= hs[k] && seq < hs[k] + n)
-            {
-                hc[k] = 0x80000000u | (seq + sz - hs[k]);
-                return;
-            }
+            count(x.R, EV_RTT_OVF);
+            return;
         }
+        const uint32_t k = rtt_pos(s, c.it_cap, s.h_n++);
+        hs[k] = seq;
+        hc[k] = sz;
+        B.h_t[c.it_off + k] = now;
     }
+    // a measurement m: the smoothed RTT and its variation, then the timeout (at least MinRto)
     static __device__ void rtt_sample(SndState &s, int64_t m)
     {
-        int64_t est = s.srtt, var = s.rttvar;
-        if (est < 0)
+        if (s.srtt < 0)
         {
-            est = m;
-            var = m / 2;
+            s.srtt = m;
+            s.rttvar = m / 2;
         }
         else
         {
-            int64_t delta = m - est;
-            est = ((est << 3) + delta) >> 3;
-            if (delta < 0)
-                delta = -delta;
-            var = ((var << 2) + delta - var) >> 2;
+            const int64_t err = m - s.srtt, mag = err < 0 ? -err : err;
+            s.srtt = ((s.srtt << 3) + err) >> 3;
+            s.rttvar = ((s.rttvar << 2) - s.rttvar + mag) >> 2;
         }
-        s.srtt = est;
-        s.rttvar = var;
-        int64_t v = 4 * var;
-        int64_t r = est + (v > CLOCK_GRAN_PS ? v : CLOCK_GRAN_PS);
-        s.rto = r > MIN_RTO_PS ? r : MIN_RTO_PS;
+        const int64_t spread = 4 * s.rttvar > CLOCK_GRAN_PS ? 4 * s.rttvar : CLOCK_GRAN_PS;
+        s.rto = s.srtt + spread > MIN_RTO_PS ? s.srtt + spread : MIN_RTO_PS;
     }
-    static __device__ void rtt_ack(Host &x, int f, uint32_t ack, int64_t now)
+    static __device__ void rtt_on_ack(Host &x, int f, uint32_t ack, int64_t now)
     {
         const Batch &B = *x.B;
         SndState &s = *x.ls;
-        if (s.h_n == 0)
+        if (!s.h_n)
             return;
         const ConnStatic &c = B.cs[f];
         const uint32_t *hs = B.h_seq + c.it_off, *hc = B.h_cnt + c.it_off;
-        uint32_t k = hat(s, c.it_cap, 0);
-        if (!(hc[k] & 0x80000000u) && ack >= hs[k] + hc[k])
-            rtt_sample(s, now - B.h_t[c.it_off + k]);
-        while (s.h_n > 0)
+        const uint32_t k0 = rtt_pos(s, c.it_cap, 0);
+        if (!(hc[k0] & 0x80000000u) && ack >= hs[k0] + hc[k0])
+            rtt_sample(s, now - B.h_t[c.it_off + k0]);
+        for (; s.h_n; s.h_n--)
         {
-            k = hat(s, c.it_cap, 0);
+            const uint32_t k = rtt_pos(s, c.it_cap, 0);
             if (hs[k] + (hc[k] & 0x7FFFFFFFu) > ack)
                 break;
-            s.h_head = hat(s, c.it_cap, 1);
-            s.h_n--;
+            s.h_head = rtt_pos(s, c.it_cap, 1);
         }
     }
 
     // -------------------------------------------------------------- congestion control
-    // TcpCubic (src/step.cuh:1215-1360)
-    static __device__ __forceinline__ uint32_t cub_u32(double d) { return (uint32_t)(uint64_t)(int64_t)d; }
-    static __device__ void cub_hystart_reset(SndState &s, int64_t now)
+    // Three controllers over the sender's record, each with ns-3's hooks: the slow-start threshold
+    // after a loss or an echo (GetSsThresh), the per-ACK measurement (PktsAcked) and the window
+    // growth (IncreaseWindow). Cubic follows ns-3's TcpCubic (with HyStart), DCTCP its alpha EWMA
+    // per window of data, NewReno slow start and congestion avoidance.
+    static __device__ __forceinline__ uint32_t to_u32(double d) { return (uint32_t)(uint64_t)(int64_t)d; }
+    static __device__ void cubic_round_start(SndState &s, int64_t now)
     {
-        s.cub_roundstart = s.cub_lastack = now;
+        s.cub_roundstart = now;
+        s.cub_lastack = now;
         s.cub_endseq = s.st == S_EST ? s.high + 1 : 0;
         s.cub_currrtt = INT64_MIN;
         s.cub_samples = 0;
     }
-    static __device__ __forceinline__ uint32_t cub_update(SndState &s, uint32_t segs, int64_t now)
+    // ACKs per window increment in congestion avoidance (TcpCubic::Update): the cubic curve's
+    // target at now + delay_min against the window, bounded by the TCP-friendly estimate
+    static __device__ uint32_t cubic_acks_per_inc(SndState &s, uint32_t segs, int64_t now)
     {
         const double C = 0.4, BETA = 0.7;
-        uint32_t seg_cwnd = s.cwnd / MSS;
+        const uint32_t w = s.cwnd / MSS;
         s.cub_ackcnt += segs;
         if (s.cub_epoch == INT64_MIN)
         {
+            // a new epoch: the curve's origin and the time K to reach it
             s.cub_epoch = now;
             s.cub_ackcnt = segs;
-            s.cub_tcpcwnd = seg_cwnd;
-            if (s.cub_lastmax <= seg_cwnd)
-            {
-                s.cub_k = 0.0;
-                s.cub_origin = seg_cwnd;
-            }
-            else
-            {
-                s.cub_k = tw_pow((double)(s.cub_lastmax - seg_cwnd) / C, 1 / 3.);
-                s.cub_origin = s.cub_lastmax;
-            }
+            s.cub_tcpcwnd = w;
+            const bool below = s.cub_lastmax > w;
+            s.cub_origin = below ? s.cub_lastmax : w;
+            s.cub_k = below ? tw_pow((double)(s.cub_lastmax - w) / C, 1 / 3.) : 0.0;
         }
-        int64_t t = (int64_t)((uint64_t)now + (uint64_t)s.cub_delaymin - (uint64_t)s.cub_epoch);
-        double ts = tw_seconds(t), k = s.cub_k;
-        double offs = ts < k ? k - ts : ts - k;
-        uint32_t delta = cub_u32(C * tw_pow(offs, 3));
-        uint32_t target = ts < k ? s.cub_origin - delta : s.cub_origin + delta;
-        uint32_t cnt = target > seg_cwnd ? seg_cwnd / (target - seg_cwnd) : 100 * seg_cwnd;
-        if (s.cub_lastmax == 0 && cnt > 20)
-            cnt = 20;
-        uint32_t scale = cub_u32(8 * (1024 + BETA * 1024) / 3 / (1024 - BETA * 1024));
-        delta = (seg_cwnd * scale) >> 3;
-        while (delta > 0 && s.cub_ackcnt > delta) // (delta >= 1 for any window of a segment or more)
-        {
-            s.cub_ackcnt -= delta;
+        const int64_t dt = (int64_t)((uint64_t)now + (uint64_t)s.cub_delaymin - (uint64_t)s.cub_epoch);
+        const double t = tw_seconds(dt), k = s.cub_k;
+        const uint32_t off = to_u32(C * tw_pow(t < k ? k - t : t - k, 3));
+        const uint32_t target = t < k ? s.cub_origin - off : s.cub_origin + off;
+        uint32_t n = target > w ? w / (target - w) : 100 * w;
+        if (!s.cub_lastmax && n > 20)
+            n = 20;
+        // the TCP-friendly window: one segment per `per` ACKs
+        const uint32_t per = (w * to_u32(8 * (1024 + BETA * 1024) / 3 / (1024 - BETA * 1024))) >> 3;
+        for (; per > 0 && s.cub_ackcnt > per; s.cub_ackcnt -= per)
             s.cub_tcpcwnd++;
-        }
-        if (s.cub_tcpcwnd > seg_cwnd)
+        if (s.cub_tcpcwnd > w)
         {
-            uint32_t max_cnt = seg_cwnd / (s.cub_tcpcwnd - seg_cwnd);
-            if (cnt > max_cnt)
-                cnt = max_cnt;
+            const uint32_t cap = w / (s.cub_tcpcwnd - w);
+            n = n > cap ? cap : n;
         }
-        return cnt > 2 ? cnt : 2;
+        return n < 2 ? 2 : n;
     }
-    static __device__ void cub_pkts_acked(SndState &s, int64_t now)
+    // HyStart's delay check and the minimum delay (TcpCubic::PktsAcked)
+    static __device__ void cubic_measure(SndState &s, int64_t now)
     {
         const int64_t ACK_DELTA = 2000000000, DELAY_MIN = 4000000000, DELAY_MAX = 1000000000000, CUBIC_DELTA = 10000000000;
         if (s.cub_epoch != INT64_MIN && now - s.cub_epoch < CUBIC_DELTA)
             return;
-        int64_t rtt = s.srtt;
-        if (s.cub_delaymin == INT64_MIN || s.cub_delaymin > rtt)
+        const int64_t rtt = s.srtt;
+        if (s.cub_delaymin == INT64_MIN || rtt < s.cub_delaymin)
             s.cub_delaymin = rtt;
-        uint32_t cwnd = s.cwnd;
-        if (!(cwnd <= s.ssthresh && cwnd >= 16 * MSS) || s.cub_found)
+        const bool slow_start_window = s.cwnd <= s.ssthresh && s.cwnd >= 16 * MSS;
+        if (!slow_start_window || s.cub_found)
             return;
         if (now - s.cub_lastack <= ACK_DELTA)
         {
+            // the ACK train
             s.cub_lastack = now;
             if (now - s.cub_roundstart > s.cub_delaymin)
                 s.cub_found = 1;
         }
-        if (s.cub_samples < 8)
+        if (s.cub_samples >= 8)
         {
-            if (s.cub_currrtt == INT64_MIN || s.cub_currrtt > rtt)
-                s.cub_currrtt = rtt;
-            s.cub_samples++;
+            // the delay increase
+            const int64_t dm = s.cub_delaymin, eta = dm < DELAY_MIN ? DELAY_MIN : dm > DELAY_MAX ? DELAY_MAX : dm;
+            if (s.cub_currrtt > dm + eta)
+                s.cub_found = 1;
         }
         else
         {
-            int64_t dm = s.cub_delaymin;
-            int64_t th = dm > DELAY_MAX ? DELAY_MAX : (dm < DELAY_MIN ? DELAY_MIN : dm);
-            if (s.cub_currrtt > dm + th)
-                s.cub_found = 1;
+            if (s.cub_currrtt == INT64_MIN || rtt < s.cub_currrtt)
+                s.cub_currrtt = rtt;
+            s.cub_samples++;
         }
         if (s.cub_found)
-            s.ssthresh = cwnd;
+            s.ssthresh = s.cwnd;
     }
-    static __device__ void cub_increase(SndState &s, uint32_t segs, uint32_t ack, int64_t now)
+    static __device__ void cubic_grow(SndState &s, uint32_t segs, uint32_t ack, int64_t now)
     {
         if (!s.cwnd_limited)
             return;
         if (s.cwnd < s.ssthresh)
         {
             if (ack + 1 > s.cub_endseq)
-                cub_hystart_reset(s, now);
+                cubic_round_start(s, now);
             s.cwnd += segs * MSS;
             segs = 0;
         }
-        if (s.cwnd >= s.ssthresh && segs > 0)
+        if (segs == 0 || s.cwnd < s.ssthresh)
+            return;
+        s.cub_cnt += segs;
+        const uint32_t n = cubic_acks_per_inc(s, segs, now);
+        if (s.cub_cnt >= n)
         {
-            s.cub_cnt += segs;
-            uint32_t cnt = cub_update(s, segs, now);
-            if (s.cub_cnt >= cnt)
-            {
-                s.cwnd += MSS;
-                s.cub_cnt -= cnt;
-            }
+            s.cub_cnt -= n;
+            s.cwnd += MSS;
         }
     }
-    static __device__ uint32_t cub_ssthresh(SndState &s)
+    // the threshold after a loss: beta times the window (fast convergence lowers the remembered
+    // maximum when the window is below it); a new epoch starts
+    static __device__ uint32_t cubic_ssthresh(SndState &s)
     {
         const double BETA = 0.7;
-        uint32_t seg_cwnd = s.cwnd / MSS;
-        if (seg_cwnd < s.cub_lastmax)
-            s.cub_lastmax = cub_u32((seg_cwnd * (1 + BETA)) / 2);
-        else
-            s.cub_lastmax = seg_cwnd;
+        const uint32_t w = s.cwnd / MSS;
+        s.cub_lastmax = w < s.cub_lastmax ? to_u32((w * (1 + BETA)) / 2) : w;
         s.cub_epoch = INT64_MIN;
-        uint32_t th = cub_u32(seg_cwnd * BETA);
-        return (th > 2 ? th : 2) * MSS;
+        const uint32_t th = to_u32(w * BETA);
+        return (th < 2 ? 2 : th) * MSS;
     }
-    static __device__ void cub_loss(SndState &s, int64_t now)
+    static __device__ void cubic_timeout(SndState &s, int64_t now)
     {
         s.cub_origin = 0;
         s.cub_k = 0;
@@ -929,118 +906,116 @@ struct Model
         s.cub_tcpcwnd = 0;
         s.cub_delaymin = INT64_MIN;
         s.cub_found = 0;
-        cub_hystart_reset(s, now);
+        cubic_round_start(s, now);
     }
-    // GetSsThresh (src/step.cuh:1057-1066)
-    static __device__ uint32_t ssthresh_for(Host &x, SndState &s, uint32_t bif)
+    // DCTCP's alpha: the marked share of the bytes acknowledged over each window of data
+    static __device__ void dctcp_measure(SndState &s, uint32_t segs, uint32_t ack)
     {
-        if (x.cubic)
-            return cub_ssthresh(s);
-        if (x.dctcp)
-            return (uint32_t)((1.0 - s.alpha / 2.0) * s.cwnd);
-        uint32_t half = bif / 2;
-        return half > 2 * MSS ? half : 2 * MSS;
-    }
-    // PktsAcked (src/step.cuh:1362-1393)
-    static __device__ void pkts_acked(Host &x, SndState &s, uint32_t segs, uint32_t ack, int64_t now)
-    {
-        if (x.cubic)
-        {
-            cub_pkts_acked(s, now);
-            return;
-        }
-        if (!x.dctcp)
-            return;
-        s.acked_total += segs * MSS;
+        const uint32_t bytes = segs * MSS;
+        s.acked_total += bytes;
         if (s.ecn_s == E_ECE_RCVD)
-            s.acked_ecn += segs * MSS;
+            s.acked_ecn += bytes;
         if (!s.next_seq_flag)
         {
-            s.next_seq = s.nxt;
             s.next_seq_flag = 1;
-        }
-        if (ack >= s.next_seq)
-        {
-            double m = s.acked_total > 0 ? (double)s.acked_ecn / s.acked_total : 0.0;
-            s.alpha = __builtin_fma(1.0 - DCTCP_G, s.alpha, DCTCP_G * m);
             s.next_seq = s.nxt;
-            s.acked_total = 0;
-            s.acked_ecn = 0;
+        }
+        if (ack < s.next_seq)
+            return;
+        const double share = s.acked_total ? (double)s.acked_ecn / s.acked_total : 0.0;
+        s.alpha = __builtin_fma(1.0 - DCTCP_G, s.alpha, DCTCP_G * share);
+        s.next_seq = s.nxt;
+        s.acked_total = s.acked_ecn = 0;
+    }
+    // DCTCP's growth: slow start up to the threshold, then a segment per window's worth of ACKs
+    static __device__ void dctcp_grow(SndState &s, uint32_t segs)
+    {
+        if (s.cwnd < s.ssthresh)
+        {
+            if (segs)
+            {
+                const uint64_t w = (uint64_t)s.cwnd + (uint64_t)segs * MSS;
+                s.cwnd = (uint32_t)(w < s.ssthresh ? w : s.ssthresh);
+            }
+            return;
+        }
+        const uint32_t per = s.cwnd / MSS ? s.cwnd / MSS : 1;
+        uint32_t acc = s.cwnd_cnt;
+        if (acc >= per)
+        {
+            acc = 0;
+            s.cwnd += MSS;
+        }
+        acc += segs;
+        if (acc >= per)
+        {
+            s.cwnd += acc / per * MSS;
+            acc %= per;
+        }
+        s.cwnd_cnt = acc;
+    }
+    // NewReno: one segment per ACK in slow start, then about MSS^2 / cwnd bytes (at least 1)
+    static __device__ void newreno_grow(SndState &s, uint32_t segs)
+    {
+        if (s.cwnd < s.ssthresh && segs)
+        {
+            s.cwnd += MSS;
+            segs--;
+        }
+        if (s.cwnd >= s.ssthresh && segs)
+        {
+            const double add = (double)MSS * MSS / s.cwnd;
+            s.cwnd += (uint32_t)(add < 1.0 ? 1.0 : add);
         }
     }
-    // IncreaseWindow (src/step.cuh:1395-1455)
-    static __device__ void increase_window(Host &x, SndState &s, uint32_t segs, uint32_t ack, int64_t now)
+    static __device__ uint32_t cc_ssthresh(Host &x, SndState &s, uint32_t in_flight)
     {
         if (x.cubic)
-        {
-            cub_increase(s, segs, ack, now);
-            return;
-        }
-        uint32_t cwnd = s.cwnd, ssth = s.ssthresh;
+            return cubic_ssthresh(s);
         if (x.dctcp)
-        {
-            if (cwnd < ssth)
-            {
-                if (segs >= 1)
-                {
-                    uint64_t n = (uint64_t)cwnd + (uint64_t)segs * MSS;
-                    s.cwnd = (uint32_t)(n < ssth ? n : ssth);
-                }
-            }
-            else
-            {
-                uint32_t w = cwnd / MSS;
-                if (w == 0)
-                    w = 1;
-                uint32_t cnt = s.cwnd_cnt;
-                if (cnt >= w)
-                {
-                    cnt = 0;
-                    cwnd += MSS;
-                }
-                cnt += segs;
-                if (cnt >= w)
-                {
-                    uint32_t delta = cnt / w;
-                    cnt -= delta * w;
-                    cwnd += delta * MSS;
-                }
-                s.cwnd = cwnd;
-                s.cwnd_cnt = cnt;
-            }
-            return;
-        }
-        if (cwnd < ssth && segs >= 1)
-        {
-            cwnd += MSS;
-            segs -= 1;
-        }
-        if (cwnd >= ssth && segs > 0)
-        {
-            double adder = (double)MSS * MSS / cwnd;
-            if (adder < 1.0)
-                adder = 1.0;
-            cwnd += (uint32_t)adder;
-        }
-        s.cwnd = cwnd;
+            return (uint32_t)((1.0 - s.alpha / 2.0) * s.cwnd);
+        return in_flight / 2 > 2 * MSS ? in_flight / 2 : 2 * MSS;
+    }
+    static __device__ void cc_acked(Host &x, SndState &s, uint32_t segs, uint32_t ack, int64_t now)
+    {
+        if (x.cubic)
+            cubic_measure(s, now);
+        else if (x.dctcp)
+            dctcp_measure(s, segs, ack);
+    }
+    static __device__ void cc_grow(Host &x, SndState &s, uint32_t segs, uint32_t ack, int64_t now)
+    {
+        if (x.cubic)
+            cubic_grow(s, segs, ack, now);
+        else if (x.dctcp)
+            dctcp_grow(s, segs);
+        else
+            newreno_grow(s, segs);
     }
 
-    // -------------------------------------------------------------- sending (src/step.cuh:1128-1213)
-    static __device__ uint32_t send_segment(Host &x, int f, uint32_t seq, uint32_t max_size, int64_t now)
+    // -------------------------------------------------------------- the sending endpoint
+    // A connection's sending side (TcpSocketBase with its TxBuffer, as the model restates it):
+    // segments leave through the host's queue, the retransmission timer follows the oldest
+    // unacknowledged byte, and ACKs drive ns-3's congestion states (open, disorder, CWR,
+    // recovery, loss). Every function works on the record of the event's connection in LDS.
+    // one segment of up to max_size bytes at seq (SendDataPacket): CWR on the first new segment
+    // after an echo, FIN on the last byte once the application closed; the timer is armed if
+    // idle; returns its bytes
+    static __device__ uint32_t tx_segment(Host &x, int f, uint32_t seq, uint32_t max_size, int64_t now)
     {
         const Batch &B = *x.B;
         SndState &s = *x.ls;
-        Tx t(B, *x.ls, f);
+        Ledger L(B, s, f);
         bool retx = false;
-        uint32_t len = tb_copy(t, x.R, max_size, seq, &retx);
-        if (len == 0)
+        const uint32_t len = lg_take(L, x.R, max_size, seq, &retx);
+        if (!len)
             return 0;
         uint8_t flags = F_DATA;
         if (!retx && s.ecn_s == E_ECE_RCVD && s.ecn_echo_seq > s.ecn_cwr_seq)
         {
+            flags |= F_CWR;
             s.ecn_s = E_CWR_SENT;
             s.ecn_cwr_seq = seq;
-            flags |= F_CWR;
         }
         if ((s.fin & FIN_CLOSE) && seq + len == s.tail)
         {
@@ -1059,91 +1034,94 @@ struct Model
             timer_chain(x, B.rchain, f, s.rto);
         }
         s.cwnd_limited = s.cwnd < inflight(s) + MSS;
-        rtt_history(x, f, seq, len, retx, now);
-        if (seq + len > s.high)
-            s.high = seq + len;
+        rtt_note(x, f, seq, len, retx, now);
+        s.high = seq + len > s.high ? seq + len : s.high;
         atomicMax(&x.R->ev[EV_MAX_INFLIGHT], (unsigned long long)s.it_n);
         return len;
     }
-    static __device__ void send_pending(Host &x, int f, int64_t now)
+    // sends while the window has room (SendPendingData): whole segments only, unless the rest of
+    // the data is smaller; nothing more once a FIN is out in the open state
+    static __device__ void tx_fill(Host &x, int f, int64_t now)
     {
-        const Batch &B = *x.B;
         SndState &s = *x.ls;
         if (s.st != S_EST)
             return;
-        Tx t(B, *x.ls, f);
-        for (int guard = 0; guard < 4096; guard++)
+        Ledger L(*x.B, s, f);
+        for (int n = 0; n < 4096; n++)
         {
-            if (s.cong == C_OPEN && (s.fin & (FIN_SENT | FIN_ACKED)) == FIN_SENT)
+            const bool fin_out = (s.fin & (FIN_SENT | FIN_ACKED)) == FIN_SENT;
+            if (s.cong == C_OPEN && fin_out)
                 return;
-            uint32_t pipe = inflight(s), cwnd = s.cwnd;
-            uint32_t avail = cwnd > pipe ? cwnd - pipe : 0;
-            if (avail == 0)
-                return;
+            const uint32_t pipe = inflight(s), room = s.cwnd > pipe ? s.cwnd - pipe : 0;
             uint32_t seq;
-            if (!tb_next_seg(t, &seq))
+            if (!room || !lg_next(L, &seq))
                 return;
-            uint32_t remaining = s.tail - seq;
-            if (avail < MSS && remaining > avail)
-                return;
+            if (room < MSS && s.tail - seq > room)
+                return; // (no runt segment while more data waits)
             s.nxt = seq;
-            uint32_t len = send_segment(x, f, seq, avail < MSS ? avail : MSS, now);
-            if (len == 0)
+            const uint32_t len = tx_segment(x, f, seq, room < MSS ? room : MSS, now);
+            if (!len)
                 return;
             s.nxt = seq + len;
         }
     }
-    static __device__ void do_retransmit(Host &x, int f, int64_t now)
+    // one segment again from the first candidate (else the oldest byte)
+    static __device__ void tx_resend_one(Host &x, int f, int64_t now)
     {
         SndState &s = *x.ls;
-        Tx t(*x.B, *x.ls, f);
+        Ledger L(*x.B, s, f);
         uint32_t seq;
-        if (!tb_next_seg(t, &seq))
+        if (!lg_next(L, &seq))
             seq = s.una;
         if (s.una >= s.tail)
             return;
         s.nxt = seq;
-        send_segment(x, f, seq, MSS, now);
+        tx_segment(x, f, seq, MSS, now);
     }
-    // NewAck (src/step.cuh:1457-1479)
-    static __device__ void new_ack(Host &x, int f, uint32_t ack, bool reset_rto, int64_t now)
+    // the retransmission timer after new data is acknowledged (NewAck): restarted from the RTT
+    // estimate when asked, stopped once all data (and no FIN) is acknowledged
+    static __device__ void rto_after_ack(Host &x, int f, uint32_t ack, bool restart, int64_t now)
     {
         SndState &s = *x.ls;
-        if (reset_rto)
+        if (restart)
         {
-            int64_t est = s.srtt >= 0 ? s.srtt : INIT_RTT_PS, v = 4 * s.rttvar;
-            int64_t r = est + (v > CLOCK_GRAN_PS ? v : CLOCK_GRAN_PS);
-            s.rto = r > MIN_RTO_PS ? r : MIN_RTO_PS;
+            const int64_t base = s.srtt >= 0 ? s.srtt : INIT_RTT_PS, spread = 4 * s.rttvar > CLOCK_GRAN_PS ? 4 * s.rttvar : CLOCK_GRAN_PS;
+            s.rto = base + spread > MIN_RTO_PS ? base + spread : MIN_RTO_PS;
             s.rto_at = now + s.rto;
             timer_chain(x, x.B->rchain, f, s.rto);
         }
         if (ack >= s.tail && !(s.fin & FIN_SENT))
             s.rto_at = -1;
-        if (ack > s.nxt)
-            s.nxt = ack;
+        s.nxt = ack > s.nxt ? ack : s.nxt;
     }
-    static __device__ __forceinline__ void enter_recovery(Host &x, int f, int64_t now)
+    // fast retransmit: the oldest segment lost, the window halved (or as the controller says),
+    // the segment sent again
+    static __device__ void fast_retransmit(Host &x, int f, int64_t now)
     {
         SndState &s = *x.ls;
-        Tx t(*x.B, *x.ls, f);
-        tb_add_reno_sack(t);
-        tb_mark_head_lost(t);
+        Ledger L(*x.B, s, f);
+        lg_dup_sack(L);
+        lg_head_lost(L);
+        s.cong = C_RECOVERY;
         s.recover = s.high;
         s.recover_active = 1;
-        s.cong = C_RECOVERY;
-        s.ssthresh = ssthresh_for(x, s, inflight(s) + MSS);
+        s.ssthresh = cc_ssthresh(x, s, inflight(s) + MSS);
         s.cwnd = s.ssthresh;
-        send_segment(x, f, s.una, MSS, now);
+        tx_segment(x, f, s.una, MSS, now);
     }
-    static __device__ void messages_acked(Host &x, int f, int64_t now)
+    // messages wholly acknowledged complete (their completion time is the flow's); the connection
+    // is done with its last message (and its FIN, if it sends one)
+    static __device__ void msgs_complete(Host &x, int f, int64_t now)
     {
         const Batch &B = *x.B;
         SndState &s = *x.ls;
         const ConnStatic &c = B.cs[f];
-        while (s.m_dn < s.m_wr && s.una >= B.mg_end[c.m_first + s.m_dn])
+        for (; s.m_dn < s.m_wr; s.m_dn++)
         {
-            B.mg_done[c.m_first + s.m_dn] = now;
-            s.m_dn++;
+            const int m = c.m_first + s.m_dn;
+            if (s.una < B.mg_end[m])
+                break;
+            B.mg_done[m] = now;
             atomicAdd(&x.R->flows_done, 1u);
         }
         if (s.m_dn == c.m_cnt && (!c.fin_on || (s.fin & FIN_ACKED)))
@@ -1152,160 +1130,169 @@ struct Model
             s.rto_at = -1;
         }
     }
-    // ReceivedAck + ProcessAck (src/step.cuh:1515-1692)
-    static __device__ void on_ack(Host &x, int f, uint32_t ack, bool ece, int64_t now)
+    // DCTCP's response to an echo (EnterCwr): a new echo beyond the last one cuts the window once
+    static __device__ void ack_echo(Host &x, SndState &s, uint32_t ack, bool ece, bool advancing)
+    {
+        if (!(advancing && x.dctcp && ece))
+        {
+            if (!ece && s.ecn_s == E_ECE_RCVD)
+                s.ecn_s = E_IDLE;
+            return;
+        }
+        if ((int64_t)ack <= s.ecn_echo_seq)
+            return;
+        s.ecn_echo_seq = ack;
+        s.ecn_s = E_ECE_RCVD;
+        if (s.cong == C_CWR)
+            return;
+        s.ssthresh = cc_ssthresh(x, s, inflight(s));
+        s.cwnd = s.ssthresh;
+        s.cong = C_CWR;
+        s.recover = s.high;
+    }
+    // a duplicate ACK (below s.high): counted outside recovery and loss; in recovery it sacks
+    // the next entry; the third (or a lost head below the highest sack) starts fast retransmit
+    static __device__ void ack_duplicate(Host &x, int f, SndState &s, uint32_t una_before, int64_t now)
+    {
+        uint8_t state = s.cong;
+        if (state == C_LOSS)
+            return;
+        if (state != C_RECOVERY)
+            s.dupacks++;
+        if (state == C_OPEN)
+            s.cong = state = C_DISORDER;
+        Ledger L(*x.B, s, f);
+        if (state == C_RECOVERY)
+            lg_dup_sack(L);
+        else if (state == C_DISORDER)
+        {
+            const bool third = s.dupacks == (uint32_t)DUPTHRESH && (!s.recover_active || una_before >= s.recover);
+            if (third || lg_head_is_lost(L))
+                fast_retransmit(x, f, now);
+        }
+    }
+    // an ACK of new data below the recovery point: partial (recovery: the next hole is resent,
+    // the timer restarted on the first partial ACK only) or during loss (slow start again)
+    static __device__ void ack_partial(Host &x, int f, SndState &s, uint32_t ack, uint32_t segs, int64_t now)
+    {
+        if (s.cong == C_RECOVERY)
+        {
+            Ledger L(*x.B, s, f);
+            lg_head_lost(L);
+            if (!lg_retx_ends_at(L, ack + MSS))
+                tx_resend_one(x, f, now);
+            cc_acked(x, s, 1, ack, now);
+            rto_after_ack(x, f, ack, s.first_partial, now);
+            s.first_partial = 0;
+            return;
+        }
+        cc_acked(x, s, segs, ack, now);
+        cc_grow(x, s, segs, ack, now);
+        rto_after_ack(x, f, ack, true, now);
+    }
+    // an ACK of new data otherwise: disorder and the ends of recovery and loss return to the
+    // open state (the window settles at the threshold after recovery); open grows the window
+    static __device__ void ack_advance(Host &x, int f, SndState &s, uint32_t ack, uint32_t una_before, uint32_t segs, uint32_t dups, int64_t now)
+    {
+        const uint8_t state = s.cong;
+        if (state == C_CWR)
+        {
+            cc_acked(x, s, segs, ack, now);
+            rto_after_ack(x, f, ack, true, now);
+            return;
+        }
+        switch (state)
+        {
+        case C_OPEN:
+            cc_acked(x, s, segs, ack, now);
+            break;
+        case C_DISORDER:
+            if (segs >= dups)
+                cc_acked(x, s, segs - dups, ack, now);
+            break;
+        case C_RECOVERY:
+            segs = (ack - una_before) / MSS; // (the segments of the whole recovery)
+            cc_acked(x, s, segs, ack, now);
+            s.first_partial = 1;
+            s.dupacks = 0;
+            break;
+        default: // C_LOSS
+            segs = (ack - s.recover) / MSS;
+            cc_acked(x, s, segs, ack, now);
+            s.first_partial = 1;
+            break;
+        }
+        s.cong = C_OPEN;
+        if (ack >= s.recover)
+            s.recover_active = 0;
+        if (state == C_RECOVERY)
+        {
+            rto_after_ack(x, f, ack, true, now);
+            s.cwnd = s.ssthresh;
+        }
+        cc_grow(x, s, segs, ack, now);
+        rto_after_ack(x, f, ack, true, now);
+    }
+    // an ACK at the sender (ReceivedAck + ProcessAck, no SACK option, classic recovery): old ones
+    // are ignored; the RTT sample and the ledger come first, then the CWR end and the echo, then
+    // by kind - a duplicate, an ACK of nothing new at s.high, one beyond it, or new data - and at
+    // last the FIN's acknowledgment, the completed messages and more sending
+    static __device__ void ack_in(Host &x, int f, uint32_t ack, bool ece, int64_t now)
     {
         SndState &s = *x.ls;
-        if (s.st != S_EST)
+        if (s.st != S_EST || ack < s.una)
             return;
-        uint32_t old = s.una;
-        if (ack < old)
-            return;
-        Tx t(*x.B, *x.ls, f);
-        rtt_ack(x, f, ack, now);
-        bool advancing = ack > old;
-        uint32_t delivered = 0;
-        if (advancing)
-            delivered = tb_discard(t, ack);
+        const uint32_t una_before = s.una;
+        const bool advancing = ack > una_before;
+        rtt_on_ack(x, f, ack, now);
+        Ledger L(*x.B, s, f);
+        const uint32_t acked = advancing ? lg_release(L, ack) : 0;
         if (s.cong == C_CWR && ack > s.recover)
         {
             s.cong = C_OPEN;
             s.cwnd = s.ssthresh;
         }
-        if (advancing && x.dctcp && ece)
+        ack_echo(x, s, ack, ece, advancing);
+        const uint32_t dups = s.dupacks;
+        if (!advancing)
         {
-            if (s.ecn_echo_seq < (int64_t)ack)
+            if (ack < s.high)
             {
-                s.ecn_echo_seq = ack;
-                s.ecn_s = E_ECE_RCVD;
-                if (s.cong != C_CWR)
-                {
-                    s.ssthresh = ssthresh_for(x, s, inflight(s));
-                    s.cwnd = s.ssthresh;
-                    s.cong = C_CWR;
-                    s.recover = s.high;
-                }
+                ack_duplicate(x, f, s, una_before, now);
+                cc_acked(x, s, 1, ack, now);
             }
+            else if (ack > s.high)
+                s.nxt = ack;
         }
-        else if (s.ecn_s == E_ECE_RCVD && !ece)
-            s.ecn_s = E_IDLE;
-        bool dupack = ack == old && ack < s.high;
-        uint32_t old_dup = s.dupacks;
-        if (dupack)
+        else
         {
-            uint8_t cg = s.cong;
-            if (cg != C_LOSS)
-            {
-                if (cg != C_RECOVERY)
-                {
-                    s.dupacks = old_dup + 1;
-                    old_dup = s.dupacks - 1;
-                }
-                if (cg == C_OPEN)
-                    s.cong = cg = C_DISORDER;
-                if (cg == C_RECOVERY)
-                    tb_add_reno_sack(t);
-                else if (cg == C_DISORDER)
-                {
-                    if (s.dupacks == (uint32_t)DUPTHRESH && (old >= s.recover || !s.recover_active))
-                        enter_recovery(x, f, now);
-                    else if (tb_head_lost(t))
-                        enter_recovery(x, f, now);
-                }
-            }
-        }
-        if (ack == old && ack == s.high)
-        {
-        }
-        else if (ack == old && ack > s.high)
-            s.nxt = ack;
-        else if (ack == old)
-            pkts_acked(x, s, 1, ack, now);
-        else if (advancing)
-        {
-            uint32_t segs = delivered / MSS;
-            s.ack_rem += delivered % MSS;
+            // whole segments acknowledged, the remainders carried over
+            uint32_t segs = acked / MSS;
+            s.ack_rem += acked % MSS;
             if (s.ack_rem >= MSS)
             {
-                segs++;
                 s.ack_rem -= MSS;
+                segs++;
             }
-            if (!dupack)
-                s.dupacks = 0;
-            uint8_t cg = s.cong;
-            if (ack < s.recover && cg == C_RECOVERY)
-            {
-                tb_mark_head_lost(t);
-                if (!tb_retx_ends_at(t, ack + MSS))
-                    do_retransmit(x, f, now);
-                pkts_acked(x, s, 1, ack, now);
-                new_ack(x, f, ack, s.first_partial, now);
-                s.first_partial = 0;
-            }
-            else if (ack < s.recover && cg == C_LOSS)
-            {
-                pkts_acked(x, s, segs, ack, now);
-                increase_window(x, s, segs, ack, now);
-                new_ack(x, f, ack, true, now);
-            }
-            else if (cg == C_CWR)
-            {
-                pkts_acked(x, s, segs, ack, now);
-                new_ack(x, f, ack, true, now);
-            }
+            s.dupacks = 0;
+            if (ack < s.recover && (s.cong == C_RECOVERY || s.cong == C_LOSS))
+                ack_partial(x, f, s, ack, segs, now);
             else
-            {
-                bool exited = false;
-                if (cg == C_OPEN)
-                    pkts_acked(x, s, segs, ack, now);
-                else if (cg == C_DISORDER)
-                {
-                    if (segs >= old_dup)
-                        pkts_acked(x, s, segs - old_dup, ack, now);
-                    if (!dupack)
-                        s.cong = C_OPEN;
-                }
-                else if (cg == C_RECOVERY)
-                {
-                    s.first_partial = 1;
-                    segs = (ack - old) / MSS;
-                    pkts_acked(x, s, segs, ack, now);
-                    s.cong = C_OPEN;
-                    exited = true;
-                    s.dupacks = 0;
-                }
-                else if (cg == C_LOSS)
-                {
-                    s.first_partial = 1;
-                    segs = (ack - s.recover) / MSS;
-                    pkts_acked(x, s, segs, ack, now);
-                    s.cong = C_OPEN;
-                }
-                if (ack >= s.recover)
-                    s.recover_active = 0;
-                if (exited)
-                {
-                    new_ack(x, f, ack, true, now);
-                    s.cwnd = s.ssthresh;
-                }
-                if (s.cong == C_OPEN)
-                {
-                    increase_window(x, s, segs, ack, now);
-                    new_ack(x, f, ack, true, now);
-                }
-            }
+                ack_advance(x, f, s, ack, una_before, segs, dups, now);
         }
-        if ((s.fin & FIN_SENT) && s.it_n == 0 && ack == s.high + 1)
+        if ((s.fin & FIN_SENT) && !s.it_n && ack == s.high + 1)
             s.fin |= FIN_ACKED;
         if (advancing)
         {
-            messages_acked(x, f, now);
+            msgs_complete(x, f, now);
             if (s.st == S_DONE)
                 return;
         }
-        send_pending(x, f, now);
+        tx_fill(x, f, now);
     }
-    static __device__ __forceinline__ void on_rto(Host &x, int f, int64_t now)
+    // the retransmission timeout: every outstanding segment lost, the timeout doubled (to 60 s at
+    // most), one segment's window, then sending again
+    static __device__ void rto_fire(Host &x, int f, int64_t now)
     {
         SndState &s = *x.ls;
         s.rto_at = -1;
@@ -1318,84 +1305,90 @@ struct Model
             return;
         }
         count(x.R, EV_RTO);
-        uint32_t before = inflight(s);
+        const uint32_t was_in_flight = inflight(s);
+        Ledger L(*x.B, s, f);
+        lg_all_lost(L);
         s.dupacks = 0;
-        Tx t(*x.B, *x.ls, f);
-        tb_all_lost(t);
         s.recover = s.high;
         s.recover_active = 1;
-        int64_t r = 2 * s.rto;
-        s.rto = r < 60000000000000ll ? r : 60000000000000ll;
+        s.rto = 2 * s.rto < 60000000000000ll ? 2 * s.rto : 60000000000000ll;
         s.h_n = 0;
-        s.ssthresh = ssthresh_for(x, s, before);
+        s.ssthresh = cc_ssthresh(x, s, was_in_flight);
         if (x.cubic)
-            cub_loss(s, now);
+            cubic_timeout(s, now);
         s.cong = C_LOSS;
         s.cwnd = MSS;
-        send_pending(x, f, now);
+        tx_fill(x, f, now);
     }
 
-    // -------------------------------------------------------------- the receiving socket
-    // (src/step.cuh:1727-2106)
-    static __device__ void send_synack(Host &x, int f, int64_t now)
+    // -------------------------------------------------------------- the receiving endpoint
+    // A connection's receiving side: the SYN-ACK and its retries, in-order delivery through the
+    // reassembly ring, delayed ACKs (every second segment or after the delay), DCTCP's CE echo
+    // state machine, and the passive close (FIN+ACK, LAST_ACK with its retries).
+    // an ACK now (the delayed one, if pending, is folded in); ECE while echoing CE unless no_ece,
+    // always with force_ece
+    static __device__ void rcv_ack(Host &x, int f, uint32_t ack, bool force_ece, bool no_ece, int64_t now)
     {
         RcvState &r = *x.lr;
-        int64_t rto = CONN_TIMEOUT_PS << r.srv_syn_n;
-        r.srv_syn_n++;
+        const bool echoing = r.ecn_r == E_CE_RCVD || r.ecn_r == E_SENDING_ECE;
+        const bool ece = force_ece || (!no_ece && echoing);
+        if (!force_ece && ece)
+            r.ecn_r = E_SENDING_ECE;
+        r.delack_at = -1;
+        r.delack_count = 0;
+        emit(x, f, (uint8_t)(F_ACK | (ece ? F_ECE : 0)), 0, ack, CTRL_FRAME, true, now);
+    }
+    static __device__ void rcv_synack(Host &x, int f, int64_t now)
+    {
+        RcvState &r = *x.lr;
+        const int64_t backoff = CONN_TIMEOUT_PS << r.srv_syn_n++;
         if (r.r_synack_t < 0)
             r.r_synack_t = now;
         emit(x, f, F_SYN | F_ACK, 0, 0, SYN_FRAME, true, now);
-        if (r.srv_syn_at < 0)
-        {
-            r.srv_syn_at = now + rto;
-            timer_chain(x, x.B->dchain, f, rto);
-        }
+        if (r.srv_syn_at >= 0)
+            return;
+        r.srv_syn_at = now + backoff;
+        timer_chain(x, x.B->dchain, f, backoff);
     }
     static __device__ __forceinline__ void rcv_established(RcvState &r)
     {
-        if (!r.r_est)
-        {
-            r.r_est = 1;
-            r.srv_syn_at = -1;
-        }
+        if (r.r_est)
+            return;
+        r.r_est = 1;
+        r.srv_syn_at = -1;
     }
-    static __device__ void ce_state(Host &x, int f, bool ce_now, int64_t now)
+    // DCTCP's receiver: a change of the CE state flushes a pending delayed ACK for the bytes
+    // before it, with the previous state's ECE (set when leaving CE, cleared when entering it)
+    static __device__ void rcv_ce_edge(Host &x, int f, bool ce, int64_t now)
     {
         RcvState &r = *x.lr;
-        if (ce_now)
-        {
-            if (!r.ce_state && r.reserved && r.prior_flag)
-                send_ack(x, f, r.prior_rcv_nxt, false, true, now);
-            r.ce_state = 1;
+        const bool flips = (bool)r.ce_state != ce;
+        if (flips && r.reserved && r.prior_flag)
+            rcv_ack(x, f, r.prior_rcv_nxt, !ce, ce, now);
+        r.ce_state = ce ? 1 : 0;
+        if (ce)
             r.ecn_r = E_CE_RCVD;
-        }
-        else
-        {
-            if (r.ce_state && r.reserved && r.prior_flag)
-                send_ack(x, f, r.prior_rcv_nxt, true, false, now);
-            r.ce_state = 0;
-            if (r.ecn_r == E_CE_RCVD || r.ecn_r == E_SENDING_ECE)
-                r.ecn_r = E_IDLE;
-        }
+        else if (r.ecn_r == E_CE_RCVD || r.ecn_r == E_SENDING_ECE)
+            r.ecn_r = E_IDLE;
         r.prior_flag = 1;
         r.prior_rcv_nxt = r.rcv_nxt;
     }
-    static __device__ void rx_ecn(Host &x, int f, const Frame &p, int64_t now)
+    static __device__ void rcv_ecn_in(Host &x, int f, const Frame &p, int64_t now)
     {
         if (!x.dctcp)
             return;
         RcvState &r = *x.lr;
-        bool ce = p.flags & F_CE;
-        if (ce && r.ecn_ce_seq < (int64_t)p.sa)
+        if ((p.flags & F_CE) && r.ecn_ce_seq < (int64_t)p.sa)
         {
             r.ecn_ce_seq = p.sa;
             r.ecn_r = E_CE_RCVD;
-            ce_state(x, f, true, now);
+            rcv_ce_edge(x, f, true, now);
         }
         else if (p.flags & F_ECT)
-            ce_state(x, f, false, now);
+            rcv_ce_edge(x, f, false, now);
     }
-    static __device__ void snd_ecn(Host &x, SndState &s, const Frame &p)
+    // the CE state of the sending side's own ECN (ACKs carrying CE)
+    static __device__ void snd_ecn_in(Host &x, SndState &s, const Frame &p)
     {
         if (!x.dctcp)
             return;
@@ -1403,184 +1396,182 @@ struct Model
         {
             s.s_ce = 3;
             s.ecn_s = E_CE_RCVD;
+            return;
         }
-        else if (p.flags & F_ECT)
-        {
-            s.s_ce &= ~2;
-            if (s.ecn_s == E_CE_RCVD || s.ecn_s == E_SENDING_ECE)
-                s.ecn_s = E_IDLE;
-        }
+        if (!(p.flags & F_ECT))
+            return;
+        s.s_ce &= ~2;
+        if (s.ecn_s == E_CE_RCVD || s.ecn_s == E_SENDING_ECE)
+            s.ecn_s = E_IDLE;
     }
-    // TcpRxBuffer::Add for [seq, end): the out-of-order blocks stay disjoint and sorted
-    static __device__ bool rx_add(Host &x, int f, uint32_t seq, uint32_t end)
+    // The reassembly ring (TcpRxBuffer::Add): out-of-order blocks, disjoint and sorted, in a ring
+    // per connection. [seq, end) above rcv_nxt joins the blocks it touches; a block reaching
+    // rcv_nxt is delivered. False if nothing was new (or the ring is full).
+    static __device__ bool reasm_add(Host &x, int f, uint32_t seq, uint32_t end)
     {
         const Batch &B = *x.B;
-        WorldRun *R = x.R;
         RcvState &r = *x.lr;
         const ConnStatic &c = B.cs[f];
-        uint32_t head = seq > r.rcv_nxt ? seq : r.rcv_nxt;
-        if (head >= end)
+        const uint32_t from = seq > r.rcv_nxt ? seq : r.rcv_nxt;
+        if (from >= end)
             return false;
-        uint32_t *bs = B.oo_s + c.oo_off, *be = B.oo_e + c.oo_off;
+        uint32_t *lo = B.oo_s + c.oo_off, *hi = B.oo_e + c.oo_off;
         const uint32_t cap = c.oo_cap, h0 = r.ooo_h;
         uint32_t n = r.ooo_n;
-        auto at = [&](uint32_t i) { uint32_t v = h0 + i; return v >= cap ? v - cap : v; };
-        uint32_t i;
-        if (n == 0 || be[at(n - 1)] < head)
-            i = n;
-        else
+        auto at = [&](uint32_t i) { return h0 + i < cap ? h0 + i : h0 + i - cap; };
+        // the first block ending at or after `from`
+        uint32_t i = n;
+        if (n && hi[at(n - 1)] >= from)
         {
-            uint32_t lo = 0, hi = n - 1;
-            while (lo < hi)
+            uint32_t a = 0, b = n - 1;
+            while (a < b)
             {
-                uint32_t m = (lo + hi) / 2;
-                if (be[at(m)] < head)
-                    lo = m + 1;
+                const uint32_t m = (a + b) / 2;
+                if (hi[at(m)] < from)
+                    a = m + 1;
                 else
-                    hi = m;
+                    b = m;
             }
-            i = lo;
+            i = a;
         }
-        if (i < n && bs[at(i)] <= head && be[at(i)] >= end)
-            return false;
-        uint32_t us = head, ue = end, j = i;
-        while (j < n && bs[at(j)] <= ue)
+        if (i < n && lo[at(i)] <= from && hi[at(i)] >= end)
+            return false; // (all of it is held already)
+        // blocks i .. j - 1 touch [from, end): they merge into one
+        uint32_t j = i, nlo = from, nhi = end;
+        for (; j < n && lo[at(j)] <= nhi; j++)
         {
-            if (bs[at(j)] < us)
-                us = bs[at(j)];
-            if (be[at(j)] > ue)
-                ue = be[at(j)];
-            j++;
+            nlo = lo[at(j)] < nlo ? lo[at(j)] : nlo;
+            nhi = hi[at(j)] > nhi ? hi[at(j)] : nhi;
         }
         if (j == i)
         {
+            // a new block at i
             if (n >= cap)
             {
-                count(R, EV_OOO_OVF);
+                count(x.R, EV_OOO_OVF);
                 return false;
             }
             for (uint32_t k = n; k > i; k--)
             {
-                bs[at(k)] = bs[at(k - 1)];
-                be[at(k)] = be[at(k - 1)];
+                lo[at(k)] = lo[at(k - 1)];
+                hi[at(k)] = hi[at(k - 1)];
             }
             n++;
         }
-        else
+        else if (j - i > 1)
         {
-            uint32_t gone = j - i - 1;
-            if (gone)
-                for (uint32_t k = j; k < n; k++)
-                {
-                    bs[at(k - gone)] = bs[at(k)];
-                    be[at(k - gone)] = be[at(k)];
-                }
+            // the merged blocks after the first close up
+            const uint32_t gone = j - i - 1;
+            for (uint32_t k = j; k < n; k++)
+            {
+                lo[at(k - gone)] = lo[at(k)];
+                hi[at(k - gone)] = hi[at(k)];
+            }
             n -= gone;
         }
-        bs[at(i)] = us;
-        be[at(i)] = ue;
-        uint32_t h = h0;
-        if (head == r.rcv_nxt)
+        lo[at(i)] = nlo;
+        hi[at(i)] = nhi;
+        uint32_t head = h0;
+        if (from == r.rcv_nxt)
         {
-            r.rcv_nxt = be[at(0)];
-            h = at(1);
+            // the first block is in order now: delivered
+            r.rcv_nxt = hi[at(0)];
+            head = at(1);
             n--;
         }
-        r.ooo_h = n ? h : 0;
         r.ooo_n = n;
+        r.ooo_h = n ? head : 0;
         if (r.r_fin >= 0 && r.rcv_nxt == (uint32_t)r.r_fin)
-            r.rcv_nxt++;
+            r.rcv_nxt++; // (the FIN's sequence number)
         return true;
     }
-    static __device__ __forceinline__ bool rx_finished(const RcvState &r) { return r.r_fin >= 0 && (uint32_t)r.r_fin < r.rcv_nxt; }
-    static __device__ void send_fin_ack(Host &x, int f, int64_t now)
+    static __device__ __forceinline__ bool rcv_fin_in(const RcvState &r) { return r.r_fin >= 0 && (uint32_t)r.r_fin < r.rcv_nxt; }
+    static __device__ __forceinline__ int64_t rcv_rto(const RcvState &r)
+    {
+        const int64_t base = r.r_srtt >= 0 ? r.r_srtt : INIT_RTT_PS, spread = 4 * r.r_rttvar > CLOCK_GRAN_PS ? 4 * r.r_rttvar : CLOCK_GRAN_PS;
+        return base + spread;
+    }
+    // FIN+ACK of the passive close, with its retransmission timer (at least MinRto) if none runs
+    static __device__ void rcv_fin_ack(Host &x, int f, int64_t now)
     {
         RcvState &r = *x.lr;
         r.delack_at = -1;
         r.delack_count = 0;
         emit(x, f, F_ACK | F_FIN, 0, r.rcv_nxt, CTRL_FRAME, true, now);
-        if (r.r_frx_at < 0)
-        {
-            int64_t est = r.r_srtt >= 0 ? r.r_srtt : INIT_RTT_PS, v = 4 * r.r_rttvar;
-            int64_t rto = est + (v > CLOCK_GRAN_PS ? v : CLOCK_GRAN_PS);
-            rto = rto > MIN_RTO_PS ? rto : MIN_RTO_PS;
-            r.r_frx_at = now + rto;
-            timer_chain(x, x.B->fchain, f, rto);
-        }
+        if (r.r_frx_at >= 0)
+            return;
+        const int64_t t = rcv_rto(r) > MIN_RTO_PS ? rcv_rto(r) : MIN_RTO_PS;
+        r.r_frx_at = now + t;
+        timer_chain(x, x.B->fchain, f, t);
     }
-    static __device__ __forceinline__ int64_t last_rto(const RcvState &r)
-    {
-        int64_t est = r.r_srtt >= 0 ? r.r_srtt : INIT_RTT_PS, v = 4 * r.r_rttvar;
-        return est + (v > CLOCK_GRAN_PS ? v : CLOCK_GRAN_PS);
-    }
-    static __device__ void do_peer_close(Host &x, int f, int64_t now)
+    // the peer closed: LAST_ACK, our FIN+ACK, and the LastAck timer with its retries
+    static __device__ void rcv_peer_closed(Host &x, int f, int64_t now)
     {
         RcvState &r = *x.lr;
         r.r_st = R_LAST_ACK;
-        send_fin_ack(x, f, now);
+        rcv_fin_ack(x, f, now);
         r.r_last_n = DATA_RETRIES;
-        int64_t lr = last_rto(r);
-        r.r_last_at = now + lr;
-        timer_chain(x, x.B->dchain, f, lr);
+        const int64_t t = rcv_rto(r);
+        r.r_last_at = now + t;
+        timer_chain(x, x.B->dchain, f, t);
     }
     static __device__ __forceinline__ void rcv_closed(RcvState &r)
     {
         r.r_st = R_CLOSED;
-        r.r_last_at = -1;
-        r.r_frx_at = -1;
-        r.delack_at = -1;
+        r.delack_at = r.r_last_at = r.r_frx_at = -1;
     }
-    static __device__ void received_data(Host &x, int f, uint32_t seq, uint32_t len, bool fin, int64_t now)
+    // data delivered to the ring: an immediate ACK for anything out of order (a gap left, or one
+    // filled), for nothing new, or for every second segment; else the delayed ACK
+    static __device__ void rcv_data(Host &x, int f, uint32_t seq, uint32_t len, bool fin, int64_t now)
     {
         RcvState &r = *x.lr;
-        uint32_t expected = r.rcv_nxt;
-        if (!rx_add(x, f, seq, seq + len))
+        const uint32_t before = r.rcv_nxt;
+        if (!reasm_add(x, f, seq, seq + len))
         {
-            send_ack(x, f, r.rcv_nxt, false, false, now);
+            rcv_ack(x, f, r.rcv_nxt, false, false, now);
             return;
         }
-        if (expected < r.rcv_nxt && rx_finished(r) && !fin)
+        if (!fin && before < r.rcv_nxt && rcv_fin_in(r))
         {
-            do_peer_close(x, f, now);
+            rcv_peer_closed(x, f, now);
             return;
         }
-        bool gap = r.ooo_n > 0, filled = r.rcv_nxt > expected + len;
-        if (gap || filled)
+        const bool irregular = r.ooo_n > 0 || r.rcv_nxt > before + len;
+        const uint32_t held = r.delack_count + 1u;
+        if (irregular || held >= DELACK_COUNT)
         {
             r.reserved = 0;
-            send_ack(x, f, r.rcv_nxt, false, false, now);
+            rcv_ack(x, f, r.rcv_nxt, false, false, now);
             return;
         }
-        uint32_t cnt = r.delack_count + 1u;
-        if (cnt >= DELACK_COUNT)
-        {
-            r.reserved = 0;
-            send_ack(x, f, r.rcv_nxt, false, false, now);
-        }
-        else
-        {
-            r.delack_count = (uint8_t)cnt;
-            r.reserved = 1;
-            if (r.delack_at < 0)
-            {
-                r.delack_at = now + DELACK_PS;
-                timer_chain(x, x.B->dchain, f, DELACK_PS);
-            }
-        }
+        r.delack_count = (uint8_t)held;
+        r.reserved = 1;
+        if (r.delack_at >= 0)
+            return;
+        r.delack_at = now + DELACK_PS;
+        timer_chain(x, x.B->dchain, f, DELACK_PS);
     }
-    static __device__ void on_data(Host &x, int f, const Frame &p, int64_t now)
+    // a data (or FIN) segment at the receiver: old data in an established connection is answered
+    // with an ACK (in LAST_ACK: anything but the next byte); the CE state; a CWR ends the echo;
+    // the first segment establishes; in LAST_ACK only the close's ACK counts; a FIN is taken in
+    // order, marks the end and closes passively once delivered
+    static __device__ void rcv_segment(Host &x, int f, const Frame &p, int64_t now)
     {
         RcvState &r = *x.lr;
         if (!r.r_up || r.r_st == R_CLOSED)
             return;
-        uint32_t seq = p.sa, len = (p.flags & F_DATA) ? p.len - 42u : 0u, end = seq + len;
-        bool fin = p.flags & F_FIN;
-        if (len > 0 && r.r_est && (r.r_st == R_LAST_ACK ? seq != r.rcv_nxt : end < r.rcv_nxt))
+        const bool fin = p.flags & F_FIN;
+        const uint32_t seq = p.sa, len = (p.flags & F_DATA) ? p.len - 42u : 0u, end = seq + len;
+        if (len && r.r_est)
         {
-            send_ack(x, f, r.rcv_nxt, false, true, now);
-            return;
+            const bool stale = r.r_st == R_LAST_ACK ? seq != r.rcv_nxt : end < r.rcv_nxt;
+            if (stale)
+            {
+                rcv_ack(x, f, r.rcv_nxt, false, true, now);
+                return;
+            }
         }
-        rx_ecn(x, f, p, now);
+        rcv_ecn_in(x, f, p, now);
         if ((p.flags & F_CWR) && r.r_est && r.r_st == R_OPEN && r.ecn_r != E_CE_RCVD)
             r.ecn_r = E_IDLE;
         if (!r.r_est)
@@ -1597,7 +1588,7 @@ struct Model
         }
         if (!fin)
         {
-            received_data(x, f, seq, len, false, now);
+            rcv_data(x, f, seq, len, false, now);
             return;
         }
         if (seq < r.rcv_nxt)
@@ -1606,17 +1597,16 @@ struct Model
         if (r.rcv_nxt == end)
             r.rcv_nxt++;
         if (len)
-            received_data(x, f, seq, len, true, now);
-        if (!rx_finished(r))
-            return;
-        do_peer_close(x, f, now);
+            rcv_data(x, f, seq, len, true, now);
+        if (rcv_fin_in(r))
+            rcv_peer_closed(x, f, now);
     }
 
     // -------------------------------------------------------------- application and dispatch
     static __device__ void add_active(Host &x, bool snd, int f)
     {
         int32_t *n = snd ? x.B->n_snd : x.B->n_rcv;
-        int k = n[x.h];
+        const int k = n[x.h];
         if (k >= MAX_ACTIVE)
         {
             count(x.R, EV_ACTIVE_OVF);
@@ -1625,8 +1615,8 @@ struct Model
         (snd ? x.B->act_snd : x.B->act_rcv)[(size_t)x.h * MAX_ACTIVE + k] = f;
         n[x.h] = k + 1;
     }
-    // FlowSender::Write of the due messages; Socket::Send defers SendPendingData by 1 ps
-    // (src/step.cuh:2120-2145)
+    // the application writes its due messages (FlowSender::Write); Socket::Send defers
+    // SendPendingData by 1 ps when the window has room (src/step.cuh:2120-2145)
     static __device__ void app_write(Host &x, int f, int64_t now)
     {
         const Batch &B = *x.B;
@@ -1634,72 +1624,62 @@ struct Model
         const ConnStatic &c = B.cs[f];
         if (s.m_wr >= s.m_due)
             return;
-        while (s.m_wr < s.m_due)
+        for (; s.m_wr < s.m_due; s.m_wr++)
         {
-            int m = c.m_first + s.m_wr;
+            const int m = c.m_first + s.m_wr;
             B.mg_est[m] = now;
             s.tail = B.mg_end[m];
-            s.m_wr++;
         }
         if (c.fin_on && s.m_wr == c.m_cnt)
             s.fin |= FIN_CLOSE;
-        uint32_t pipe = inflight(s);
-        if (s.cwnd > pipe && s.syn_at < 0)
-        {
-            s.syn_at = now + 1;
-            Chain ch;
-            chain_child(ch, 2, x.echain);
-            chain_store(B.schain + (size_t)f * CW, ch);
-            due_insert(x, now + 1, f, 2);
-        }
+        if (s.cwnd <= inflight(s) || s.syn_at >= 0)
+            return;
+        s.syn_at = now + 1;
+        Chain ch;
+        chain_child(ch, 2, x.echain);
+        chain_store(B.schain + (size_t)f * CW, ch);
+        due_insert(x, now + 1, f, 2);
     }
-    // a frame at its destination host (src/step.cuh:2147-2236): the sending side's part
+    // a frame at its destination host (src/step.cuh:2147-2236): the sending side's part - the
+    // SYN-ACK establishes (RTT sample, the handshake's ACK, the first write), a FIN+ACK (the
+    // sender's close) is acknowledged once, any other ACK goes to ack_in
     static __device__ void on_frame_snd(Host &x, int f, const Frame &p, int64_t now)
     {
+        SndState &s = *x.ls;
+        if (p.flags & F_SYN)
         {
-            SndState &s = *x.ls;
-            if (p.flags & F_SYN)
-            {
-                if (s.st == S_SYN_SENT)
-                {
-                    s.st = S_EST;
-                    s.syn_at = -1;
-                    rtt_sample(s, now - s.timing_time);
-                    emit(x, f, F_ACK, 0, 0, CTRL_FRAME, false, now);
-                    chain_child(x.echain, 1, x.echain);
-                    app_write(x, f, now);
-                }
-            }
-            else if (p.flags & F_FIN)
-            {
-                if (s.fin & FIN_TW)
-                    return;
-                snd_ecn(x, s, p);
-                on_ack(x, f, p.sa, p.flags & F_ECE, now);
-                s.fin |= FIN_TW;
-                emit(x, f, F_ACK, 0, 1, CTRL_FRAME, false, now);
-            }
-            else
-            {
-                if (s.fin & FIN_TW)
-                    return;
-                snd_ecn(x, s, p);
-                on_ack(x, f, p.sa, p.flags & F_ECE, now);
-            }
+            if (s.st != S_SYN_SENT)
+                return;
+            s.st = S_EST;
+            s.syn_at = -1;
+            rtt_sample(s, now - s.timing_time);
+            emit(x, f, F_ACK, 0, 0, CTRL_FRAME, false, now);
+            chain_child(x.echain, 1, x.echain);
+            app_write(x, f, now);
             return;
         }
+        if (s.fin & FIN_TW)
+            return;
+        snd_ecn_in(x, s, p);
+        ack_in(x, f, p.sa, p.flags & F_ECE, now);
+        if (!(p.flags & F_FIN))
+            return;
+        s.fin |= FIN_TW;
+        emit(x, f, F_ACK, 0, 1, CTRL_FRAME, false, now);
     }
-    // ... and the receiving side's
+    // ... and the receiving side's: the handshake's RTT for the receiver, a SYN (the connection
+    // comes up; a SYN-ACK unless established), data, or an ACK (establishes; ends LAST_ACK)
     static __device__ void on_frame_rcv(Host &x, int f, const Frame &p, int64_t now)
     {
         RcvState &r = *x.lr;
-        if (!(p.flags & F_SYN) && r.r_up && r.r_srtt < 0 && r.r_synack_t >= 0)
+        const bool syn = p.flags & F_SYN;
+        if (!syn && r.r_up && r.r_srtt < 0 && r.r_synack_t >= 0)
         {
-            int64_t m = now - r.r_synack_t;
+            const int64_t m = now - r.r_synack_t;
             r.r_srtt = m;
             r.r_rttvar = m / 2;
         }
-        if (p.flags & F_SYN)
+        if (syn)
         {
             if (!r.r_up)
             {
@@ -1711,17 +1691,20 @@ struct Model
                 return;
             x.nic.woke = now;
             chain_child(x.echain, 1, x.echain);
-            send_synack(x, f, now);
+            rcv_synack(x, f, now);
+            return;
         }
-        else if (p.flags & (F_DATA | F_FIN))
-            on_data(x, f, p, now);
-        else if (r.r_up && r.r_st != R_CLOSED)
+        if (p.flags & (F_DATA | F_FIN))
         {
-            rx_ecn(x, f, p, now);
-            rcv_established(r);
-            if (r.r_st == R_LAST_ACK)
-                rcv_closed(r);
+            rcv_segment(x, f, p, now);
+            return;
         }
+        if (!r.r_up || r.r_st == R_CLOSED)
+            return;
+        rcv_ecn_in(x, f, p, now);
+        rcv_established(r);
+        if (r.r_st == R_LAST_ACK)
+            rcv_closed(r);
     }
     // one event per frame: the side's record in LDS for its duration
     static __device__ void on_frame(Host &x, const Frame &p, int64_t now)
@@ -1897,7 +1880,7 @@ struct Model
                 add_active(x, true, f);
                 s.st = S_SYN_SENT;
                 if (x.cubic)
-                    cub_hystart_reset(s, now);
+                    cubic_round_start(s, now);
                 s.rto = CONN_TIMEOUT_PS;
                 s.syn_at = now + CONN_TIMEOUT_PS;
                 timer_chain(x, B.rchain, f, CONN_TIMEOUT_PS);
@@ -1915,7 +1898,7 @@ struct Model
                     if (B.snd[f].rto_at == d.at)
                     {
                         snd_open(x, f);
-                        on_rto(x, f, now);
+                        rto_fire(x, f, now);
                         snd_close(x, f);
                     }
                 }
@@ -1925,7 +1908,7 @@ struct Model
                     if (s.syn_at == d.at && s.st == S_EST)
                     {
                         s.syn_at = -1; // the deferred SendPendingData
-                        send_pending(x, f, now);
+                        tx_fill(x, f, now);
                     }
                     else if (s.syn_at == d.at && s.st == S_SYN_SENT)
                     {
@@ -1945,7 +1928,7 @@ struct Model
                         r.delack_at = -1;
                         r.delack_count = 0;
                         r.reserved = 1;
-                        send_ack(x, f, r.rcv_nxt, false, false, now);
+                        rcv_ack(x, f, r.rcv_nxt, false, false, now);
                     }
                     else if (d.kind == 4 && r.r_last_at == d.at && r.r_st == R_LAST_ACK)
                     {
@@ -1954,8 +1937,8 @@ struct Model
                         else
                         {
                             r.r_last_n--;
-                            send_fin_ack(x, f, now);
-                            int64_t lr = last_rto(r);
+                            rcv_fin_ack(x, f, now);
+                            int64_t lr = rcv_rto(r);
                             r.r_last_at = now + lr;
                             timer_chain(x, B.dchain, f, lr);
                         }
@@ -1963,12 +1946,12 @@ struct Model
                     else if (d.kind == 5 && r.r_frx_at == d.at && r.r_st == R_LAST_ACK)
                     {
                         r.r_frx_at = -1;
-                        send_fin_ack(x, f, now);
+                        rcv_fin_ack(x, f, now);
                     }
                     else if (d.kind == 3 && r.srv_syn_at == d.at && !r.r_est)
                     {
                         r.srv_syn_at = -1;
-                        send_synack(x, f, now);
+                        rcv_synack(x, f, now);
                     }
                     rcv_close(x, f);
                 }
