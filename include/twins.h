@@ -136,6 +136,16 @@ typedef struct tw_run_stats
 /* Runs every world of the batch to completion (all flows done or max_slots).
  * (src/main.cu:1024-1048, the graph-launch loop) */
 int tw_batch_run(tw_batch *b, tw_run_stats *stats);
+/* tw_batch_run without allocating anything: no hipMalloc or hipFree, no graph, no allocating
+ * library call. The launch's scratch memory (the batch record the kernel reads, the teams' barrier
+ * words, the world-queue counters and the switch groups' sort space) comes from a static device
+ * array of the library's own, 64 MiB per GPU, held until the process ends; a batch whose launch
+ * needs more (above about 33 workgroups of 256 threads: make it with tw_pack_cfg.blocks <= 32) is
+ * refused with TW_EINVAL. Results equal tw_batch_run's. Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (from several threads or batches at once) race. No
+ * counterpart in the reference; it is for callers that cannot allocate device memory while they
+ * run (code under stream capture, tight serving loops, harnesses that forbid allocation). */
+int tw_batch_run_noalloc(tw_batch *b, tw_run_stats *stats);
 
 /* Per flow of world `world`, in its file order: when the socket got it (established_ns) and when
  * its last byte was acknowledged (completed_ns); -1 if not reached. (src/main.cu:1143-1153) */

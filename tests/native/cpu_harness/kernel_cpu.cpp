@@ -148,6 +148,13 @@ hipError_t tcp_reset(uint32_t *queue_next, int nq, void *teams, size_t team_byte
     std::fill((char *)teams, (char *)teams + team_bytes, (char)0);
     return hipSuccess;
 }
+// the static scratch array of tw_batch_run_noalloc: host memory here
+hipError_t tcp_static_scratch(void **ptr)
+{
+    static std::vector<unsigned char> host(TW_SCRATCH_BYTES);
+    *ptr = host.data();
+    return hipSuccess;
+}
 hipError_t tcp_server_launch_9(const void *, int, int, hipStream_t) { return hipErrorNotSupported; }
 hipError_t tcp_server_launch_25(const void *, int, int, hipStream_t) { return hipErrorNotSupported; }
 hipError_t tcp_max_blocks_9(int, int *per_cu)

@@ -71,3 +71,20 @@ def test_kernel_code_qtrace_equals_oracle(harness, name, tmp_path):
     assert r.returncode == 0, r.stderr[-3000:]
     a, b = o.read_bytes(), h.read_bytes()
     assert len(a) > 10000 and a == b
+
+
+def test_noalloc_entry_on_cpu(harness, tmp_path):
+    """tw_batch_run_noalloc through `twins --noalloc=1`: the launch's record, barrier words, queue
+    counters and switch sort space moved into the static scratch array (host memory here), every
+    access bounds-checked; the flows equal ns-3's. A launch larger than the array is refused."""
+    by_name = {c[0]: c for c in util.cases()}
+    _, fl, ref, args = by_name["k8_r30-newreno-sprayfin"]
+    out = str(tmp_path / "fct.csv")
+    r = subprocess.run([harness] + args + [f"--flows={fl}", f"--out={out}", "--noalloc=1", "--blocks=8"], cwd=util.GOLDEN,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    same, n = util.same_times(os.path.join(util.GOLDEN, ref), out)
+    assert same == n
+    r = subprocess.run([harness] + args + [f"--flows={fl}", f"--out={out}", "--noalloc=1", "--blocks=40"], cwd=util.GOLDEN,
+                       capture_output=True, text=True, timeout=600, env={**os.environ, "ASAN_OPTIONS": "detect_leaks=0"})
+    assert r.returncode == 1 and "static scratch" in r.stderr

@@ -107,6 +107,7 @@ def lib():
         L.tw_world_destroy.argtypes = [vp]
         L.tw_batch_create.argtypes = [ctypes.POINTER(vp), i32, ctypes.POINTER(PackCfg), ctypes.POINTER(vp)]
         L.tw_batch_run.argtypes = [vp, ctypes.POINTER(RunStats)]
+        L.tw_batch_run_noalloc.argtypes = [vp, ctypes.POINTER(RunStats)]
         L.tw_fct_get.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
         L.tw_counters_get.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_uint64)]
         L.tw_world_stats_get.argtypes = [vp, i32, ctypes.POINTER(WorldStats)]
@@ -191,6 +192,13 @@ class Batch:
     def run(self):
         st = RunStats()
         _check(lib().tw_batch_run(self._h, ctypes.byref(st)))
+        return st
+
+    def run_noalloc(self):
+        """run() through tw_batch_run_noalloc: the launch's scratch in the library's static device
+        array, nothing allocated (the batch's launch must fit it: at most about 33 workgroups)."""
+        st = RunStats()
+        _check(lib().tw_batch_run_noalloc(self._h, ctypes.byref(st)))
         return st
 
     def fct(self, w):

@@ -25,6 +25,10 @@ hipError_t tcp_max_blocks_9(int threads, int *per_cu);
 hipError_t tcp_max_blocks_25(int threads, int *per_cu);
 // zeroes the world-queue counters and the team barrier state before a launch
 hipError_t tcp_reset(uint32_t *queue_next, int nq, void *teams, size_t team_bytes, hipStream_t stream);
+// the library's static scratch array on the current device (tw_batch_run_noalloc): *ptr is its
+// device address, looked up once per device
+constexpr size_t TW_SCRATCH_BYTES = 64ull << 20;
+hipError_t tcp_static_scratch(void **ptr);
 
 // the persistent world-queue kernel over a batch (BatchT<runs> in device memory)
 inline hipError_t tcp_launch(int runs, const void *batch_dev, const LaunchShape &shape, hipStream_t stream)

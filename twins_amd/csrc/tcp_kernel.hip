@@ -16,6 +16,8 @@
 // at most what the GPU holds at once, so all of a team's workgroups are resident, and every wait
 // is bounded (a timeout ends the world with an error flag instead of hanging the queue). Each world's team
 // accounts the device clock time it spent on it (WorldRun::ticks), the input of the autotuner.
+#include <mutex>
+
 #include "tcp_model.h"
 #include "tcp_kernel.h"
 
@@ -422,6 +424,36 @@ hipError_t tcp_reset(uint32_t *queue_next, int nq, void *teams, size_t team_byte
     blocks = blocks < 1 ? 1 : blocks > 1024 ? 1024 : blocks;
     hipLaunchKernelGGL(tcp_reset_kernel, dim3(blocks), dim3(256), 0, stream, queue_next, nq, (uint32_t *)teams, nb);
     return hipGetLastError();
+}
+
+// The scratch memory of tw_batch_run_noalloc: a launch's batch record, team barrier words, queue
+// counters and switch sort space, in a device array of the library's own (every GPU the library
+// runs on holds one copy until the process ends), so that the entry point allocates nothing. Host
+// code never passes the array's name to a kernel: the address to use is the one
+// hipGetSymbolAddress gives, looked up once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[TW_SCRATCH_BYTES];
+
+hipError_t tcp_static_scratch(void **ptr)
+{
+    static std::mutex mu;
+    static void *per_dev[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    if (dev < 0 || dev >= 64)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(mu);
+    if (!per_dev[dev])
+    {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        per_dev[dev] = p;
+    }
+    *ptr = per_dev[dev];
+    return hipSuccess;
 }
 #endif
 

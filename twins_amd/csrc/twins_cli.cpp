@@ -18,6 +18,7 @@ static int usage()
                     "          [--ecmp_from=csv [--hash_check=1]] [--dump_routes=file] [--spray=1 [--RngSeed=1 --RngRun=1]] [--chain_runs=9|25]\n"
                     "          [--qtrace=file --qtrace_from=ps --qtrace_until=ps] (every enqueue in [from, until))\n"
                     "          [--block_threads=256 --blocks=N --team_blocks=T --hosts_per_wave=H] (launch shape; 0: automatic)\n"
+                    "          [--noalloc=1] (run with the library's static scratch, allocating nothing: at most 33 workgroups)\n"
                     "          | --worlds=<file>: one world per line, flows=..,tcp=..,ecnK=..,queue=..,fin=..,ecmp_from=..,out=..)\n"
                     "         leaf-spine: max(N+S, L) <= 32\n");
     return 1;
@@ -31,6 +32,7 @@ int main(int argc, char **argv)
     std::string ecn_k, queue, fin, qtrace;
     long long qtrace_from = 0, qtrace_until = 0;
     int hash_check = 0;
+    int noalloc = 0; // run through tw_batch_run_noalloc (the library's static scratch)
     std::string dump_routes;
     for (int i = 1; i < argc; i++)
     {
@@ -73,6 +75,8 @@ int main(int argc, char **argv)
             dump_routes = val("--dump_routes=");
         else if (is("--hash_check="))
             hash_check = atoi(val("--hash_check=").c_str());
+        else if (is("--noalloc="))
+            noalloc = atoi(val("--noalloc=").c_str());
         else if (a == "--general" || is("--graph_k=") || is("--host_lanes="))
             ; // the reference's kernel-shape and debugging switches: nothing to choose here
         else if (is("--k=") || is("--topo=") || is("--topofile=") || is("--fail=") || is("--leaves=") || is("--spines=") ||
@@ -215,7 +219,7 @@ int main(int argc, char **argv)
         return 1;
     }
     tw_run_stats st;
-    if (tw_batch_run(b, &st))
+    if ((noalloc ? tw_batch_run_noalloc : tw_batch_run)(b, &st))
     {
         fprintf(stderr, "%s\n", tw_last_error());
         return 1;

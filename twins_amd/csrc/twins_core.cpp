@@ -24,6 +24,9 @@
 
 using namespace tw;
 
+// the batch record's place at the head of the static scratch (tw_batch_run_noalloc)
+constexpr size_t NOALLOC_RECORD = 4096;
+
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
 static int fail(int code, const char *fmt, ...)
@@ -843,6 +846,9 @@ struct tw_batch
     hipGraphExec_t gexec = nullptr;
     WorldRun *run = nullptr;
     std::vector<WorldRun> run_host;
+    std::vector<int64_t> slots_before; // (per world, for the launch loops)
+    std::vector<char> batch_host;      // the batch record as uploaded (BatchT<runs>)
+    size_t off_teams = 0, off_queue_next = 0, off_scr_ts = 0, off_scr_ref = 0, off_scr_up = 0; // its scratch pointers
     int64_t *mg_est = nullptr, *mg_done = nullptr;
     QTrace qt{}; // the queue trace's buffer, when enabled
     ~tw_batch()
@@ -1291,6 +1297,16 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     b.mg_est = B.mg_est;
     b.mg_done = B.mg_done;
     b.run_host = run;
+    b.slots_before.assign(run.size(), 0);
+    // the record as uploaded, and where its scratch pointers sit (tw_batch_run_noalloc points
+    // them into the library's static scratch)
+    static_assert(sizeof(Batch) <= NOALLOC_RECORD, "the batch record outgrew its place in the static scratch");
+    b.batch_host.assign((const char *)&B, (const char *)&B + sizeof B);
+    b.off_teams = offsetof(Batch, teams);
+    b.off_queue_next = offsetof(Batch, queue_next);
+    b.off_scr_ts = offsetof(Batch, scr_ts);
+    b.off_scr_ref = offsetof(Batch, scr_ref);
+    b.off_scr_up = offsetof(Batch, scr_up);
     Batch *bd = nullptr;
     if ((rc = dev_alloc(b, &bd, 1)))
         return rc;
@@ -1419,6 +1435,97 @@ extern "C" int tw_batch_run(tw_batch *b, tw_run_stats *stats)
         st.worlds_done += r.finished != 0;
     }
     st.graph = b->gexec != nullptr;
+    if (stats)
+        *stats = st;
+    return TW_OK;
+}
+
+// tw_batch_run with the launch's scratch memory (the batch record the kernel reads, the teams'
+// barrier words, the world-queue counters and the switch groups' sort space) in the library's
+// static device array instead of the batch's own: no allocation of any kind, no graph. Scratch
+// layout: the record, then each region on a 256-byte boundary.
+extern "C" int tw_batch_run_noalloc(tw_batch *b, tw_run_stats *stats)
+{
+    if (!b)
+        return fail(TW_EINVAL, "null batch");
+    if (b->for_server || b->batch_host.empty())
+        return fail(TW_EINVAL, "tw_batch_run_noalloc: a batch made by tw_batch_create");
+    void *sp = nullptr;
+    HIPCHK(tcp_static_scratch(&sp));
+    const size_t per_wave = (size_t)64 * 2 * MAX_TX, waves = (size_t)b->shape.blocks * (b->shape.threads / 64);
+    size_t end = 0;
+    auto region = [&](size_t bytes) {
+        const size_t at = (end + 255) & ~(size_t)255;
+        end = at + bytes;
+        return at;
+    };
+    const size_t o_rec = region(NOALLOC_RECORD), o_teams = region(sizeof(TeamState) * std::max(1, b->nteams)),
+                 o_queue = region(4 * (size_t)std::max(1, b->nwq)), o_ts = region(8 * waves * per_wave), o_ref = region(4 * waves * per_wave),
+                 o_up = region(waves * per_wave);
+    if (end > TW_SCRATCH_BYTES)
+        return fail(TW_EINVAL,
+                    "tw_batch_run_noalloc: the batch's launch (%d workgroups of %d threads) needs %zu bytes of scratch, the static "
+                    "scratch holds %zu (at most %zu wavefronts): make the batch with fewer workgroups (tw_pack_cfg.blocks)",
+                    b->shape.blocks, b->shape.threads, end, TW_SCRATCH_BYTES, (size_t)(TW_SCRATCH_BYTES - 2 * NOALLOC_RECORD) / (13 * per_wave));
+    char *base = (char *)sp;
+    // the record with its scratch pointers moved into the static array (a stack copy: no allocation)
+    alignas(64) char rec[NOALLOC_RECORD];
+    memcpy(rec, b->batch_host.data(), b->batch_host.size());
+    auto put = [&](size_t field, size_t at) {
+        void *v = base + at;
+        memcpy(rec + field, &v, sizeof v);
+    };
+    put(b->off_teams, o_teams);
+    put(b->off_queue_next, o_queue);
+    put(b->off_scr_ts, o_ts);
+    put(b->off_scr_ref, o_ref);
+    put(b->off_scr_up, o_up);
+    HIPCHK(hipMemcpyAsync(base + o_rec, rec, b->batch_host.size(), hipMemcpyHostToDevice, b->stream));
+    const int W = (int)b->worlds.size();
+    tw_run_stats st;
+    memset(&st, 0, sizeof st);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < W; i++)
+        b->slots_before[i] = b->run_host[i].slots_run;
+    auto t0 = std::chrono::steady_clock::now();
+    for (;;)
+    {
+        int unfinished = 0;
+        for (auto &r : b->run_host)
+            unfinished += !r.finished;
+        if (!unfinished)
+            break;
+        HIPCHK(hipEventRecord(b->ev0, b->stream));
+        // counters and barrier words start at zero every launch (a fill kernel, no memset)
+        HIPCHK(tcp_reset((uint32_t *)(base + o_queue), b->nwq, base + o_teams, sizeof(TeamState) * b->nteams, b->stream));
+        HIPCHK(tcp_launch(b->runs, base + o_rec, b->shape, b->stream));
+        HIPCHK(hipEventRecord(b->ev1, b->stream));
+        HIPCHK(hipMemcpyAsync(b->run_host.data(), b->run, sizeof(WorldRun) * W, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        st.kernel_s += ms / 1000.0;
+        st.launches++;
+        int64_t ran = 0;
+        for (int i = 0; i < W; i++)
+        {
+            ran += b->run_host[i].slots_run - b->slots_before[i];
+            b->slots_before[i] = b->run_host[i].slots_run;
+        }
+        if (ran == 0)
+            return fail(TW_ESTATE, "a launch ran no slot of the %d unfinished worlds", unfinished);
+    }
+    st.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (auto &r : b->run_host)
+        if (r.error)
+            return fail(TW_EDEVICE, "a team barrier timed out: the world's results are incomplete");
+    for (auto &r : b->run_host)
+    {
+        st.slots_max = std::max<int64_t>(st.slots_max, r.slots_run);
+        st.packet_events += r.ev[EV_PKT_EVENTS];
+        st.flows_done += r.flows_done;
+        st.worlds_done += r.finished != 0;
+    }
     if (stats)
         *stats = st;
     return TW_OK;
