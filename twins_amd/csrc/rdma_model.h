@@ -604,291 +604,254 @@ RD_HD void cpush(const Sim &s, TailAlloc &ta, Chain *dst, uint32_t code, const C
     *dst = c;
 }
 
-// the runs of a chain (or of the chain cpush(code0, ch) would make), newest first
-struct Cursor
+// Reading a chain's runs newest first, as if `code0` had been pushed on it (a candidate event's
+// chain is compared before it exists): the pushed code (merged with the first run when equal), the
+// inline runs, then the tail nodes, which may have been overwritten since (`lost`).
+struct RunReader
 {
     const Chain *c;
-    int i;         // inline run, RUNS + 1: in the tail
-    uint64_t node; // the current tail node's position + 1
-    uint64_t nxt;
-    uint32_t code; // 0: the end
-    uint32_t left; // repetitions left of the current run
+    int at;         // -1: the pushed code alone; 0 .. RUNS - 1: an inline run; RUNS + 1: in the tail
+    uint64_t node;  // the tail node read (its position), 0: none yet
+    uint64_t after; // the node after it
+    uint32_t code;  // 0: the chain has ended (or its tail is lost)
+    uint32_t left;  // repetitions left of the current run
     bool lost;
 };
-RD_HD void cur_load(const Sim &s, Cursor &k)
+RD_HD void rr_fetch(const Sim &s, RunReader &r)
 {
-    if (k.i < RUNS)
+    if (r.at < RUNS)
     {
-        k.code = k.c->r[k.i].code;
-        k.left = k.c->r[k.i].cnt;
-        if (k.code || k.i < RUNS - 1)
-            return;
-        k.i = RUNS;
+        const Run &x = r.c->r[r.at];
+        r.code = x.code;
+        r.left = x.cnt;
+        if (x.code || r.at + 1 < RUNS)
+            return; // an inline run, or the chain's end within them
+        r.at = RUNS + 1; // every inline run is full: the tail follows
+        r.node = r.c->tail;
     }
-    if (k.i == RUNS && k.node == 0)
+    if (!r.node)
     {
-        k.node = k.c->tail;
-        k.i = RUNS + 1;
-    }
-    if (k.node == 0)
-    {
-        k.code = 0;
+        r.code = 0;
         return;
     }
-    const TailNode n = s.tail[tail_slot(s, k.node)];
-    if (n.pos != k.node)
+    const TailNode n = s.tail[tail_slot(s, r.node)];
+    if (n.pos != r.node)
     {
-        k.lost = true;
-        k.code = 0;
+        r.lost = true; // the ring has wrapped over it
+        r.code = 0;
         return;
     }
-    k.code = n.code;
-    k.left = n.cnt;
-    k.nxt = n.next;
+    r.code = n.code;
+    r.left = n.cnt;
+    r.after = n.next;
 }
-RD_HD void cur_init(const Sim &s, Cursor &k, const Chain &ch, uint32_t code0)
+RD_HD void rr_open(const Sim &s, RunReader &r, const Chain &ch, uint32_t code0)
 {
-    k.c = &ch;
-    k.i = 0;
-    k.node = 0;
-    k.nxt = 0;
-    k.lost = false;
-    if (code0)
+    r.c = &ch;
+    r.node = r.after = 0;
+    r.lost = false;
+    if (!code0)
     {
-        k.code = code0;
-        if (ch.r[0].code == code0)
-            k.left = ch.r[0].cnt + 1; // merged with the first run, consumed with it
-        else
-        {
-            k.left = 1;
-            k.i = -1;
-        }
+        r.at = 0;
+        rr_fetch(s, r);
         return;
     }
-    cur_load(s, k);
+    const bool joins = ch.r[0].code == code0;
+    r.code = code0;
+    r.at = joins ? 0 : -1;
+    r.left = joins ? ch.r[0].cnt + 1 : 1;
 }
-RD_HD void cur_next(const Sim &s, Cursor &k)
+RD_HD void rr_step(const Sim &s, RunReader &r)
 {
-    if (k.i < RUNS)
+    if (r.at > RUNS)
+        r.node = r.after;
+    else if (++r.at == RUNS)
     {
-        k.i++;
-        if (k.i == RUNS)
-        {
-            k.node = k.c->tail;
-            k.i = RUNS + 1;
-        }
+        r.at = RUNS + 1;
+        r.node = r.c->tail;
     }
-    else
-        k.node = k.nxt;
-    cur_load(s, k);
+    rr_fetch(s, r);
 }
 
-// the memo of comparisons decided by leads, keyed by the two lead hashes (src/rdma/rstep.cuh:374-405)
+// The memo of comparisons that leads alone decided, keyed by the two events' lead hashes: a
+// small open-addressed table (four probes, a slot taken by compare-and-swap), shared by a world's
+// entities, so that a comparison once walked is answered from the hashes (src/rdma/rstep.cuh:374-405
+// describes the same table: its key and probe sequence are kept so that hits fall alike).
 RD_HD uint64_t memo_key(uint64_t ha, uint64_t hb)
 {
-    uint64_t k = chash(ha * 0x9e3779b97f4a7c15ull ^ hb, 0x5bd1e995u) & ~3ull;
+    const uint64_t k = chash(ha * 0x9e3779b97f4a7c15ull ^ hb, 0x5bd1e995u) & ~3ull;
     return k ? k : 4;
 }
-RD_HD int memo_get(const Sim &s, uint64_t ha, uint64_t hb)
+RD_HD int memo_find(const Sim &s, uint64_t ha, uint64_t hb)
 {
-    uint64_t k = memo_key(ha, hb);
-    for (int i = 0; i < 4; i++)
+    const uint64_t k = memo_key(ha, hb);
+    for (uint64_t i = 0; i < 4; i++)
     {
-        unsigned long long v = s.memo[(k + i) & s.memo_mask];
-        if (v == 0)
-            return 0;
+        const unsigned long long v = s.memo[(k + i) & s.memo_mask];
+        if (!v)
+            break; // (an empty slot ends the probe)
         if ((v & ~3ull) == k)
             return (v & 3) == 1 ? -1 : 1;
     }
     return 0;
 }
-RD_HD void memo_put(const Sim &s, uint64_t ha, uint64_t hb, int r)
+RD_HD void memo_record(const Sim &s, uint64_t ha, uint64_t hb, int r)
 {
-    uint64_t k = memo_key(ha, hb);
-    unsigned long long v = k | (r < 0 ? 1 : 2);
-    for (int i = 0; i < 4; i++)
+    const uint64_t k = memo_key(ha, hb);
+    for (uint64_t i = 0; i < 4; i++)
     {
-        unsigned long long old = at_cas64(&s.memo[(k + i) & s.memo_mask], 0ull, v);
-        if (old == 0 || (old & ~3ull) == k)
+        const unsigned long long was = at_cas64(&s.memo[(k + i) & s.memo_mask], 0ull, k | (r < 0 ? 1ull : 2ull));
+        if (!was || (was & ~3ull) == k)
             return;
     }
 }
 
-// < 0: a first, > 0: b first, 0: undecided (src/rdma/rstep.cuh:406-503): the first differing lead
-// decides (larger: an earlier ancestor); a chain ending first runs first; with equal leads the
-// deepest differing ordinal decides; distinct roots decide by start order. *by_lead: decided by a
-// lead or an end (a function of the leads alone).
-RD_COLD int ccmp_walk(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b, bool *inexact, bool *by_lead)
+// The order of two simultaneous events by their chains (< 0: a first, > 0: b, 0: undecided),
+// walking both from the newest run (src/rdma/rstep.cuh:406-503 gives the rules): the first
+// differing lead decides (the larger lead is the earlier ancestor); a chain that ends first runs
+// first; with all leads equal the deepest differing ordinal decides, and distinct roots by their
+// start order. *by_lead: decided by a lead or an end, i.e. by the leads alone; *inexact: a chain
+// ended on an overwritten tail node.
+RD_COLD int order_walk(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b, bool *inexact, bool *by_lead)
 {
-    *inexact = false;
-    *by_lead = false;
-    int ord = 0;
-    Cursor A, B;
-    cur_init(s, A, a, pa);
-    cur_init(s, B, b, pb);
-    for (;;)
+    RunReader A, B;
+    rr_open(s, A, a, pa);
+    rr_open(s, B, b, pb);
+    int deep = 0;
+    *inexact = *by_lead = false;
+    while (A.code && B.code)
     {
-        uint32_t ca = A.code, cb = B.code;
-        if (ca == 0 || cb == 0)
-        {
-            *inexact = (ca == 0 && A.lost) || (cb == 0 && B.lost);
-            if (ca == 0 && cb == 0)
-            {
-                if (a.root && b.root && a.root != b.root)
-                    return a.root < b.root ? -1 : 1;
-                return ord;
-            }
-            if (ca == 0)
-            {
-                if (A.lost)
-                    return ord;
-                *by_lead = true;
-                return -1;
-            }
-            if (B.lost)
-                return ord;
-            *by_lead = true;
-            return 1;
-        }
-        if (ca >> ORD_BITS != cb >> ORD_BITS)
+        if ((A.code >> ORD_BITS) != (B.code >> ORD_BITS))
         {
             *by_lead = true;
-            return ca > cb ? -1 : 1;
+            return A.code > B.code ? -1 : 1;
         }
-        if (ca != cb)
-            ord = (ca & ORD_MAX) > (cb & ORD_MAX) ? -1 : 1;
-        // the same place in the same tail node: one lineage from here on
-        if (A.i > RUNS && B.i > RUNS && A.node == B.node && A.left == B.left)
-            return ord;
-        uint32_t m = A.left < B.left ? A.left : B.left;
-        A.left -= m;
-        B.left -= m;
-        if (A.left == 0)
-            cur_next(s, A);
-        if (B.left == 0)
-            cur_next(s, B);
+        if (A.code != B.code)
+            deep = (A.code & ORD_MAX) > (B.code & ORD_MAX) ? -1 : 1;
+        if (A.at > RUNS && B.at > RUNS && A.node == B.node && A.left == B.left)
+            return deep; // both at one place of one tail node: a single lineage from here
+        const uint32_t m = A.left < B.left ? A.left : B.left;
+        if (!(A.left -= m))
+            rr_step(s, A);
+        if (!(B.left -= m))
+            rr_step(s, B);
     }
+    const bool a_end = !A.code, b_end = !B.code;
+    *inexact = (a_end && A.lost) || (b_end && B.lost);
+    if (a_end && b_end)
+        return a.root && b.root && a.root != b.root ? (a.root < b.root ? -1 : 1) : deep;
+    if ((a_end ? A : B).lost)
+        return deep;
+    *by_lead = true;
+    return a_end ? -1 : 1;
 }
-
-// ccmp_walk with the root rule for lockstep lineages and the memo (src/rdma/rstep.cuh:504-575)
-RD_COLD int ccmp(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b, bool *inexact)
+// the lead hashes of the event that code0 pushed on c would be, then of its ancestors
+RD_HD void lead_hashes(uint64_t *h, uint32_t code0, const Chain &c)
+{
+    h[0] = code0 ? chash(c.hash, code0) : c.hash;
+    for (int i = 1; i <= NANC; i++)
+        h[i] = !code0 ? c.anc[i - 1] : i == 1 ? c.hash : c.anc[i - 2];
+}
+// order_walk behind two shortcuts (src/rdma/rstep.cuh:504-575): lineages in lockstep (equal lead
+// hashes, distinct roots) run in their roots' rank order; and the memo, probed at each depth
+// while the leads agree. A walk decided by leads fills the memo at the depths it passed.
+RD_COLD int order_events(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b, bool *inexact)
 {
     uint64_t ha[NANC + 1], hb[NANC + 1];
-    ha[0] = pa ? chash(a.hash, pa) : a.hash;
-    hb[0] = pb ? chash(b.hash, pb) : b.hash;
-    for (int i = 1; i <= NANC; i++)
-    {
-        ha[i] = pa ? (i == 1 ? a.hash : a.anc[i - 2]) : a.anc[i - 1];
-        hb[i] = pb ? (i == 1 ? b.hash : b.anc[i - 2]) : b.anc[i - 1];
-    }
+    lead_hashes(ha, pa, a);
+    lead_hashes(hb, pb, b);
+    *inexact = false;
     if (a.root && b.root && a.root != b.root && ha[0] == hb[0])
-    {
-        *inexact = false;
         return s.root_rank[a.root - 1] < s.root_rank[b.root - 1] ? -1 : 1;
-    }
-    Cursor A, B;
-    cur_init(s, A, a, pa);
-    cur_init(s, B, b, pb);
-    int eq = 0;
-    for (int j = 0;; j++)
+    RunReader A, B;
+    rr_open(s, A, a, pa);
+    rr_open(s, B, b, pb);
+    int depth = 0; // the depths passed with equal leads
+    for (;; depth++)
     {
-        int m = memo_get(s, ha[j], hb[j]);
-        if (m)
-        {
-            *inexact = false;
+        if (const int m = memo_find(s, ha[depth], hb[depth]))
             return m;
-        }
-        if (j == NANC || A.code == 0 || B.code == 0 || (A.code >> ORD_BITS) != (B.code >> ORD_BITS))
+        if (depth == NANC || !A.code || !B.code || (A.code >> ORD_BITS) != (B.code >> ORD_BITS))
             break;
-        eq = j + 1;
-        if (--A.left == 0)
-            cur_next(s, A);
-        if (--B.left == 0)
-            cur_next(s, B);
+        if (!--A.left)
+            rr_step(s, A);
+        if (!--B.left)
+            rr_step(s, B);
     }
     bool by_lead;
-    int r = ccmp_walk(s, pa, a, pb, b, inexact, &by_lead);
+    const int r = order_walk(s, pa, a, pb, b, inexact, &by_lead);
     if (by_lead && !*inexact)
-        for (int j = 0; j <= eq; j++)
-            memo_put(s, ha[j], hb[j], r);
+        for (int j = 0; j <= depth && j <= NANC; j++)
+            memo_record(s, ha[j], hb[j], r);
     return r;
 }
 
-// the first PRE runs of cpush(code0, ch) (src/rdma/rstep.cuh:577-600), and their order by leads
-// (src/rdma/rstep.cuh:602-624): -1, 1, or 0 when the prefixes do not decide
-constexpr int PRE = 3;
-RD_HD void cprefix(Run *out, uint32_t code0, const Chain &ch)
+// A quick test on the chains' first runs (src/rdma/rstep.cuh:577-624): the newest HEAD runs of the
+// chain code0 would make, compared by leads; 0 when they do not decide
+constexpr int HEAD = 3;
+RD_HD void head_runs(Run *out, uint32_t code0, const Chain &c)
 {
-    if (!code0)
-    {
-        for (int i = 0; i < PRE; i++)
-            out[i] = ch.r[i];
-    }
-    else if (ch.r[0].code == code0)
-    {
-        out[0].code = code0;
-        out[0].cnt = ch.r[0].cnt + 1;
-        for (int i = 1; i < PRE; i++)
-            out[i] = ch.r[i];
-    }
-    else
+    const bool joins = code0 && c.r[0].code == code0;
+    const int shift = code0 && !joins ? 1 : 0;
+    for (int i = shift; i < HEAD; i++)
+        out[i] = c.r[i - shift];
+    if (code0)
     {
         out[0].code = code0;
-        out[0].cnt = 1;
-        for (int i = 1; i < PRE; i++)
-            out[i] = ch.r[i - 1];
+        out[0].cnt = joins ? c.r[0].cnt + 1 : 1;
     }
 }
-RD_HD int pcmp(const Run *a, const Run *b)
+RD_HD int heads_cmp(const Run *a, const Run *b)
 {
-    int ia = 0, ib = 0;
-    uint32_t ra = a[0].cnt, rb = b[0].cnt;
-    while (ia < PRE && ib < PRE)
+    int i = 0, j = 0;
+    uint32_t la = a[0].cnt, lb = b[0].cnt;
+    while (i < HEAD && j < HEAD)
     {
-        uint32_t ca = a[ia].code, cb = b[ib].code;
+        const uint32_t ca = a[i].code, cb = b[j].code;
         if (!ca || !cb)
-            return 0;
-        if (ca >> ORD_BITS != cb >> ORD_BITS)
+            break;
+        if ((ca >> ORD_BITS) != (cb >> ORD_BITS))
             return ca > cb ? -1 : 1;
-        uint32_t m = ra < rb ? ra : rb;
-        ra -= m;
-        rb -= m;
-        if (ra == 0 && ++ia < PRE)
-            ra = a[ia].cnt;
-        if (rb == 0 && ++ib < PRE)
-            rb = b[ib].cnt;
+        const uint32_t m = la < lb ? la : lb;
+        la -= m;
+        lb -= m;
+        if (!la && ++i < HEAD)
+            la = a[i].cnt;
+        if (!lb && ++j < HEAD)
+            lb = b[j].cnt;
     }
     return 0;
 }
-// does (ta, cpush(pa, a)) run before (tb, cpush(pb, b))? Undecided ties keep b (src/rdma/rstep.cuh:635-657)
+// does (ta, the event code pa pushes on a) run before (tb, the one pb pushes on b)? Times first,
+// then the chains; an undecided tie keeps b (src/rdma/rstep.cuh:635-657)
 RD_COLD bool before(const Sim &s, int64_t ta, uint32_t pa, const Chain &a, int64_t tb, uint32_t pb, const Chain &b)
 {
     if (ta != tb)
         return ta < tb;
     bool inexact;
-    int c = ccmp(s, pa, a, pb, b, &inexact);
+    const int r = order_events(s, pa, a, pb, b, &inexact);
     if (inexact)
         count(s, EV_LOST);
-    if (c == 0)
+    if (!r)
         count(s, EV_TIE);
-    return c < 0;
+    return r < 0;
 }
-// a tie first by the prefixes (and the root rule), then by before(): the quick test of
-// src/rdma/rstep.cuh:626-633 that the reference applies to arrivals, TransmitCompletes and sends
+// at one time: the root rule on the events' own lead hashes, then the head runs, then before()
+// (the quick test src/rdma/rstep.cuh:626-633 applies to arrivals, TransmitCompletes and sends)
 RD_COLD bool before_q(const Sim &s, int64_t t, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b)
 {
-    Run xa[PRE], xb[PRE];
-    cprefix(xa, pa, a);
-    cprefix(xb, pb, b);
     const uint64_t ha = pa ? chash(a.hash, pa) : a.hash, hb = pb ? chash(b.hash, pb) : b.hash;
     int r;
     if (a.root && b.root && a.root != b.root && ha == hb)
         r = s.root_rank[a.root - 1] < s.root_rank[b.root - 1] ? -1 : 1;
     else
-        r = pcmp(xa, xb);
-    if (r)
-        return r < 0;
-    return before(s, t, pa, a, t, pb, b);
+    {
+        Run xa[HEAD], xb[HEAD];
+        head_runs(xa, pa, a);
+        head_runs(xb, pb, b);
+        r = heads_cmp(xa, xb);
+    }
+    return r ? r < 0 : before(s, t, pa, a, t, pb, b);
 }
 
 // ------------------------------------------------------------------ frames (src/rdma/rstep.cuh:855-913)
