@@ -62,3 +62,22 @@ def test_dump_routes_fattree8(tmp_path):
     got = out.read_text().splitlines()
     assert got == expected(os.path.join(util.GOLDEN, topo))
     assert len(got) == 128 * 80
+
+
+def test_dump_routes_ignored_without_topology_file(tmp_path):
+    """A FatTree built from --k has no topology file: the reference writes the dump in its graph
+    branch only and runs on, and so does this command line (no file, exit 0 at the hash check)."""
+    name, fl, ref, args = next(c for c in util.cases() if c[0] == "k8_paper-dctcp-base")
+    out = tmp_path / "routes.txt"
+    r = subprocess.run([util.TWINS] + args + [f"--flows={fl}", f"--dump_routes={out}", f"--ecmp_from={ref}", "--hash_check=1"],
+                       cwd=util.GOLDEN, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert not out.exists()
+
+
+def test_empty_worlds_file(tmp_path):
+    w = tmp_path / "worlds.txt"
+    w.write_text("# nothing but a comment\n\n")
+    r = subprocess.run([util.TWINS, "--k=4", f"--worlds={w}", f"--dump_routes={tmp_path / 'r.txt'}"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "no worlds" in r.stderr

@@ -176,8 +176,17 @@ int main(int argc, char **argv)
         }
         hw.push_back(x);
     }
-    // --dump_routes=F: the first world's routes (src/main.cu:466-486)
-    if (!dump_routes.empty() && tw_world_dump_routes(hw[0], dump_routes.c_str()))
+    if (hw.empty())
+    {
+        fprintf(stderr, "%s: no worlds\n", worlds_file.c_str());
+        return 1;
+    }
+    // --dump_routes=F: the first world's routes, written for a topology-file world only and
+    // otherwise ignored, as the reference writes them in its graph branch (src/main.cu:466-486)
+    bool graph = false;
+    for (auto &a : worlds[0].args)
+        graph |= a == "--topo=graph";
+    if (!dump_routes.empty() && graph && tw_world_dump_routes(hw[0], dump_routes.c_str()))
     {
         fprintf(stderr, "%s\n", tw_last_error());
         return 1;
