@@ -44,6 +44,8 @@ typedef struct trd_pack_cfg
     int32_t hosts_per_wave; /* hosts side by side on a wave's lanes (0: batches of up to 8 worlds 8, or 16 where a world's
                                work items would outnumber a 32-workgroup team's waves; larger batches 64) */
     int32_t tail_exact;  /* see tail_log2 */
+    int32_t serial;      /* 1: every switch slot runs its events in order on one lane, never the per-port parallel safe
+                            path (the reference's --serial, src/rdma/rmain.cu:1090-1091, 1949); results are the same */
 } trd_pack_cfg;
 
 /* Worlds side by side on the GPU (the reference's --batch): one slot (the shortest link delay of

@@ -180,3 +180,18 @@ def test_gpu_lost_tails_rerun():
         assert b.fct_text(i) == rdma_util.ref_fct(n), n
     assert rdma_util.sum_counters([b.summary_line()]) == rdma_util.sum_counters([by[n][5] for n in names])
     b.close()
+
+
+def test_gpu_serial_flag():
+    """The reference's --serial on the GPU: every switch slot on one lane in event order (the safe
+    per-port path off); 20 short reference jobs in one batch stay identical, counters included."""
+    from twins_amd import rdma
+    I = rdma_util.INPUTS
+    sel = rdma_util.jobs(max_ref_s=1.0)[:20]
+    print(f"rdma serial batch: {len(sel)} worlds", flush=True)
+    b = rdma.Batch([rdma.World(rdma_util.CONFIG, f"{I}/{t}", f"{I}/{f}", kv) for _, t, f, kv, _, _ in sel], serial=True)
+    b.run()
+    bad = [n for i, (n, *_) in enumerate(sel) if b.fct_text(i) != rdma_util.ref_fct(n)]
+    assert not bad, bad
+    assert rdma_util.sum_counters([b.summary_line()]) == rdma_util.sum_counters([j[5] for j in sel])
+    b.close()

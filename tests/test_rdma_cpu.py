@@ -202,3 +202,18 @@ def test_cli_tail_set_never_grows(cpu_bin, tmp_path):
     assert "running again" not in r.stderr and "a larger --tail helps" in r.stderr
     r = run_cli(cpu_bin, "ft4.topo", "inc1.csv", ["CC_MODE=1", "--tail=3"], tmp_path / "y.fct")
     assert r.returncode == 1 and "from 4 to 30" in r.stderr
+
+
+SERIAL = [j for j in rdma_util.jobs(max_ref_s=1.5) if "PFC" in j[0] or "b1" in j[0] or "incast" in j[0]][:8]
+
+
+@pytest.mark.parametrize("job", SERIAL, ids=[j[0] for j in SERIAL])
+def test_cli_serial_flag(cpu_bin, job, tmp_path):
+    """--serial (src/rdma/rmain.cu:1090-1091, 1949): every switch slot in event order on one lane,
+    never the per-port safe path; completions and counters stay the reference's."""
+    name, topo, flows, kv, _, counters = job
+    out = tmp_path / (name + ".fct")
+    r = run_cli(cpu_bin, topo, flows, kv + ["--serial"], out)
+    assert r.returncode == 0, r.stderr
+    assert out.read_text() == rdma_util.ref_fct(name)
+    assert [l for l in r.stdout.splitlines() if l.startswith("drops ")][-1] == counters

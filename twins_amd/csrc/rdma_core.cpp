@@ -595,6 +595,7 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
     b.W = W, b.H = H, b.S = S, b.F = F;
     b.fbase = fbase;
     s.H = H, s.S = S, s.F = F, s.W = W, s.L = L, s.D = D;
+    s.serial = cfg.serial ? 1 : 0;
     // the batch's stop: the latest world's (src/rdma/rmain.cu:1290)
     int64_t max_slots = 0;
     for (int i = 0; i < W; i++)

@@ -346,6 +346,7 @@ struct TxTrace
 struct Sim
 {
     int H, S, F, W, L, R;
+    int serial; // every switch slot on the serial path (--serial)
     int64_t D; // the slot (ns)
     const Params RD_G *Pw;
     const World RD_G *wd;
@@ -2370,7 +2371,7 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
         }
     RD_WAVE_SYNC();
     // is the slot safe? (every lane decides alike)
-    bool safe = total <= LK;
+    bool safe = !s.serial && total <= LK;
     if (safe)
     {
         uint64_t sum = 0;

@@ -110,7 +110,9 @@ int main(int argc, char **argv)
             fprintf(stderr, "%s: not modelled in this build\n", k.c_str());
             return 1;
         }
-        else if (k == "--serial" || k == "--spread" || k == "--win" || k == "--gwin" || k == "--persist" || k.compare(0, 6, "--rank") == 0 ||
+        else if (k == "--serial")
+            pc.serial = 1; // every switch slot in order on one lane (src/rdma/rmain.cu:1090-1091, 1949)
+        else if (k == "--spread" || k == "--win" || k == "--gwin" || k == "--persist" || k.compare(0, 6, "--rank") == 0 ||
                  k == "--dcap") // (--dcap sizes DAG message buffers; DAG flow files are refused)
             continue;
         else if (k.compare(0, 2, "--") != 0 && eq != string::npos)

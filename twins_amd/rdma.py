@@ -17,7 +17,7 @@ INEXACT = BAD + (EV_LOST,)  # a result that is not known to be exact
 
 class PackCfg(ctypes.Structure):
     _fields_ = [("q3cap", ctypes.c_int32), ("tail_log2", ctypes.c_int32), ("team_blocks", ctypes.c_int32),
-                ("hosts_per_wave", ctypes.c_int32), ("tail_exact", ctypes.c_int32)]
+                ("hosts_per_wave", ctypes.c_int32), ("tail_exact", ctypes.c_int32), ("serial", ctypes.c_int32)]
 
 
 class RunStats(ctypes.Structure):
@@ -108,9 +108,9 @@ class Batch:
 
     TAIL_MAX = 27  # the largest ring run_exact() asks for (the reference's rerun limit)
 
-    def __init__(self, worlds, q3cap=0, tail_log2=0, team_blocks=0, hosts_per_wave=0, tail_exact=False):
+    def __init__(self, worlds, q3cap=0, tail_log2=0, team_blocks=0, hosts_per_wave=0, tail_exact=False, serial=False):
         self.worlds = list(worlds)
-        self._cfg = (q3cap, tail_log2, team_blocks, hosts_per_wave, 1 if tail_exact else 0)
+        self._cfg = (q3cap, tail_log2, team_blocks, hosts_per_wave, 1 if tail_exact else 0, 1 if serial else 0)
         self._h = ctypes.c_void_p()
         self._create()
         _track(self, 1)
@@ -141,8 +141,8 @@ class Batch:
             if not self.counters()[EV_LOST] or lg >= self.TAIL_MAX:
                 return st, reruns
             self.close()
-            q3cap, _, team_blocks, hpw, _ = self._cfg
-            self._cfg = (q3cap, 26 if lg < 26 else self.TAIL_MAX, team_blocks, hpw, 1)
+            q3cap, _, team_blocks, hpw, _, serial = self._cfg
+            self._cfg = (q3cap, 26 if lg < 26 else self.TAIL_MAX, team_blocks, hpw, 1, serial)
             self._create()
             reruns += 1
 
