@@ -240,14 +240,24 @@ def serve_leg(a):
     t_start = time.perf_counter()
     last_log = t_start
     inexact = 0
+    slowest = [0.0, ""]  # the longest host call since the last progress line, and which
+
+    def timed(what, fn, *args):
+        c0 = time.perf_counter()
+        r = fn(*args)
+        d = time.perf_counter() - c0
+        if d > slowest[0]:
+            slowest[0], slowest[1] = d, what
+        return r
     while len(finish) < n:
         now = time.perf_counter() - t_start
         while submitted < n and reqs[submitted][0] <= now:
-            sid[submitted] = (rsrv if is_rdma[submitted] else srv).submit(worlds[submitted])
+            srv_ = rsrv if is_rdma[submitted] else srv
+            sid[submitted] = timed("rdma submit" if is_rdma[submitted] else "tcp submit", srv_.submit, worlds[submitted])
             arrival[submitted] = reqs[submitted][0]
             submitted += 1
-        st = srv.poll()
-        rst = rsrv.poll() if rsrv else None
+        st = timed("tcp poll", srv.poll)
+        rst = timed("rdma poll", rsrv.poll) if rsrv else None
         for i in list(arrival):
             if i in finish:
                 continue
@@ -256,7 +266,7 @@ def serve_leg(a):
                 if s_ in (twins_amd.REQ_DONE, twins_amd.REQ_FAILED):
                     if s_ == twins_amd.REQ_FAILED:
                         raise RuntimeError(f"RDMA request {i} failed")
-                    start, fct, ev = rsrv.result(sid[i])
+                    start, fct, ev = timed("rdma result", rsrv.result, sid[i])
                     finish[i] = time.perf_counter() - t_start
                     events += ev[38]
                     inexact += any(ev[j] for j in rdma.INEXACT)
@@ -265,7 +275,7 @@ def serve_leg(a):
             else:
                 s_ = srv.state(sid[i])
                 if s_ in (twins_amd.REQ_DONE, twins_amd.REQ_FAILED):
-                    est, done, ev = srv.result(sid[i])
+                    est, done, ev = timed("tcp result", srv.result, sid[i])
                     finish[i] = time.perf_counter() - t_start
                     events += ev[26]
                     if i % a.serve_check == 0:
@@ -276,9 +286,11 @@ def serve_leg(a):
             held = sum(worlds[i].info.device_bytes for i in arrival if not is_rdma[i] and i not in finish and
                        srv.state(sid[i]) != twins_amd.REQ_WAITING)
             log(f"serve: {now:.0f} s, {submitted} submitted, {len(finish)} done; TCP resident {st.resident}, waiting {st.waiting}, "
-                f"arena {st.arena_used / 2**30:.1f}/{st.arena_bytes / 2**30:.1f} GiB (admitted worlds {held / 2**30:.1f} GiB)" +
-                (f"; RDMA resident {rst.resident}, waiting {rst.waiting}, arena {rst.arena_used / 2**30:.1f}/{rst.arena_bytes / 2**30:.1f} GiB"
-                 if rst else ""))
+                f"done {st.done}, relaunches {st.relaunches}, arena {st.arena_used / 2**30:.1f}/{st.arena_bytes / 2**30:.1f} GiB "
+                f"(admitted worlds {held / 2**30:.1f} GiB)" +
+                (f"; RDMA resident {rst.resident}, waiting {rst.waiting}, launches {rst.launches}, arena {rst.arena_used / 2**30:.1f}/"
+                 f"{rst.arena_bytes / 2**30:.1f} GiB" if rst else "") + f"; slowest host call {slowest[1]} {slowest[0]:.3f} s")
+            slowest[0], slowest[1] = 0.0, ""
         time.sleep(0.0005)
     wall = max(finish.values()) - reqs[0][0]
     srv.close()

@@ -43,7 +43,7 @@ template <int RUNS>
 static void run_cpu(BatchT<RUNS> &B, int64_t budget)
 {
     typedef Model<RUNS> Md;
-    std::vector<int> lds(64 * 33 + 64 * 3 + 64 + 64);
+    std::vector<int> lds(Md::SW_LDS_TOTAL);
     for (int wq = 0; wq < B.nwq; wq++)
     for (int qi = B.queue_next[wq]; qi < B.wq_len[wq]; qi = ++B.queue_next[wq])
     {

@@ -103,8 +103,10 @@ def test_solved_plan_is_valid_and_not_worse_than_one_workgroup_each(seed):
 
 def test_saturation_counts_work_items():
     s = autotune.WorldShape(info(16, 20, 4))
-    # k=4 FatTree on one workgroup of 4 waves: 9 entities per item -> 2 host items, 3 switch items
-    assert s.items(1, 4) == 2 + 3
+    # k=4 FatTree on one workgroup of 4 waves: the fewest hosts per item whose items fit the 4
+    # waves is 10 (2 host items, 2 switch items of 10 switches)
+    assert s.items(1, 4) == 2 + 2
+    assert s.items_at(9) == 2 + 3
     # 36 entities: one per wave from 9 workgroups of 4 waves (team sizes are powers of two: 16)
     assert s.saturation(4) == 16
     big = autotune.WorldShape(info(1024, 320, 16))

@@ -39,13 +39,20 @@ class WorldShape:
         self.k = max(1, info.ports_per_switch)
         self.flows = info.flows
 
+    def items_at(self, per):
+        """Work items of one slot with `per` hosts per host item (and min(per, 64 / k) switches per
+        switch item)."""
+        sp = min(per, max(1, 64 // self.k))
+        return -(-self.hosts // per) + -(-self.switches // sp)
+
     def items(self, T, waves):
         """Work items of one slot for a team of T workgroups of `waves` wavefronts, as the kernel
-        deals them: about one per wave, ceil((hosts + switches) / (T waves)) hosts or switches each
-        (at most 64 hosts, 64 / k switches)."""
-        per = max(1, min(64, -(-(self.hosts + self.switches) // (T * waves))))
-        spw = min(per, max(1, 64 // self.k))
-        return -(-self.hosts // per) + -(-self.switches // spw)
+        deals them (slot_items in tcp_state.h): the fewest hosts per item whose items do not
+        outnumber the team's waves, at most 64."""
+        per = 1
+        while per < 64 and self.items_at(per) > T * waves:
+            per += 1
+        return self.items_at(per)
 
     def saturation(self, waves):
         """Team size beyond which more workgroups add nothing: one host or switch per item."""

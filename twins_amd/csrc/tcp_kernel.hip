@@ -27,7 +27,7 @@ namespace tw
 // per wave, for the item it runs: a switch item's rows, in_win, base, cnt; a host item's slots for
 // the connection records of an event, one SndState and one RcvState per lane. A wave runs its
 // items one after another, so the two share one region.
-constexpr int LDS_SWITCH = 64 * 33 + 64 * 3 + 64 + 64;
+constexpr int LDS_SWITCH = Model<9>::SW_LDS_TOTAL;
 constexpr int LDS_SND = 0, LDS_RCV = LDS_SND + 64 * (int)(sizeof(SndState) / 4);
 constexpr int LDS_RECORDS = LDS_RCV + 64 * (int)(sizeof(RcvState) / 4);
 constexpr int LDS_PER_WAVE = LDS_RECORDS > LDS_SWITCH ? LDS_RECORDS : LDS_SWITCH;
@@ -44,19 +44,6 @@ __device__ __forceinline__ T *uniform_ptr(T *p)
     const uint64_t v = (uint64_t)p;
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
     return (T *)((uint64_t)hi << 32 | lo);
-}
-
-// Work items when the plan leaves them to the world: about one item per wave of the team, i.e.
-// ceil((hosts + switches) / waves) hosts side by side in a host item and as many switches (at
-// most 64 / k) in a switch item. Fewer entities per wave cost lanes but save the divergence of
-// unlike event loops run in lockstep, which dominates a slot's time.
-__device__ __forceinline__ void auto_items(const WorldDesc &W, int T, int nwaves, int *hpw, int *spw)
-{
-    const int tw = T * nwaves, n = W.H + W.S;
-    int per = (n + tw - 1) / tw;
-    per = per < 1 ? 1 : per > 64 ? 64 : per;
-    *hpw = per;
-    *spw = per < 64 / W.k ? per : 64 / W.k;
 }
 
 // The world loop of both persistent kernels: SRV = false takes worlds from a batch's queues
@@ -193,7 +180,7 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
         const WorldDesc &W = B.worlds[w];
         int hpw = P.hpw, spw = 64 / W.k;
         if (hpw <= 0)
-            auto_items(W, T, nwaves, &hpw, &spw);
+            slot_items(W.H, W.S, W.k, T * nwaves, &hpw, &spw);
         WorldRun *R = B.run + w;
         const int k = W.k;
         const int host_items = (W.H + hpw - 1) / hpw, items = host_items + (W.S + spw - 1) / spw;

@@ -173,25 +173,10 @@ struct Model
             if (sent)
                 count(R, EV_PKT_EVENTS, (unsigned long long)sent);
         }
-        // the event that starts a frame: the previous frame's TransmitComplete when the frame had
+        // Starts, in order, the frames whose transmission begins by `until` within the slot. The
+        // event that starts a frame is the previous frame's TransmitComplete when the frame had
         // to wait for the link (its chain: the last transmission's, one more run), else the
-        // frame's own enqueue (a link that never sent counts as idle)
-        __device__ __forceinline__ void start_event(Frame &o, int64_t start)
-        {
-            const bool waited = o.ts <= link_free && link_free > last_start;
-            if (waited)
-            {
-                chain_child(last, (uint32_t)((link_free - last_start) / PS_PER_BYTE), last);
-                chain_store(o.chain, last);
-            }
-            else
-            {
-                chain_load(last, o.chain);
-                idle_start = start;
-            }
-            last_start = start;
-        }
-        // starts, in order, the frames whose transmission begins by `until` within the slot
+        // frame's own enqueue (a link that never sent counts as idle).
         __device__ void advance(int64_t until)
         {
             for (; size > 0; size--)
@@ -201,11 +186,22 @@ struct Model
                     count(R, EV_TXCAP);
                     return;
                 }
-                Frame o = ring[head];
-                const int64_t start = o.ts > link_free ? o.ts : link_free;
+                const Frame &f = ring[head];
+                const int64_t start = f.ts > link_free ? f.ts : link_free;
                 if (start >= slot_end || start > until)
                     return;
-                start_event(o, start);
+                Frame o = f;
+                if (f.ts <= link_free && link_free > last_start)
+                {
+                    chain_child(last, (uint32_t)((link_free - last_start) / PS_PER_BYTE), last);
+                    chain_store(o.chain, last);
+                }
+                else
+                {
+                    chain_load(last, o.chain);
+                    idle_start = start;
+                }
+                last_start = start;
                 link_free = start + (int64_t)o.len * PS_PER_BYTE;
                 o.ts = link_free + PROP_PS; // its arrival at the far end
                 out[sent++] = o;
@@ -2067,6 +2063,10 @@ struct Model
     // (src/step.cuh:2706-2886). The k lanes of one switch within one wavefront, lane q = port q;
     // phases separated by wave_sync(). LDS per switch: rows[k][k+1] counts, in_win[k][3], base[k],
     // cnt[k]; per-wave global scratch holds the arrivals sorted per egress.
+    // a switch item's LDS: per lane rows (33 ints), in_win (3), base, cnt, then (after the
+    // spray merge's 64) per lane SORT_LDS arrival times and refs for the egress sort
+    static constexpr int SW_LDS_INTS = 64 * 33 + 64 * 3 + 64 + 64, SORT_LDS = 24;
+    static constexpr int SW_LDS_TOTAL = SW_LDS_INTS + 64 * SORT_LDS * 3;
     struct SwLds
     {
         int *rows, *in_win, *base, *cnt;
@@ -2217,15 +2217,31 @@ struct Model
     {
         if (!x.active)
             return 0;
-        const int64_t E0 = (t + 1) * SLOT_PS;
         const int n = L.cnt[x.q], b0 = L.base[x.q];
         int64_t *ts = x.scr_ts + b0;
         uint32_t *ref = x.scr_ref + b0;
         auto frame_at = [&](uint32_t r) -> const Frame & { return lane_input(B, W, x, t, L, (int)(r >> 16)).at((int)(r & 0xFFFF)); };
+        if (n <= SORT_LDS)
+        {
+            // a short list (the usual case) is sorted in the lane's LDS slots: one round of loads
+            // instead of a device-memory round trip per comparison and move
+            const int lane = (int)(L.cnt - L.wave - (64 * 33 + 64 * 3 + 64)) + x.q;
+            int64_t *lts = reinterpret_cast<int64_t *>(L.wave + SW_LDS_INTS) + lane * SORT_LDS;
+            uint32_t *lref = reinterpret_cast<uint32_t *>(L.wave + SW_LDS_INTS + 2 * 64 * SORT_LDS) + lane * SORT_LDS;
+            for (int i = 0; i < n; i++)
+            {
+                lts[i] = ts[i];
+                lref[i] = ref[i];
+            }
+            ts = lts;
+            ref = lref;
+        }
+        // an insertion sort by (time, frame_before): chain comparison is no strict weak order
+        // when chains are truncated, and this gives the reference's order in every case
         for (int i = 1; i < n; i++)
         {
-            int64_t xv = ts[i];
-            uint32_t r = ref[i];
+            const int64_t xv = ts[i];
+            const uint32_t r = ref[i];
             int j = i - 1;
             while (j >= 0 && (ts[j] > xv || (ts[j] == xv && frame_before(R, frame_at(r), frame_at(ref[j])))))
             {
@@ -2248,7 +2264,7 @@ struct Model
             chain_store(p.chain, c);
             eg.enqueue(p, p.ts);
         }
-        eg.advance(E0 - 1);
+        eg.advance((t + 1) * SLOT_PS - 1);
         eg.store(t, rt);
         return (eg.sent > 0 || eg.size > 0) ? 1 : 0;
     }

@@ -22,8 +22,35 @@
 #define TW_G
 #endif
 
+#if defined(__HIPCC__)
+#define TW_HD __host__ __device__
+#else
+#define TW_HD
+#endif
+
 namespace tw
 {
+
+// A slot's work items on a team's waves: host items of `per` hosts side by side (one serial
+// event loop per lane) and switch items of min(per, 64 / k) switches (a lane per port). The work
+// items of a world with `per` hosts per item:
+TW_HD inline int slot_items_at(int H, int S, int k, int per)
+{
+    const int smax = k > 0 && 64 / k > 0 ? 64 / k : 1, sp = per < smax ? per : smax;
+    return (H + per - 1) / per + (S + sp - 1) / sp;
+}
+// the fewest hosts per item whose items do not outnumber the waves (a wave then runs one item per
+// slot; fewer entities per item cost lanes but save the divergence of unlike event loops), at most
+// 64 when even that does not fit
+TW_HD inline void slot_items(int H, int S, int k, int waves, int *hpw, int *spw)
+{
+    int per = 1;
+    while (per < 64 && slot_items_at(H, S, k, per) > waves)
+        per++;
+    const int smax = k > 0 && 64 / k > 0 ? 64 / k : 1;
+    *hpw = per;
+    *spw = per < smax ? per : smax;
+}
 
 constexpr int64_t SLOT_PS = 1000000, PROP_PS = 1000000, PS_PER_BYTE = 80;
 constexpr uint32_t MSS = 1460;
