@@ -9,7 +9,8 @@
 
 namespace rd
 {
-static SwWork g_work; // a switch's slot workspace (LDS on the GPU)
+static SwWork g_work;      // a switch's slot workspace (LDS on the GPU)
+static SwWorkWide g_wide;  // the workspace of a switch of KS + 1 to KW ports (device memory on the GPU)
 // TWINS_RDMA_CPU_SERIAL=1: every switch on the serial path of wide switches (switch_step), so that
 // the reference's jobs (switches of at most 32 ports) check that path too
 static const bool g_serial = getenv("TWINS_RDMA_CPU_SERIAL") != nullptr;
@@ -42,8 +43,10 @@ int launch_worlds(const LaunchCfg *cfg, int, int, hipStream_t)
                     const int x = wd.s0 + (e - wd.nh);
                     bool busy = false;
                     if (s.sw_busy[x] || switch_has_input(s, x, r.slot))
-                        busy = s.si[x].deg <= KS && !g_serial ? switch_slot(s, x, r.slot, pend, wd.pend_n, g_work, 0, 1)
-                                                              : switch_step(s, x, r.slot, pend, wd.pend_n);
+                        busy = g_serial              ? switch_step(s, x, r.slot, pend, wd.pend_n)
+                               : s.si[x].deg <= KS ? switch_slot(s, x, r.slot, pend, wd.pend_n, g_work, 0, 1)
+                               : s.si[x].deg <= KW ? switch_slot(s, x, r.slot, pend, wd.pend_n, g_wide, 0, 1)
+                                                   : switch_step(s, x, r.slot, pend, wd.pend_n);
                     s.sw_busy[x] = busy ? 1 : 0;
                     act |= busy ? 1u : 0u;
                 }

@@ -489,7 +489,9 @@ struct trd_batch
     unsigned *qnext_d = nullptr;
     rd::TeamState *teams_d = nullptr;
     rd::LaunchCfg *cfg_d = nullptr;
+    rd::SwWorkWide *wide_d = nullptr; // a switch workspace per wave of a launch (worlds with switches of KS + 1 to KW ports)
     int max_teams = 0;
+    int kmax = 0; // the most ports of a switch
     int tail_lg = 0; // the largest world's tail ring: 2^tail_lg nodes
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -642,6 +644,7 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
             q.rt_off = (uint32_t)rt.size();
             q.pbase = pb;
             q.deg = w.deg[w.H + x];
+            b.kmax = std::max(b.kmax, (int)q.deg);
             for (int h = 0; h < w.H; h++)
             {
                 rd::Route r = w.rt[(size_t)x * w.H + h];
@@ -880,6 +883,8 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
     TRY(b.alloc(&b.qnext_d, 1));
     TRY(b.alloc(&b.teams_d, cap));
     TRY(b.alloc(&b.cfg_d, 1));
+    if (b.kmax > rd::KS && b.kmax <= rd::KW)
+        TRY(b.alloc(&b.wide_d, (size_t)cap * rd::LAUNCH_WAVES));
     return TW_OK;
 }
 
@@ -969,6 +974,7 @@ extern "C" int trd_batch_run(trd_batch *b, trd_run_stats *st)
         L.qnext = b->qnext_d;
         L.teams = b->teams_d;
         L.sims = b->sims_d; // (queue entries: batch 0's worlds)
+        L.wide = b->wide_d;
         if (hipMemcpyAsync(b->queue_d, queue.data(), sizeof(int) * queue.size(), hipMemcpyHostToDevice, b->st) != hipSuccess ||
             hipMemcpyAsync(b->cfg_d, &L, sizeof L, hipMemcpyHostToDevice, b->st) != hipSuccess ||
             hipMemsetAsync(b->qnext_d, 0, sizeof(unsigned), b->st) != hipSuccess ||
@@ -1176,6 +1182,7 @@ struct RSrvLane
     rd::TeamState *teams_d = nullptr;
     rd::LaunchCfg *cfg_d = nullptr;
     const rd::Sim **sims_d = nullptr;
+    rd::SwWorkWide *wide_d = nullptr; // a switch workspace per wave (switches of KS + 1 to KW ports)
     vector<int> queue_h, launched;
     vector<const rd::Sim *> sims_h;
     rd::LaunchCfg L = {};
@@ -1204,7 +1211,7 @@ struct trd_server
             delete r.b;
         for (auto &l : lane)
         {
-            for (void *p : {(void *)l.queue_d, (void *)l.qnext_d, (void *)l.teams_d, (void *)l.cfg_d, (void *)l.sims_d})
+            for (void *p : {(void *)l.queue_d, (void *)l.qnext_d, (void *)l.teams_d, (void *)l.cfg_d, (void *)l.sims_d, (void *)l.wide_d})
                 if (p)
                     (void)hipFree(p);
             if (l.done_ev)
@@ -1258,7 +1265,8 @@ extern "C" int trd_server_create(const trd_server_cfg *cfg, trd_server **out)
             return fail(TW_EDEVICE, "hipStreamCreate failed");
         if (hipMalloc((void **)&l.queue_d, sizeof(int) * RSRV_CAP) != hipSuccess || hipMalloc((void **)&l.qnext_d, sizeof(unsigned)) != hipSuccess ||
             hipMalloc((void **)&l.teams_d, sizeof(rd::TeamState) * l.blocks) != hipSuccess ||
-            hipMalloc((void **)&l.cfg_d, sizeof(rd::LaunchCfg)) != hipSuccess || hipMalloc((void **)&l.sims_d, sizeof(void *) * RSRV_CAP) != hipSuccess)
+            hipMalloc((void **)&l.cfg_d, sizeof(rd::LaunchCfg)) != hipSuccess || hipMalloc((void **)&l.sims_d, sizeof(void *) * RSRV_CAP) != hipSuccess ||
+            hipMalloc((void **)&l.wide_d, sizeof(rd::SwWorkWide) * (size_t)l.blocks * rd::LAUNCH_WAVES) != hipSuccess)
             return fail(TW_ENOMEM, "hipMalloc failed");
     }
     *out = s.release();
@@ -1351,6 +1359,7 @@ static int rsrv_launch(trd_server &s, int c)
     L.qnext = l.qnext_d;
     L.teams = l.teams_d;
     L.sims = l.sims_d;
+    L.wide = l.wide_d;
     const int nteams = std::max(1, std::min(L.nq, l.blocks / L.T));
     if (hipMemcpyAsync(l.queue_d, l.queue_h.data(), sizeof(int) * L.nq, hipMemcpyHostToDevice, l.ks) != hipSuccess ||
         hipMemcpyAsync(l.sims_d, l.sims_h.data(), sizeof(void *) * L.nq, hipMemcpyHostToDevice, l.ks) != hipSuccess ||

@@ -19,6 +19,7 @@ struct TeamState
 // an entry is sim << QSHIFT | world: world `world` of the batch whose Sim is sims[sim] (a launch can
 // run the worlds of several separately packed batches: the serving loop's requests)
 constexpr int QSHIFT = 20;
+constexpr int LAUNCH_WAVES = 4; // waves of a launch's 256-thread workgroups
 struct LaunchCfg
 {
     int T, nq, xcd_local, hpw; // hpw: hosts of a host item (side by side on a wave's lanes, at most 64)
@@ -26,6 +27,7 @@ struct LaunchCfg
     unsigned *qnext;
     TeamState *teams;
     const Sim *const *sims;
+    SwWorkWide *wide; // a device-memory switch workspace per wave of the launch (null: no switch of KS + 1 to KW ports)
 };
 // the workgroups of 256 threads the GPU holds at once (-1: unknown)
 int launch_capacity(int threads);

@@ -15,7 +15,7 @@ namespace rd
 {
 constexpr long long LAUNCH_TICKS = 100000000;          // a launch's time: 1 s of the 100 MHz constant clock
 constexpr unsigned long long WAIT_TICKS = 2000000000ull; // a barrier wait gives up after 20 s
-constexpr int WAVES = 4;                               // waves of a 256-thread workgroup
+constexpr int WAVES = LAUNCH_WAVES;                    // waves of a 256-thread workgroup
 
 __device__ __forceinline__ unsigned ld_acquire(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
 
@@ -155,6 +155,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
                     {
                         if (deg <= KS)
                             busy = switch_slot(s, x, sl, pend, wd.pend_n, work[wv], lane, 64);
+                        else if (deg <= KW && L.wide)
+                            busy = switch_slot(s, x, sl, pend, wd.pend_n, L.wide[(size_t)blockIdx.x * WAVES + wv], lane, 64);
                         else
                         {
                             int b = 0;

@@ -2156,16 +2156,19 @@ RD_HD bool switch_step(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
 // to it, merged from the ingress ports in event order, and its TransmitCompletes) run on a lane of
 // their own; or lane 0 runs all events in order. The same code runs on the CPU with one lane, port
 // after port (egress ports of a safe slot are independent, so the order of their runs is free).
-constexpr int KS = 32;  // ports of a switch whose slot runs from the workspace
+constexpr int KS = 32;  // ports of a switch whose slot runs from the LDS workspace
+constexpr int KW = 64;  // ports of a switch whose slot runs from a device-memory workspace (a lane per port of wave64)
 constexpr int LK = 512; // arrivals of a slot a safe slot takes
-struct SwWork
+template <int K>
+struct SwWorkT
 {
-    Port pt[KS];
-    PortRun w[KS];
-    PortCfg pc[KS]; // (a copy: no load of a port's configuration per event)
-    int cnt[KS], first[KS];
-    uint32_t abytes[KS], ing0[KS];
-    int16_t mc[KS][KS]; // [egress][ingress]: the ingress's next arrival for the egress (-1: none)
+    static constexpr int ports = K;
+    Port pt[K];
+    PortRun w[K];
+    PortCfg pc[K]; // (a copy: no load of a port's configuration per event)
+    int cnt[K], first[K];
+    uint32_t abytes[K], ing0[K];
+    int16_t mc[K][K]; // [egress][ingress]: the ingress's next arrival for the egress (-1: none)
     int16_t onext[LK];  // an arrival's next one of the same ingress and egress (-1: none)
     int64_t ats[LK];
     int8_t own[LK];     // the egress port it belongs to in a safe slot (-1: none)
@@ -2174,6 +2177,8 @@ struct SwWork
     unsigned act;
     unsigned nout;
 };
+typedef SwWorkT<KS> SwWork;     // in LDS, a wave's
+typedef SwWorkT<KW> SwWorkWide; // in device memory, a wave's (switches of KS + 1 to KW ports)
 
 #if defined(__HIP_DEVICE_COMPILE__)
 #define RD_WAVE_SYNC()                                                                                                            \
@@ -2191,7 +2196,8 @@ struct SwWork
 #endif
 
 // the arrival code and chain of arrival j (of ingress p) in the workspace
-RD_HD const LFrame &sw_arr(const Sim &s, const SwWork &k, int p, int j) { return s.msg[k.w[p].abase + (uint64_t)(j - k.first[p])]; }
+template <typename WK>
+RD_HD const LFrame &sw_arr(const Sim &s, const WK &k, int p, int j) { return s.msg[k.w[p].abase + (uint64_t)(j - k.first[p])]; }
 RD_HD uint32_t sw_acode(const Sim &s, const SwSlot &c, int p, const LFrame &f)
 {
     const PortRun &r = c.w[p];
@@ -2212,21 +2218,22 @@ RD_HD uint32_t sw_acode(const Sim &s, const SwSlot &c, int p, const LFrame &f)
 #define RD_PROF_T(v)
 #define RD_PROF_ADD(i, d)
 #endif
-RD_HD void sw_egress_events(const Sim &s, SwSlot &c, SwWork &k, int o, int deg, int64_t t1)
+template <typename WK>
+RD_HD void sw_egress_events(const Sim &s, SwSlot &c, WK &k, int o, int deg, int64_t t1)
 {
     // the ingress ports with arrivals left for o (bit p: port p)
-    uint32_t live = 0;
+    uint64_t live = 0;
     for (int p = 0; p < deg; p++)
         if (k.mc[o][p] >= 0)
-            live |= 1u << p;
+            live |= 1ull << p;
     for (;;)
     {
         RD_PROF_T(q0);
         // the next arrival for o: the ingress heads in port order, the earlier (time, chains) kept
         int bp = -1, kb = -1;
-        for (uint32_t m = live; m; m &= m - 1)
+        for (uint64_t m = live; m; m &= m - 1)
         {
-            const int p = __builtin_ctz(m);
+            const int p = __builtin_ctzll(m);
             const int j = k.mc[o][p];
             if (bp < 0)
             {
@@ -2276,7 +2283,7 @@ RD_HD void sw_egress_events(const Sim &s, SwSlot &c, SwWork &k, int o, int deg, 
         }
         k.mc[o][bp] = k.onext[kb];
         if (k.onext[kb] < 0)
-            live &= ~(1u << bp);
+            live &= ~(1ull << bp);
         c.now = ta;
         c.cur_src = &fa->c;
         c.cur_code = acode;
@@ -2300,7 +2307,8 @@ RD_HD void sw_egress_events(const Sim &s, SwSlot &c, SwWork &k, int o, int deg, 
 
 // One switch's slot by lane `lane` of `nlanes` (a wave on the GPU, one lane on the CPU); ports
 // p = lane, lane + nlanes, ... are this lane's. Returns (on every lane) whether the switch is busy.
-RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pend_n, SwWork &k, int lane, int nlanes)
+template <typename WK>
+RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pend_n, WK &k, int lane, int nlanes)
 {
     const SwInfo &si = s.si[x];
     const int deg = si.deg;
