@@ -486,7 +486,6 @@ def main():
     ap.add_argument("--rdma", type=int, default=1, help="also run config 3, the RDMA model (the \"rdma\" object)")
     ap.add_argument("--refspeed", type=int, default=1, help="also time the reference's published single-world workloads")
     ap.add_argument("--ref_runs", type=int, default=3, help="runs per reference workload (the best is reported)")
-    ap.add_argument("--serve_first", type=int, default=0, help="run the serving leg before the sweep")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU (the worlds of a sweep are independent and would be "
@@ -570,11 +569,9 @@ def main():
             "sample": "the oracle (sequential CPU restatement of the model) on the same 3 worlds, event loop only",
         },
     }
-    if a.serve and a.serve_first:
-        line["serve"] = serve_leg(a)
     if a.sweep:
         line["sweep"] = sweep_leg(a)
-    if a.serve and not a.serve_first:
+    if a.serve:
         line["serve"] = serve_leg(a)
     if a.rdma:
         line["rdma"] = rdma_leg(a)

@@ -93,7 +93,6 @@ hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorNotSuppo
 hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
 // the serving loop needs a GPU: its calls are refused here
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = (hipStream_t)malloc(8); return hipSuccess; }
-hipError_t hipExtStreamCreateWithCUMask(hipStream_t *s, uint32_t, const uint32_t *) { *s = (hipStream_t)malloc(8); return hipSuccess; }
 hipError_t hipStreamQuery(hipStream_t) { return hipSuccess; }
 hipError_t hipHostMalloc(void **, size_t, unsigned) { return hipErrorNotSupported; }
 hipError_t hipHostFree(void *) { return hipSuccess; }

@@ -6,28 +6,6 @@
 #include <cstdint>
 #include <map>
 #include <mutex>
-#include <vector>
-
-#include <hip/hip_runtime.h>
-
-// A stream for a serving loop's long-running kernels, on a hardware queue of its own. HIP maps
-// streams onto at most GPU_MAX_HW_QUEUES hardware queues (4 by default) and, beyond that, shares
-// one queue between streams; a queue runs its kernels in order, so work of a stream that shares
-// the persistent kernel's queue waits until that kernel leaves, and a host that waits on such work
-// stalls the loop (it publishes no request, so the kernel leaves only when its teams go idle). A
-// stream made with a CU mask gets a hardware queue that no other stream shares; the mask here
-// names every CU, so kernels on it run where they would on any stream.
-inline hipError_t serving_stream_create(hipStream_t *s)
-{
-    int dev = 0, cus = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess)
-        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e != hipSuccess)
-        return e;
-    std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0xFFFFFFFFu);
-    return hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data()); // (its size in 32-bit words)
-}
 
 struct Arena
 {

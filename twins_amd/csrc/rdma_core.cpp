@@ -1260,8 +1260,7 @@ extern "C" int trd_server_create(const trd_server_cfg *cfg, trd_server **out)
         return fail(TW_EDEVICE, "hipStreamCreate failed");
     for (auto &l : s->lane)
     {
-        // (each class's launches on a hardware queue of their own, which the uploads never share)
-        if (serving_stream_create(&l.ks) != hipSuccess || hipEventCreateWithFlags(&l.done_ev, hipEventDisableTiming) != hipSuccess)
+        if (hipStreamCreateWithFlags(&l.ks, hipStreamNonBlocking) != hipSuccess || hipEventCreateWithFlags(&l.done_ev, hipEventDisableTiming) != hipSuccess)
             return fail(TW_EDEVICE, "hipStreamCreate failed");
         if (hipMalloc((void **)&l.queue_d, sizeof(int) * RSRV_CAP) != hipSuccess || hipMalloc((void **)&l.qnext_d, sizeof(unsigned)) != hipSuccess ||
             hipMalloc((void **)&l.teams_d, sizeof(rd::TeamState) * l.blocks) != hipSuccess ||

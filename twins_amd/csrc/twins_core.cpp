@@ -1840,9 +1840,7 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
     s->arena.base = (char *)ab;
     s->arena.size = want;
     s->arena.free_[0] = want;
-    // the persistent kernel's stream on a hardware queue of its own: the uploads the host waits
-    // for never queue behind it
-    HIPCHK(serving_stream_create(&s->ks));
+    HIPCHK(hipStreamCreateWithFlags(&s->ks, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking));
     int rc;
     if ((rc = srv_pinned(*s, &s->req, (size_t)s->nq * s->cap)) || (rc = srv_pinned(*s, &s->tail, s->nq)) ||
