@@ -960,16 +960,11 @@ extern "C" int trd_batch_run(trd_batch *b, trd_run_stats *st)
         for (int w : order)
             if (!b->runs[w].finished)
                 queue.push_back(w);
-        int nteams = std::min<int>((int)queue.size(), cap / T);
+        const int nteams = std::min<int>((int)queue.size(), cap / T);
         rd::LaunchCfg L = {};
         L.T = T;
         L.hpw = hpw;
         L.nq = (int)queue.size();
-        if (nteams >= 8)
-        {
-            nteams -= nteams % 8;
-            L.xcd_local = 1;
-        }
         L.queue = b->queue_d;
         L.qnext = b->qnext_d;
         L.teams = b->teams_d;

@@ -22,7 +22,7 @@ constexpr int QSHIFT = 20;
 constexpr int LAUNCH_WAVES = 4; // waves of a launch's 256-thread workgroups
 struct LaunchCfg
 {
-    int T, nq, xcd_local, hpw; // hpw: hosts of a host item (side by side on a wave's lanes, at most 64)
+    int T, nq, hpw; // hpw: hosts of a host item (side by side on a wave's lanes, at most 64)
     const int *queue;
     unsigned *qnext;
     TeamState *teams;
