@@ -35,7 +35,9 @@ packet-event (a 176-byte link frame written and read, a 64-byte queue entry writ
 
 Reference workloads (the "reference_workloads" object): the reference's own single-world speed
 table (data/twins_speed.csv, an RX 9060 XT) on the workloads whose ns-3 outputs are stored here,
-each world alone through the plain launch, the event loop timed, every flow checked against ns-3.
+each world alone through the plain launch, the event loop timed, every flow checked against ns-3;
+and its twins-rdma table (README.md:324-333, the same GPU) on the validation jobs each row runs,
+the event loop and the whole command timed, every flow checked against the reference's output.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--sweep 0|1] [--serve 0|1] [--rdma 0|1] [--refspeed 0|1]
 """
@@ -390,6 +392,50 @@ def refspeed_leg(a):
                                 "flows_identical_to_ns3": same, "flows": len(exp)}
         log(f"refspeed: {name} {tcp}: {best:.4f} s ({slots} slots, {1e6 * best / max(1, slots):.1f} us/slot), reference "
             f"{ref_s} s on an RX 9060 XT; {same}/{len(exp)} flows identical to ns-3")
+    out.update(refspeed_rdma(a))
+    return out
+
+
+# the reference's twins-rdma speed table (README.md:324-333: wall times on an RX 9060 XT), by the
+# validation job each row runs
+REF_SPEED_RDMA = [("k16_a2a30_cc1", 0.99), ("k16_a2a30_cc3", 1.14), ("k16_a2a50s_b2_cc1", 1.17), ("k8_a2a50s_cc3", 0.44),
+                  ("k8_perm_cc3", 0.094), ("ls_a2a30_cc1", 0.63), ("jf32_a2a30_cc3", 0.76), ("k8_incast64_b1_cc1", 0.92)]
+
+
+def refspeed_rdma(a):
+    """The reference's published twins-rdma runs: each job alone, the event loop timed (trd_batch_run,
+    min of a.ref_runs) and the whole `twins-rdma` command (set-up, upload, event loop, output) timed
+    once, as the reference's table times its runs; every flow checked against the reference's output."""
+    from twins_amd import rdma
+    rg = os.path.join(ROOT, "tests", "golden", "rdma")
+    inp, cfg = os.path.join(rg, "inputs"), os.path.join(rg, "inputs", "hpcc.cfg")
+    jobs = {}
+    for line in open(os.path.join(rg, "manifest.txt")):
+        if not line.startswith("#") and line.strip():
+            name, topo, flows, kv, _ = line.split("|", 4)
+            jobs[name] = (topo, flows, kv.split())
+    out = {}
+    for name, ref_s in REF_SPEED_RDMA:
+        topo, flows, kv = jobs[name]
+        w = rdma.World(cfg, f"{inp}/{topo}", f"{inp}/{flows}", kv)
+        best = None
+        for _ in range(a.ref_runs):
+            b = rdma.Batch([w])
+            st = b.run()
+            best = st.wall_s if best is None else min(best, st.wall_s)
+            got = b.fct_text(0)
+            b.close()
+        same = got == open(os.path.join(rg, "ref", name + ".fct")).read()
+        o = f"/tmp/twins_refspeed_{name}.fct"
+        t0 = time.perf_counter()
+        r = subprocess.run([os.path.join(ROOT, "twins_amd", "bin", "twins-rdma"), cfg, f"--topo={inp}/{topo}", f"--flows={inp}/{flows}",
+                            f"--out={o}"] + kv, capture_output=True, text=True)
+        cli = time.perf_counter() - t0
+        same = same and r.returncode == 0 and open(o).read() == got
+        out[f"rdma:{name}"] = {"event_loop_s": round(best, 4), "command_wall_s": round(cli, 4), "reference_rx9060xt_s": ref_s,
+                               "speedup_vs_reference": round(ref_s / cli, 3), "flows_identical_to_reference": same}
+        log(f"refspeed: rdma {name}: event loop {best:.4f} s, twins-rdma command {cli:.3f} s, reference {ref_s} s on an RX 9060 XT; "
+            f"{'identical to' if same else 'DIFFERENT from'} the reference's output")
     return out
 
 
@@ -590,7 +636,8 @@ def main():
         if r["parity"]["flows_identical_to_reference"] != r["parity"]["flows"] or r["config"]["inexact_counters"]:
             bad.append("rdma")
     if "reference_workloads" in line:
-        bad += [k for k, v in line["reference_workloads"].items() if v["flows_identical_to_ns3"] != v["flows"]]
+        bad += [k for k, v in line["reference_workloads"].items()
+                if (v["flows_identical_to_ns3"] != v["flows"] if "flows" in v else not v["flows_identical_to_reference"])]
     if same != total or bad:
         sys.exit(1)
 

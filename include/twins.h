@@ -183,11 +183,13 @@ int tw_world_stats_get(tw_batch *b, int world, tw_world_stats *st);
 int tw_batch_destroy(tw_batch *b);
 
 /* The serving loop: one persistent kernel serves requests (one world each) as they arrive.
- * A request is admitted in arrival order once its device footprint fits the server's arena,
- * packed there (it stays resident until its result is taken), and handed to the kernel's queue
- * for its size (teams of 1, 4 or 16 workgroups); requests run side by side, and each one's
- * results are those of the same world run alone (tw_batch_run). No counterpart in the
- * reference, which runs one simulation per process (src/main.cu). */
+ * A request goes to the kernel's queue for its size (teams of 1, 4 or 16 workgroups). It is
+ * admitted in arrival order within that queue once its device footprint fits the server's arena
+ * and the queue holds fewer than 4 resident requests per team; a request waiting for either
+ * does not hold up the other queues. Admitted, it is packed into the arena and stays resident
+ * until its result is taken; requests run side by side, and each one's results are those of the
+ * same world run alone (tw_batch_run). No counterpart in the reference, which runs one
+ * simulation per process (src/main.cu). */
 typedef struct tw_server tw_server;
 typedef struct tw_server_cfg
 {

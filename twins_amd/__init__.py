@@ -238,8 +238,10 @@ class Batch:
 
 class Server:
     """The serving loop (tw_server_* in include/twins.h): one persistent kernel runs requests (one
-    world each) as they are submitted; poll() collects completions and admits waiting requests by
-    device footprint; result() returns a finished request's flow times and counters."""
+    world each) as they are submitted; poll() collects completions and admits waiting requests
+    (in arrival order within each size class, while their device footprint fits and their class's
+    teams have at most 4 resident requests each); result() returns a finished request's flow times
+    and counters."""
 
     def __init__(self, chain_runs=9, block_threads=0, small_teams=0, mid_teams=0, large_teams=0, max_requests=0,
                  arena_bytes=0, idle_s=0.0):

@@ -1389,11 +1389,20 @@ extern "C" int trd_server_poll(trd_server *s, trd_server_stats *st)
             else if (q != hipErrorNotReady)
                 return fail(TW_EDEVICE, "the rdma kernel failed: %s", hipGetErrorString(q));
         }
-    // admissions: reruns first, then in arrival order, while the arena has room
-    while (!s->redo.empty() || s->wait_head < s->waiting.size())
+    // admissions: reruns first, then in arrival order within each size class while the arena has
+    // room (a large request waiting for memory does not hold up small ones)
+    bool blocked[2] = {false, false};
+    size_t scan = s->wait_head;
+    while (!s->redo.empty() || scan < s->waiting.size())
     {
         const bool rerun = !s->redo.empty();
-        RSrvReq &r = s->reqs[rerun ? s->redo.front() : s->waiting[s->wait_head]];
+        RSrvReq &r = s->reqs[rerun ? s->redo.front() : s->waiting[scan]];
+        const int cls = r.items > RSRV_SMALL_ITEMS;
+        if (!rerun && (r.state != TW_REQ_WAITING || blocked[cls]))
+        {
+            scan++;
+            continue;
+        }
         std::unique_ptr<trd_batch> b(new trd_batch);
         b->arena = &s->arena;
         b->st = s->up;
@@ -1409,13 +1418,20 @@ extern "C" int trd_server_poll(trd_server *s, trd_server_stats *st)
             (void)hipStreamSynchronize(s->up); // (the partial batch's memsets end before its memory returns)
             b.reset();
             if (s->arena.used > 0)
-                break; // it waits for memory
+            {
+                // it waits for memory (a rerun holds up every admission, as it did before)
+                if (rerun)
+                    break;
+                blocked[cls] = true;
+                scan++;
+                continue;
+            }
             // (it does not fit even an empty arena: it never will)
         }
         if (rerun)
             s->redo.erase(s->redo.begin());
         else
-            s->wait_head++;
+            scan++;
         if (e)
         {
             r.state = TW_REQ_FAILED;
@@ -1424,6 +1440,8 @@ extern "C" int trd_server_poll(trd_server *s, trd_server_stats *st)
         r.b = b.release();
         r.state = TW_REQ_RESIDENT;
     }
+    while (s->wait_head < s->waiting.size() && s->reqs[s->waiting[s->wait_head]].state != TW_REQ_WAITING)
+        s->wait_head++;
     for (int c = 0; c < 2; c++)
         if (!s->lane[c].in_flight)
             if (int e = rsrv_launch(*s, c))

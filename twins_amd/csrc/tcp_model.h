@@ -413,8 +413,8 @@ struct Model
     struct Ledger
     {
         SndState &s;
-        uint32_t *first;
-        uint8_t *marks;
+        uint32_t TW_G *first;
+        uint8_t TW_G *marks;
         uint32_t cap;
         __device__ Ledger(const Batch &B, SndState &rec, int f) : s(rec)
         {
@@ -2218,9 +2218,40 @@ struct Model
         if (!x.active)
             return 0;
         const int n = L.cnt[x.q], b0 = L.base[x.q];
-        int64_t *ts = x.scr_ts + b0;
-        uint32_t *ref = x.scr_ref + b0;
         auto frame_at = [&](uint32_t r) -> const Frame & { return lane_input(B, W, x, t, L, (int)(r >> 16)).at((int)(r & 0xFFFF)); };
+        Fifo eg;
+        const int rt = (int)(t % NRING);
+        eg.load(B, R, x.lid, t, rt);
+        // an insertion sort by (time, frame_before): chain comparison is no strict weak order
+        // when chains are truncated, and this gives the reference's order in every case; then the
+        // arrivals in that order into the queue. (Inlined at two calls, for the list in LDS and for
+        // the list in device scratch, so that each copy's accesses have one address space: LDS or
+        // global instructions, not flat ones that wait on both.)
+        auto sort_enqueue = [&](auto *ts, auto *ref) {
+            for (int i = 1; i < n; i++)
+            {
+                const int64_t xv = ts[i];
+                const uint32_t r = ref[i];
+                int j = i - 1;
+                while (j >= 0 && (ts[j] > xv || (ts[j] == xv && frame_before(R, frame_at(r), frame_at(ref[j])))))
+                {
+                    ts[j + 1] = ts[j];
+                    ref[j + 1] = ref[j];
+                    j--;
+                }
+                ts[j + 1] = xv;
+                ref[j + 1] = r;
+            }
+            for (int i = 0; i < n; i++)
+            {
+                Frame p = frame_at(ref[i]);
+                Chain up, c;
+                chain_load(up, p.chain);
+                chain_child(c, 2048u + p.len, up);
+                chain_store(p.chain, c);
+                eg.enqueue(p, p.ts);
+            }
+        };
         if (n <= SORT_LDS)
         {
             // a short list (the usual case) is sorted in the lane's LDS slots: one round of loads
@@ -2230,40 +2261,13 @@ struct Model
             uint32_t *lref = reinterpret_cast<uint32_t *>(L.wave + SW_LDS_INTS + 2 * 64 * SORT_LDS) + lane * SORT_LDS;
             for (int i = 0; i < n; i++)
             {
-                lts[i] = ts[i];
-                lref[i] = ref[i];
+                lts[i] = x.scr_ts[b0 + i];
+                lref[i] = x.scr_ref[b0 + i];
             }
-            ts = lts;
-            ref = lref;
+            sort_enqueue(lts, lref);
         }
-        // an insertion sort by (time, frame_before): chain comparison is no strict weak order
-        // when chains are truncated, and this gives the reference's order in every case
-        for (int i = 1; i < n; i++)
-        {
-            const int64_t xv = ts[i];
-            const uint32_t r = ref[i];
-            int j = i - 1;
-            while (j >= 0 && (ts[j] > xv || (ts[j] == xv && frame_before(R, frame_at(r), frame_at(ref[j])))))
-            {
-                ts[j + 1] = ts[j];
-                ref[j + 1] = ref[j];
-                j--;
-            }
-            ts[j + 1] = xv;
-            ref[j + 1] = r;
-        }
-        Fifo eg;
-        const int rt = (int)(t % NRING);
-        eg.load(B, R, x.lid, t, rt);
-        for (int i = 0; i < n; i++)
-        {
-            Frame p = frame_at(ref[i]);
-            Chain up, c;
-            chain_load(up, p.chain);
-            chain_child(c, 2048u + p.len, up);
-            chain_store(p.chain, c);
-            eg.enqueue(p, p.ts);
-        }
+        else
+            sort_enqueue(x.scr_ts + b0, x.scr_ref + b0);
         eg.advance((t + 1) * SLOT_PS - 1);
         eg.store(t, rt);
         return (eg.sent > 0 || eg.size > 0) ? 1 : 0;

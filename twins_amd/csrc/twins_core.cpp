@@ -1733,6 +1733,7 @@ struct tw_server
     TeamState *teams_d = nullptr;
     int nteams = 0;
     int64_t small_max = 512, mid_max = 4096; // queue by entities: small, mid, large
+    int q_teams[3] = {0, 0, 0};              // teams per queue
     std::vector<unsigned> published;
     std::vector<SrvReq> reqs;
     std::vector<int> waiting; // ids, in arrival order
@@ -1830,6 +1831,7 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
     add(ns, 1, 0);
     add(nm, 4, 1);
     add(nl, 16, 2);
+    s->q_teams[0] = ns, s->q_teams[1] = nm, s->q_teams[2] = nl;
     place_teams(plan, team_of);
     s->blocks = (int)team_of.size();
     s->nteams = (int)plan.size();
@@ -1877,6 +1879,9 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
     return TW_OK;
 }
 
+// resident requests a serving queue holds per team of it
+constexpr int SRV_DEPTH = 4;
+
 // completions, admissions, and a relaunch of the kernel when it has left while requests wait
 static int srv_service(tw_server &s)
 {
@@ -1887,12 +1892,32 @@ static int srv_service(tw_server &s)
             if (d)
                 r.state = d == 1 ? TW_REQ_DONE : TW_REQ_FAILED;
         }
-    while (s.wait_head < s.waiting.size())
+    // Admissions, in arrival order within each queue. A queue holds at most SRV_DEPTH resident
+    // requests per team (more would only wait for its teams while holding device memory others
+    // could run in), and a request that is over its queue's share or does not fit the arena is
+    // passed over by later ones of other queues, so a large world waiting for memory or for the
+    // large teams does not hold up small ones.
+    int resident_q[3] = {0, 0, 0};
+    for (auto &r : s.reqs)
+        if (r.state == TW_REQ_RESIDENT)
+            resident_q[r.queue]++;
+    bool blocked[3] = {false, false, false};
+    for (size_t i = s.wait_head; i < s.waiting.size(); i++)
     {
-        const int id = s.waiting[s.wait_head];
+        const int id = s.waiting[i];
         SrvReq &r = s.reqs[id];
-        if (world_bytes(*r.w, s.runs) + (64u << 20) > s.arena.size - s.arena.used)
-            break; // admitted in order: the next waits for memory
+        if (r.state != TW_REQ_WAITING)
+            continue;
+        const int64_t n = (int64_t)r.w->H + r.w->S;
+        const int q = n <= s.small_max ? 0 : n <= s.mid_max ? 1 : 2;
+        if (blocked[q])
+            continue; // (in order within a queue)
+        if (resident_q[q] >= SRV_DEPTH * std::max(1, s.q_teams[q]) ||
+            world_bytes(*r.w, s.runs) + (64u << 20) > s.arena.size - s.arena.used)
+        {
+            blocked[q] = true;
+            continue;
+        }
         std::unique_ptr<tw_batch> b(new tw_batch);
         b->runs = s.runs;
         b->arena = &s.arena;
@@ -1903,11 +1928,14 @@ static int srv_service(tw_server &s)
         memset(&pc, 0, sizeof pc);
         int rc = s.runs == 25 ? pack<25>(*b, pc) : pack<9>(*b, pc);
         if (rc == TW_ENOMEM)
-            break; // the arena is fragmented: wait for more to free
+        {
+            blocked[q] = true; // the arena is fragmented: wait for more to free
+            continue;
+        }
         if (rc)
             return rc;
-        const int64_t n = (int64_t)r.w->H + r.w->S;
-        r.queue = n <= s.small_max ? 0 : n <= s.mid_max ? 1 : 2;
+        resident_q[q]++;
+        r.queue = q;
         r.index = s.published[r.queue];
         if ((int)r.index >= s.cap)
             return fail(TW_ESTATE, "queue %d is full (%d requests)", r.queue, s.cap);
@@ -1917,8 +1945,9 @@ static int srv_service(tw_server &s)
         r.state = TW_REQ_RESIDENT;
         s.published[r.queue] = r.index + 1;
         __atomic_store_n(&s.tail[r.queue], r.index + 1, __ATOMIC_RELEASE);
-        s.wait_head++;
     }
+    while (s.wait_head < s.waiting.size() && s.reqs[s.waiting[s.wait_head]].state != TW_REQ_WAITING)
+        s.wait_head++;
     bool unserved = false;
     for (auto &r : s.reqs)
         unserved |= r.state == TW_REQ_RESIDENT;
