@@ -61,7 +61,8 @@ static void run_cpu(BatchT<RUNS> &B, int64_t budget)
                 int64_t x = INT64_MAX;
                 SndState ls;
                 RcvState lr;
-                act += Md::host_step(B, W, R, hl, t, &x, &ls, &lr);
+                ConnStatic lc;
+                act += Md::host_step(B, W, R, hl, t, &x, &ls, &lr, &lc);
                 wk = std::min(wk, (unsigned long long)x);
             }
             for (int sw = 0; sw < W.S; sw++)

@@ -24,14 +24,15 @@
 namespace tw
 {
 
-// per wave, for the item it runs: a switch item's rows, in_win, base, cnt; a host item's slots for
-// the connection records of an event, one SndState and one RcvState per lane. A wave runs its
-// items one after another, so the two share one region.
+// per wave, for the item it runs: a switch item's rows, in_win, base, cnt, sort and route slots;
+// a host item's slots for the connection records of an event, one SndState, one RcvState and one
+// ConnStatic per lane. A wave runs its items one after another, so the two share one region.
 constexpr int LDS_SWITCH = Model<9>::SW_LDS_TOTAL;
 constexpr int LDS_SND = 0, LDS_RCV = LDS_SND + 64 * (int)(sizeof(SndState) / 4);
-constexpr int LDS_RECORDS = LDS_RCV + 64 * (int)(sizeof(RcvState) / 4);
+constexpr int LDS_CONN = LDS_RCV + 64 * (int)(sizeof(RcvState) / 4);
+constexpr int LDS_RECORDS = LDS_CONN + 64 * (int)(sizeof(ConnStatic) / 4);
 constexpr int LDS_PER_WAVE = LDS_RECORDS > LDS_SWITCH ? LDS_RECORDS : LDS_SWITCH;
-static_assert(sizeof(SndState) % 128 == 0 && sizeof(RcvState) % 128 == 0, "LDS record alignment");
+static_assert(sizeof(SndState) % 128 == 0 && sizeof(RcvState) % 128 == 0 && sizeof(ConnStatic) % 16 == 0, "LDS record alignment");
 static_assert(4 * (size_t)LDS_PER_WAVE * 4 + 1024 <= 160 * 1024, "a 256-thread workgroup's LDS");
 constexpr unsigned long long WAIT_TICKS = 2000000000ull;  // a team-barrier wait gives up after 20 s (100 MHz clock)
 constexpr unsigned long long SERVER_MAX_WAIT = (unsigned long long)(SERVER_WAIT_S * 100e6); // (100 MHz clock)
@@ -234,7 +235,8 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
                     if (lane < hpw && hl < W.H)
                     {
                         int64_t x = INT64_MAX;
-                        act += Md::host_step(B, W, R, hl, t, &x, (SndState *)(wl + LDS_SND) + lane, (RcvState *)(wl + LDS_RCV) + lane);
+                        act += Md::host_step(B, W, R, hl, t, &x, (SndState *)(wl + LDS_SND) + lane, (RcvState *)(wl + LDS_RCV) + lane,
+                                             (ConnStatic *)(wl + LDS_CONN) + lane);
                         wk = (unsigned long long)x < wk ? (unsigned long long)x : wk;
                     }
 #ifdef TW_PROF

@@ -6,6 +6,7 @@
 //
 // Build with -ffp-contract=off: the only fused multiply-adds are the explicit fma() calls.
 #pragma once
+#include <cassert>
 #include <hip/hip_runtime.h>
 #include "exact_math.h"
 #include "tcp_state.h"
@@ -137,7 +138,7 @@ struct Model
         WorldRun *R;
         int lid;
         Frame *ring;
-        int cap, head, size, red_k, limit, sent;
+        int cap, head, size, red_k, limit, sent, early;
         int64_t link_free, last_start, woke, idle_start, slot_end;
         Chain last; // the chain of the transmission in progress (kept in registers for the slot)
         Frame *out;
@@ -158,13 +159,13 @@ struct Model
             chain_load(last, b.q_last_chain + (size_t)link * CW);
             woke = idle_start = -1;
             slot_end = (t + 1) * SLOT_PS;
-            sent = 0;
+            sent = early = 0;
             out = b.msg + ((size_t)ring_t * b.nlinks + link) * MAX_TX;
         }
-        // the slot's message count and the queue back to memory
+        // the slot's message count word (msg_word) and the queue back to memory
         __device__ __forceinline__ void store(int64_t t, int ring_t)
         {
-            B->msg_cnt[(size_t)ring_t * B->nlinks + lid] = (long long)t << 16 | sent;
+            B->msg_cnt[(size_t)ring_t * B->nlinks + lid] = msg_word(t, early, sent);
             B->q_head[lid] = head;
             B->q_size[lid] = size;
             B->q_link_free[lid] = link_free;
@@ -204,6 +205,7 @@ struct Model
                 last_start = start;
                 link_free = start + (int64_t)o.len * PS_PER_BYTE;
                 o.ts = link_free + PROP_PS; // its arrival at the far end
+                early += o.ts < slot_end + SLOT_PS;
                 out[sent++] = o;
                 head = head + 1 < cap ? head + 1 : 0;
             }
@@ -284,11 +286,16 @@ struct Model
         Chain echain; // the chain of the event being processed
         Due *due;
         int nd, id;
+        int64_t due_head; // due[id].at (INT64_MAX when none is left): the next timer of the slot
         int64_t slot_end;
         // the records of the connection side an event works on, copied into this lane's LDS
         // slots for the event (snd_open / rcv_open) and written back once at its end
         SndState *ls;
         RcvState *lr;
+        ConnStatic *lc; // the event's connection's static record, copied with its side's record
+#ifndef __HIP_DEVICE_COMPILE__
+        int lc_flow = -1; // (CPU builds: which connection lc holds, checked by conn())
+#endif
     };
     // an event on connection f's sending (receiving) side: its record from device memory into
     // the lane's LDS slot, and back when the event ends (a record is 16 (8) independent 16-byte
@@ -302,15 +309,33 @@ struct Model
         for (int i = 0; i < (int)(sizeof(D) / 16); i++)
             reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(src)[i];
     }
+    // the static record comes along with either side's record (its loads issued with the record's)
+    static __device__ __forceinline__ void conn_open(Host &x, int f)
+    {
+        rec_move(x.lc, &x.B->cs[f]);
+#ifndef __HIP_DEVICE_COMPILE__
+        x.lc_flow = f;
+#endif
+    }
+    // the open connection's static record (connection f's: an event works on one connection)
+    static __device__ __forceinline__ const ConnStatic &conn(const Host &x, int f)
+    {
+#ifndef __HIP_DEVICE_COMPILE__
+        assert(x.lc_flow == f);
+#endif
+        return *x.lc;
+    }
     static __device__ __forceinline__ SndState &snd_open(Host &x, int f)
     {
         rec_move(x.ls, &x.B->snd[f]);
+        conn_open(x, f);
         return *x.ls;
     }
     static __device__ __forceinline__ void snd_close(Host &x, int f) { rec_move(&x.B->snd[f], x.ls); }
     static __device__ __forceinline__ RcvState &rcv_open(Host &x, int f)
     {
         rec_move(x.lr, &x.B->rcv[f]);
+        conn_open(x, f);
         return *x.lr;
     }
     static __device__ __forceinline__ void rcv_close(Host &x, int f) { rec_move(&x.B->rcv[f], x.lr); }
@@ -379,6 +404,8 @@ struct Model
         }
         x.due[j] = {at, f, kind};
         x.nd++;
+        if (j == x.id)
+            x.due_head = at;
     }
     static __device__ __forceinline__ void timer_chain(Host &x, uint32_t *base, int f, int64_t lead)
     {
@@ -388,7 +415,7 @@ struct Model
     }
     static __device__ void emit(Host &x, int f, uint8_t flags, uint32_t seq, uint32_t ack, uint16_t len, bool to_sender, int64_t now)
     {
-        const ConnStatic &cs = x.B->cs[f];
+        const ConnStatic &cs = conn(x, f);
         Frame fr;
         fr.ts = now;
         fr.flow = (uint32_t)f;
@@ -416,9 +443,8 @@ struct Model
         uint32_t TW_G *first;
         uint8_t TW_G *marks;
         uint32_t cap;
-        __device__ Ledger(const Batch &B, SndState &rec, int f) : s(rec)
+        __device__ Ledger(const Batch &B, SndState &rec, const ConnStatic &c) : s(rec)
         {
-            const ConnStatic &c = B.cs[f];
             first = B.it_start + c.it_off;
             marks = B.it_fl + c.it_off;
             cap = c.it_cap;
@@ -715,7 +741,7 @@ struct Model
     {
         const Batch &B = *x.B;
         SndState &s = *x.ls;
-        const ConnStatic &c = B.cs[f];
+        const ConnStatic &c = conn(x, f);
         uint32_t *hs = B.h_seq + c.it_off, *hc = B.h_cnt + c.it_off;
         if (retx)
         {
@@ -764,7 +790,7 @@ struct Model
         SndState &s = *x.ls;
         if (!s.h_n)
             return;
-        const ConnStatic &c = B.cs[f];
+        const ConnStatic &c = conn(x, f);
         const uint32_t *hs = B.h_seq + c.it_off, *hc = B.h_cnt + c.it_off;
         const uint32_t k0 = rtt_pos(s, c.it_cap, 0);
         if (!(hc[k0] & 0x80000000u) && ack >= hs[k0] + hc[k0])
@@ -1001,7 +1027,7 @@ struct Model
     {
         const Batch &B = *x.B;
         SndState &s = *x.ls;
-        Ledger L(B, s, f);
+        Ledger L(B, s, conn(x, f));
         bool retx = false;
         const uint32_t len = lg_take(L, x.R, max_size, seq, &retx);
         if (!len)
@@ -1042,7 +1068,7 @@ struct Model
         SndState &s = *x.ls;
         if (s.st != S_EST)
             return;
-        Ledger L(*x.B, s, f);
+        Ledger L(*x.B, s, conn(x, f));
         for (int n = 0; n < 4096; n++)
         {
             const bool fin_out = (s.fin & (FIN_SENT | FIN_ACKED)) == FIN_SENT;
@@ -1065,7 +1091,7 @@ struct Model
     static __device__ void tx_resend_one(Host &x, int f, int64_t now)
     {
         SndState &s = *x.ls;
-        Ledger L(*x.B, s, f);
+        Ledger L(*x.B, s, conn(x, f));
         uint32_t seq;
         if (!lg_next(L, &seq))
             seq = s.una;
@@ -1095,7 +1121,7 @@ struct Model
     static __device__ void fast_retransmit(Host &x, int f, int64_t now)
     {
         SndState &s = *x.ls;
-        Ledger L(*x.B, s, f);
+        Ledger L(*x.B, s, conn(x, f));
         lg_dup_sack(L);
         lg_head_lost(L);
         s.cong = C_RECOVERY;
@@ -1111,7 +1137,7 @@ struct Model
     {
         const Batch &B = *x.B;
         SndState &s = *x.ls;
-        const ConnStatic &c = B.cs[f];
+        const ConnStatic &c = conn(x, f);
         for (; s.m_dn < s.m_wr; s.m_dn++)
         {
             const int m = c.m_first + s.m_dn;
@@ -1157,7 +1183,7 @@ struct Model
             s.dupacks++;
         if (state == C_OPEN)
             s.cong = state = C_DISORDER;
-        Ledger L(*x.B, s, f);
+        Ledger L(*x.B, s, conn(x, f));
         if (state == C_RECOVERY)
             lg_dup_sack(L);
         else if (state == C_DISORDER)
@@ -1173,7 +1199,7 @@ struct Model
     {
         if (s.cong == C_RECOVERY)
         {
-            Ledger L(*x.B, s, f);
+            Ledger L(*x.B, s, conn(x, f));
             lg_head_lost(L);
             if (!lg_retx_ends_at(L, ack + MSS))
                 tx_resend_one(x, f, now);
@@ -1241,7 +1267,7 @@ struct Model
         const uint32_t una_before = s.una;
         const bool advancing = ack > una_before;
         rtt_on_ack(x, f, ack, now);
-        Ledger L(*x.B, s, f);
+        Ledger L(*x.B, s, conn(x, f));
         const uint32_t acked = advancing ? lg_release(L, ack) : 0;
         if (s.cong == C_CWR && ack > s.recover)
         {
@@ -1302,7 +1328,7 @@ struct Model
         }
         count(x.R, EV_RTO);
         const uint32_t was_in_flight = inflight(s);
-        Ledger L(*x.B, s, f);
+        Ledger L(*x.B, s, conn(x, f));
         lg_all_lost(L);
         s.dupacks = 0;
         s.recover = s.high;
@@ -1407,7 +1433,7 @@ struct Model
     {
         const Batch &B = *x.B;
         RcvState &r = *x.lr;
-        const ConnStatic &c = B.cs[f];
+        const ConnStatic &c = conn(x, f);
         const uint32_t from = seq > r.rcv_nxt ? seq : r.rcv_nxt;
         if (from >= end)
             return false;
@@ -1617,7 +1643,7 @@ struct Model
     {
         const Batch &B = *x.B;
         SndState &s = *x.ls;
-        const ConnStatic &c = B.cs[f];
+        const ConnStatic &c = conn(x, f);
         if (s.m_wr >= s.m_due)
             return;
         for (; s.m_wr < s.m_due; s.m_wr++)
@@ -1742,30 +1768,30 @@ struct Model
     // -------------------------------------------------------------- one slot of a host
     // (src/step.cuh:2263-2528): returns 1 if it sent or holds frames; *wake: its next slot
     static __device__ int host_step(const Batch &B, const WorldDesc &W, WorldRun *R, int hl, int64_t t, int64_t *wake, SndState *ls,
-                                    RcvState *lr)
+                                    RcvState *lr, ConnStatic *lc)
     {
         const int64_t S0 = t * SLOT_PS, E0 = S0 + SLOT_PS;
         const int h = W.host0 + hl, lid = W.link0 + hl;
         const int in = B.in_link[lid];
         const Frame *m2 = nullptr, *m1 = nullptr;
         int n2 = 0, n1 = 0, i2 = 0, i1 = 0;
+        // this slot's arrivals: the later part of slot t-2's message, the earlier of t-1's
         if (t >= 2)
         {
             size_t r = (size_t)((t - 2) % NRING) * B.nlinks + in;
-            long long v = B.msg_cnt[r];
-            n2 = (v >> 16) == t - 2 ? (int)(v & 0xFFFF) : 0;
+            const long long v = B.msg_cnt[r];
+            if (msg_slot(v) == t - 2)
+                i2 = msg_early(v), n2 = msg_sent(v);
             m2 = B.msg + r * MAX_TX;
-            while (i2 < n2 && m2[i2].ts < S0)
-                i2++;
         }
         if (t >= 1)
         {
             size_t r = (size_t)((t - 1) % NRING) * B.nlinks + in;
-            long long v = B.msg_cnt[r];
-            n1 = (v >> 16) == t - 1 ? (int)(v & 0xFFFF) : 0;
+            const long long v = B.msg_cnt[r];
+            n1 = msg_slot(v) == t - 1 ? msg_early(v) : 0;
             m1 = B.msg + r * MAX_TX;
         }
-        bool input = i2 < n2 || (n1 > 0 && m1[0].ts < E0);
+        bool input = i2 < n2 || n1 > 0;
         if (!input && t < B.wake[h] && B.q_size[lid] == 0)
         {
             *wake = B.wake[h];
@@ -1785,6 +1811,7 @@ struct Model
         x.slot_end = E0;
         x.ls = ls;
         x.lr = lr;
+        x.lc = lc;
         Due *due = x.due;
         // timers due in this slot, by time. A list is scanned only when one of its timers falls in
         // the slot: tmin_s / tmin_r are its earliest, exact since the end of the host's last slot
@@ -1832,20 +1859,29 @@ struct Model
         }
         x.nd = nd;
         x.id = 0;
+        x.due_head = nd ? due[0].at : INT64_MAX;
         int pn = B.pend_next[h];
         const int pe = B.pend_end[h];
+        // The three candidates' times are kept between events, each read again only when its
+        // own event is taken (the next reads are then in flight while that event runs): the
+        // next arrival, the next timer (due_head, which due_insert keeps), the next application
+        // start of the host's order.
+        auto next_arrival = [&]() -> int64_t { return i2 < n2 ? m2[i2].ts : i1 < n1 ? m1[i1].ts : INT64_MAX; };
+        auto next_start = [&](int &m) -> int64_t {
+            m = pn < pe ? B.pend_msg[pn] : -1;
+            if (m >= 0)
+            {
+                const int64_t at = B.mg_start[m];
+                if (at < E0)
+                    return at;
+            }
+            return INT64_MAX;
+        };
+        int pm;
+        int64_t tr = next_arrival(), tp = next_start(pm);
         for (;;)
         {
-            int64_t tr = INT64_MAX, td = INT64_MAX, tp = INT64_MAX;
-            if (i2 < n2)
-                tr = m2[i2].ts;
-            else if (i1 < n1 && m1[i1].ts < E0)
-                tr = m1[i1].ts;
-            if (x.id < x.nd)
-                td = due[x.id].at;
-            int pm = pn < pe ? B.pend_msg[pn] : -1;
-            if (pm >= 0 && B.mg_start[pm] < E0)
-                tp = B.mg_start[pm];
+            const int64_t td = x.due_head;
             int64_t now = tr < td ? tr : td;
             now = now < tp ? now : tp;
             if (now == INT64_MAX)
@@ -1856,9 +1892,11 @@ struct Model
             if (now == tp && B.cs[B.mg_conn[pm]].m_first != pm)
             {
                 // a later message of a connection (FlowSender::SendMessage)
+                const int m = pm;
                 pn++;
-                int f = B.mg_conn[pm];
-                chain_root_msg(x.echain, pm - W.msg0);
+                tp = next_start(pm);
+                int f = B.mg_conn[m];
+                chain_root_msg(x.echain, m - W.msg0);
                 SndState &s = snd_open(x, f);
                 s.m_due++;
                 if (s.st == S_EST)
@@ -1868,8 +1906,10 @@ struct Model
             else if (now == tp)
             {
                 // a connection's application starts: SYN
+                const int m = pm;
                 pn++;
-                int f = B.mg_conn[pm];
+                tp = next_start(pm);
+                int f = B.mg_conn[m];
                 SndState &s = snd_open(x, f);
                 chain_root(x.echain, hl, f - W.conn0);
                 s.m_due = 1;
@@ -1887,6 +1927,7 @@ struct Model
             else if (now == td && (now != tr || timer_before_arrival(x, due[x.id], i2 < n2 ? m2[i2] : m1[i1])))
             {
                 Due d = due[x.id++];
+                x.due_head = x.id < x.nd ? due[x.id].at : INT64_MAX;
                 chain_load(x.echain, due_chain(B, d));
                 const int f = d.flow;
                 if (d.kind == 0)
@@ -1955,6 +1996,7 @@ struct Model
             else
             {
                 Frame p = i2 < n2 ? m2[i2++] : m1[i1++];
+                tr = next_arrival();
                 Chain pc;
                 chain_load(pc, p.chain);
                 chain_child(x.echain, 2048u + p.len, pc);
@@ -2064,9 +2106,14 @@ struct Model
     // phases separated by wave_sync(). LDS per switch: rows[k][k+1] counts, in_win[k][3], base[k],
     // cnt[k]; per-wave global scratch holds the arrivals sorted per egress.
     // a switch item's LDS: per lane rows (33 ints), in_win (3), base, cnt, then (after the
-    // spray merge's 64) per lane SORT_LDS arrival times and refs for the egress sort
-    static constexpr int SW_LDS_INTS = 64 * 33 + 64 * 3 + 64 + 64, SORT_LDS = 24;
-    static constexpr int SW_LDS_TOTAL = SW_LDS_INTS + 64 * SORT_LDS * 3;
+    // spray merge's 64) per lane SORT_LDS arrival times and refs for the egress sort, then per
+    // lane the egress ports of its first EGRESS_LDS arrivals (one byte each: the count phase
+    // routes them, the scatter phase reads the routes back), then per lane its port's in-link
+    static constexpr int SW_LDS_INTS = 64 * 33 + 64 * 3 + 64 + 64, SORT_LDS = 24, EGRESS_LDS = 64;
+    static constexpr int SW_LDS_EGRESS = SW_LDS_INTS + 64 * SORT_LDS * 3;
+    static constexpr int SW_LDS_INLINK = SW_LDS_EGRESS + 64 * EGRESS_LDS / 4;
+    static constexpr int SW_LDS_TOTAL = SW_LDS_INLINK + 64;
+    static constexpr uint8_t NO_EGRESS = 0xFF;
     struct SwLds
     {
         int *rows, *in_win, *base, *cnt;
@@ -2103,9 +2150,19 @@ struct Model
         uint32_t *scr_ref;
         uint8_t *scr_up;
     };
+    // port q's input of the slot (its in-link from LDS, where the window phase put it)
     static __device__ In lane_input(const Batch &B, const WorldDesc &W, const SwLane &x, int64_t t, const SwLds &L, int q)
     {
-        return input_of(B, B.in_link[W.link0 + W.H + x.swl * W.k + q], t, L.in_win + 3 * q);
+        return input_of(B, L.wave[SW_LDS_INLINK + sw_lane_of(L, x) - x.q + q], t, L.in_win + 3 * q);
+    }
+    // the switch lane's place in its wave (its switch's first lane, where L.cnt points, plus its port)
+    static __device__ __forceinline__ int sw_lane_of(const SwLds &L, const SwLane &x)
+    {
+        return (int)(L.cnt - L.wave - (64 * 33 + 64 * 3 + 64)) + x.q;
+    }
+    static __device__ __forceinline__ uint8_t *lane_egress_lds(const SwLds &L, const SwLane &x)
+    {
+        return reinterpret_cast<uint8_t *>(L.wave + SW_LDS_EGRESS) + sw_lane_of(L, x) * EGRESS_LDS;
     }
     static __device__ int lane_egress(const Batch &B, const WorldDesc &W, const SwLane &x, int j, const Frame &p)
     {
@@ -2120,28 +2177,21 @@ struct Model
     {
         if (!x.active)
             return;
-        const int64_t S0 = t * SLOT_PS, E0 = S0 + SLOT_PS;
         const int in_link = B.in_link[x.lid];
+        L.wave[SW_LDS_INLINK + sw_lane_of(L, x)] = in_link;
         int w0 = 0, w1 = 0, w2 = 0;
         if (in_link >= 0)
         {
             if (t >= 2)
             {
-                size_t r = (size_t)((t - 2) % NRING) * B.nlinks + in_link;
-                long long v = B.msg_cnt[r];
-                w1 = (v >> 16) == t - 2 ? (int)(v & 0xFFFF) : 0;
-                const Frame *m = B.msg + r * MAX_TX;
-                while (w0 < w1 && m[w0].ts < S0)
-                    w0++;
+                const long long v = B.msg_cnt[(size_t)((t - 2) % NRING) * B.nlinks + in_link];
+                if (msg_slot(v) == t - 2)
+                    w0 = msg_early(v), w1 = msg_sent(v);
             }
             if (t >= 1)
             {
-                size_t r = (size_t)((t - 1) % NRING) * B.nlinks + in_link;
-                long long v = B.msg_cnt[r];
-                int n = (v >> 16) == t - 1 ? (int)(v & 0xFFFF) : 0;
-                const Frame *m = B.msg + r * MAX_TX;
-                while (w2 < n && m[w2].ts < E0)
-                    w2++;
+                const long long v = B.msg_cnt[(size_t)((t - 1) % NRING) * B.nlinks + in_link];
+                w2 = msg_slot(v) == t - 1 ? msg_early(v) : 0;
             }
         }
         L.in_win[3 * x.q] = w0;
@@ -2158,10 +2208,13 @@ struct Model
             return;
         In in = lane_input(B, W, x, t, L, x.q);
         int *row = L.rows + x.q * (W.k + 1);
+        uint8_t *eg = lane_egress_lds(L, x);
         const int n = in.count();
         for (int j = 0; j < n; j++)
         {
             int e = lane_egress(B, W, x, j, in.at(j));
+            if (j < EGRESS_LDS)
+                eg[j] = e >= 0 ? (uint8_t)e : NO_EGRESS;
             if (e >= 0)
                 row[e]++;
             else
@@ -2199,17 +2252,39 @@ struct Model
             return;
         In in = lane_input(B, W, x, t, L, x.q);
         int *row = L.rows + x.q * (W.k + 1);
+        const uint8_t *eg = lane_egress_lds(L, x);
         const int n = in.count();
-        for (int j = 0; j < n; j++)
-        {
-            const Frame &p = in.at(j);
-            int e = lane_egress(B, W, x, j, p);
-            if (e < 0)
-                continue;
-            int at = L.base[e] + row[e]++;
-            x.scr_ts[at] = p.ts;
-            x.scr_ref[at] = (uint32_t)(x.q << 16 | j);
-        }
+        auto scatter = [&](auto *ts, auto *ref) {
+            for (int j = 0; j < n; j++)
+            {
+                const Frame &p = in.at(j);
+                const int e = j < EGRESS_LDS ? (eg[j] == NO_EGRESS ? -1 : (int)eg[j]) : lane_egress(B, W, x, j, p);
+                if (e < 0)
+                    continue;
+                int at = L.base[e] + row[e]++;
+                ts[at] = p.ts;
+                ref[at] = (uint32_t)(x.q << 16 | j);
+            }
+        };
+        if (sw_lists_in_lds(W, L))
+            scatter(sort_ts_lds(L, x), sort_ref_lds(L, x));
+        else
+            scatter(x.scr_ts, x.scr_ref);
+    }
+    // the switch's egress lists go to LDS (its lanes' SORT_LDS slots each, laid out by L.base)
+    // when all of them fit there, else to the wave's device scratch
+    static __device__ __forceinline__ bool sw_lists_in_lds(const WorldDesc &W, const SwLds &L)
+    {
+        return L.base[W.k - 1] + L.cnt[W.k - 1] <= W.k * SORT_LDS;
+    }
+    // the switch's sort slots in LDS (from its first lane's)
+    static __device__ __forceinline__ int64_t *sort_ts_lds(const SwLds &L, const SwLane &x)
+    {
+        return reinterpret_cast<int64_t *>(L.wave + SW_LDS_INTS) + (sw_lane_of(L, x) - x.q) * SORT_LDS;
+    }
+    static __device__ __forceinline__ uint32_t *sort_ref_lds(const SwLds &L, const SwLane &x)
+    {
+        return reinterpret_cast<uint32_t *>(L.wave + SW_LDS_INTS + 2 * 64 * SORT_LDS) + (sw_lane_of(L, x) - x.q) * SORT_LDS;
     }
     // 6: egress q's arrivals sorted by (time, frame_before), enqueued with the arrival event's
     // chain (RED), and the slot's transmissions; returns 1 if the port sent or holds frames
@@ -2252,11 +2327,13 @@ struct Model
                 eg.enqueue(p, p.ts);
             }
         };
-        if (n <= SORT_LDS)
+        if (sw_lists_in_lds(W, L))
+            sort_enqueue(sort_ts_lds(L, x) + b0, sort_ref_lds(L, x) + b0); // (the usual case: scattered into LDS)
+        else if (n <= SORT_LDS)
         {
-            // a short list (the usual case) is sorted in the lane's LDS slots: one round of loads
-            // instead of a device-memory round trip per comparison and move
-            const int lane = (int)(L.cnt - L.wave - (64 * 33 + 64 * 3 + 64)) + x.q;
+            // a short list of a switch whose lists overflow LDS is sorted in the lane's own slots:
+            // one round of loads instead of a device-memory round trip per comparison and move
+            const int lane = sw_lane_of(L, x);
             int64_t *lts = reinterpret_cast<int64_t *>(L.wave + SW_LDS_INTS) + lane * SORT_LDS;
             uint32_t *lref = reinterpret_cast<uint32_t *>(L.wave + SW_LDS_INTS + 2 * 64 * SORT_LDS) + lane * SORT_LDS;
             for (int i = 0; i < n; i++)
@@ -2351,7 +2428,7 @@ struct Model
         return (uint32_t)(f.ts - S0) | nr << 20;
     }
     // this port's lane in the wavefront (its switch's first lane, base, plus the port)
-    static __device__ int spray_lane(const SwLane &x, const SwLds &L) { return (int)(L.cnt - L.wave - (64 * 33 + 64 * 3 + 64)) + x.q; }
+    static __device__ int spray_lane(const SwLane &x, const SwLds &L) { return sw_lane_of(L, x); }
     static __device__ uint32_t spray_entry(const Batch &B, const WorldDesc &W, const SwLane &x, const SwLds &L, const Spray &sp, int64_t S0, int j)
     {
         if (j < SPRAY_CACHE)

@@ -57,6 +57,17 @@ constexpr uint32_t MSS = 1460;
 constexpr int CTRL_FRAME = 42, SYN_FRAME = 46;
 constexpr int MAX_TX = 300;    // frames one link can start in a slot (the model's cap)
 constexpr int NRING = 3;       // link message ring: slots t-2, t-1 (read) and t (written)
+// A link's message count word for slot t: the slot in the high 32 bits, then how many of its
+// frames arrive before the end of slot t + 1, then how many it holds. The receiver takes the
+// first ones in slot t + 1 and the rest in slot t + 2; frames start in order, one after another,
+// so their arrivals increase and the first ones are a prefix.
+TW_HD inline long long msg_word(int64_t t, int early, int sent)
+{
+    return (long long)((unsigned long long)t << 32) | (long long)early << 16 | sent;
+}
+TW_HD inline int64_t msg_slot(long long v) { return v >> 32; }
+TW_HD inline int msg_early(long long v) { return (int)((v >> 16) & 0xFFFF); }
+TW_HD inline int msg_sent(long long v) { return (int)(v & 0xFFFF); }
 constexpr int MAX_ACTIVE = 256, MAX_DUE = 16;
 constexpr int QDISC_CAP = 200, RED_K = 30;
 constexpr int64_t MIN_RTO_PS = 1000000000, DELACK_PS = 10000000, CONN_TIMEOUT_PS = 10000000000LL,
@@ -121,7 +132,7 @@ struct alignas(16) FrameT
 static_assert(sizeof(FrameT<9>) == 64 && sizeof(FrameT<25>) == 128, "frame layout");
 
 // per connection, read-only
-struct ConnStatic
+struct alignas(16) ConnStatic // (64 bytes: moved into LDS in 16-byte pieces with the event's record)
 {
     int32_t src, dst; // global host ids
     uint32_t hash_data, hash_ack;
@@ -129,8 +140,9 @@ struct ConnStatic
     uint32_t it_cap, oo_cap;
     uint64_t it_off, oo_off; // ring offsets (sent list + RTT history; out-of-order blocks)
     int32_t world;
-    uint8_t fin_on, pad[3];
+    uint8_t fin_on, pad[11];
 };
+static_assert(sizeof(ConnStatic) == 64, "ConnStatic: 64 bytes");
 
 // the sending side (touched only by the sender's host)
 struct alignas(128) SndState
@@ -306,7 +318,7 @@ struct BatchT
     int32_t TW_G *q_cap, *q_head, *q_size, *q_red_k, *q_limit;
     int64_t TW_G *q_link_free, *q_last_start;
     uint32_t TW_G *q_last_chain; // [link][CW]
-    // link messages [NRING][link][MAX_TX] and their tagged counts slot << 16 | n
+    // link messages [NRING][link][MAX_TX] and their count words (msg_word)
     Frame TW_G *msg;
     long long TW_G *msg_cnt;
     // topology-file tables (all worlds)

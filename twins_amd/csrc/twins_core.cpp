@@ -1149,7 +1149,7 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
     if ((rc = dev_alloc(b, &B.msg, (size_t)NRING * L * MAX_TX)))
         return rc;
     {
-        std::vector<long long> init((size_t)NRING * L, -(1ll << 16)); // tagged with no slot
+        std::vector<long long> init((size_t)NRING * L, msg_word(-1, 0, 0)); // of no slot
         UP(msg_cnt, init);
     }
     UP(g_host_sw, g_host_sw);
@@ -1802,6 +1802,19 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
     int ndev = 0, dev = 0, cus = 0, per_cu = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(TW_EDEVICE, "no GPU: libtwins_core runs the model on the GPU only");
+    {
+        // HIP maps the process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless set);
+        // a stream sharing a queue with this server's persistent kernel waits until it leaves
+        const char *q = getenv("GPU_MAX_HW_QUEUES");
+        static bool told = false;
+        if ((!q || atoi(q) < 8) && !told)
+        {
+            told = true;
+            fprintf(stderr, "twins: serving loop with %s hardware queues for HIP streams: work on a stream that shares one with "
+                            "the server's persistent kernel waits for it to leave; set GPU_MAX_HW_QUEUES=16 before the first HIP call\n",
+                    q ? q : "4 (the default)");
+        }
+    }
     std::unique_ptr<tw_server> s(new tw_server);
     s->runs = c.chain_runs == 25 ? 25 : 9;
     s->threads = c.block_threads > 0 ? c.block_threads : 256;
