@@ -140,6 +140,7 @@ struct Model
         Frame *ring;
         int cap, head, size, red_k, limit, sent, early;
         int64_t link_free, last_start, woke, idle_start, slot_end;
+        int64_t head_ts; // the first queued frame's enqueue time (while size > 0)
         Chain last; // the chain of the transmission in progress (kept in registers for the slot)
         Frame *out;
 
@@ -160,6 +161,7 @@ struct Model
             woke = idle_start = -1;
             slot_end = (t + 1) * SLOT_PS;
             sent = early = 0;
+            head_ts = size > 0 ? ring[head].ts : 0;
             out = b.msg + ((size_t)ring_t * b.nlinks + link) * MAX_TX;
         }
         // the slot's message count word (msg_word) and the queue back to memory
@@ -180,19 +182,19 @@ struct Model
         // frame's own enqueue (a link that never sent counts as idle).
         __device__ void advance(int64_t until)
         {
-            for (; size > 0; size--)
+            while (size > 0)
             {
                 if (sent >= MAX_TX)
                 {
                     count(R, EV_TXCAP);
                     return;
                 }
-                const Frame &f = ring[head];
-                const int64_t start = f.ts > link_free ? f.ts : link_free;
+                // (the head's time is kept in a register, so a call that starts nothing reads no frame)
+                const int64_t start = head_ts > link_free ? head_ts : link_free;
                 if (start >= slot_end || start > until)
                     return;
-                Frame o = f;
-                if (f.ts <= link_free && link_free > last_start)
+                Frame o = ring[head];
+                if (head_ts <= link_free && link_free > last_start)
                 {
                     chain_child(last, (uint32_t)((link_free - last_start) / PS_PER_BYTE), last);
                     chain_store(o.chain, last);
@@ -208,6 +210,8 @@ struct Model
                 early += o.ts < slot_end + SLOT_PS;
                 out[sent++] = o;
                 head = head + 1 < cap ? head + 1 : 0;
+                if (--size > 0)
+                    head_ts = ring[head].ts;
             }
         }
         // a queue-trace record (src/step.cuh:266-279), stored field by field (no record on the stack)
@@ -269,7 +273,8 @@ struct Model
             }
             const int at = head + size < cap ? head + size : head + size - cap;
             ring[at] = g;
-            size++;
+            if (size++ == 0)
+                head_ts = g.ts;
             return true;
         }
     };
@@ -345,22 +350,39 @@ struct Model
     // no receiver timer pending), as in the reference (src/step.cuh:2470-2510; with a full list
     // the add is refused and counted, so the removal's timing decides results in that regime);
     // compact() drops those and returns the earliest timer of the rest.
+    // (Both walk their list COMPACT_RUN entries at a time: the entries' records are all read
+    // before any entry is written back, so the reads of a run are in flight together.)
+    static constexpr int COMPACT_RUN = 4;
     static __device__ __forceinline__ int64_t compact_snd(const Batch &B, int h)
     {
         int64_t next = INT64_MAX;
         int32_t *l = B.act_snd + (size_t)h * MAX_ACTIVE;
         int k = 0, n = B.n_snd[h];
-        for (int i = 0; i < n; i++)
+        for (int i0 = 0; i0 < n; i0 += COMPACT_RUN)
         {
-            int f = l[i];
-            const SndState &s = B.snd[f];
-            if (s.st == S_DONE)
-                continue;
-            l[k++] = f;
-            if (s.rto_at >= 0 && s.rto_at < next)
-                next = s.rto_at;
-            if (s.syn_at >= 0 && s.syn_at < next)
-                next = s.syn_at;
+            int f[COMPACT_RUN];
+            bool done[COMPACT_RUN];
+            int64_t rto[COMPACT_RUN], syn[COMPACT_RUN];
+#pragma unroll
+            for (int j = 0; j < COMPACT_RUN; j++)
+                if (i0 + j < n)
+                {
+                    f[j] = l[i0 + j];
+                    const SndState &s = B.snd[f[j]];
+                    done[j] = s.st == S_DONE;
+                    rto[j] = s.rto_at;
+                    syn[j] = s.syn_at;
+                }
+#pragma unroll
+            for (int j = 0; j < COMPACT_RUN; j++)
+                if (i0 + j < n && !done[j])
+                {
+                    l[k++] = f[j];
+                    if (rto[j] >= 0 && rto[j] < next)
+                        next = rto[j];
+                    if (syn[j] >= 0 && syn[j] < next)
+                        next = syn[j];
+                }
         }
         B.n_snd[h] = k;
         return next;
@@ -370,18 +392,32 @@ struct Model
         int64_t next = INT64_MAX;
         int32_t *l = B.act_rcv + (size_t)h * MAX_ACTIVE;
         int k = 0, n = B.n_rcv[h];
-        for (int i = 0; i < n; i++)
+        for (int i0 = 0; i0 < n; i0 += COMPACT_RUN)
         {
-            int f = l[i];
-            const RcvState &r = B.rcv[f];
-            if (B.snd[f].st == S_DONE && r.delack_at < 0 && r.r_last_at < 0 && r.r_frx_at < 0)
-                continue;
-            l[k++] = f;
-            int64_t v[4] = {r.delack_at, r.srv_syn_at, r.r_last_at, r.r_frx_at};
+            int f[COMPACT_RUN];
+            bool gone[COMPACT_RUN];
+            int64_t tmin[COMPACT_RUN];
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (v[j] >= 0 && v[j] < next)
-                    next = v[j];
+            for (int j = 0; j < COMPACT_RUN; j++)
+                if (i0 + j < n)
+                {
+                    f[j] = l[i0 + j];
+                    const RcvState &r = B.rcv[f[j]];
+                    const int64_t v[4] = {r.delack_at, r.srv_syn_at, r.r_last_at, r.r_frx_at};
+                    gone[j] = B.snd[f[j]].st == S_DONE && v[0] < 0 && v[2] < 0 && v[3] < 0;
+                    tmin[j] = INT64_MAX;
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        if (v[q] >= 0 && v[q] < tmin[j])
+                            tmin[j] = v[q];
+                }
+#pragma unroll
+            for (int j = 0; j < COMPACT_RUN; j++)
+                if (i0 + j < n && !gone[j])
+                {
+                    l[k++] = f[j];
+                    next = tmin[j] < next ? tmin[j] : next;
+                }
         }
         B.n_rcv[h] = k;
         return next;
@@ -1772,6 +1808,10 @@ struct Model
     {
         const int64_t S0 = t * SLOT_PS, E0 = S0 + SLOT_PS;
         const int h = W.host0 + hl, lid = W.link0 + hl;
+#ifdef TW_PROF
+        const unsigned long long pr0 = wall_clock64();
+        unsigned long long pr_ev = 0, pr_n = 0;
+#endif
         const int in = B.in_link[lid];
         const Frame *m2 = nullptr, *m1 = nullptr;
         int n2 = 0, n1 = 0, i2 = 0, i1 = 0;
@@ -1879,6 +1919,9 @@ struct Model
         };
         int pm;
         int64_t tr = next_arrival(), tp = next_start(pm);
+#ifdef TW_PROF
+        const unsigned long long pr1 = wall_clock64();
+#endif
         for (;;)
         {
             const int64_t td = x.due_head;
@@ -1889,6 +1932,10 @@ struct Model
             if (now < S0)
                 now = S0;
             x.nic.advance(now);
+#ifdef TW_PROF
+            const unsigned long long pq = wall_clock64();
+            pr_n++;
+#endif
             if (now == tp && B.cs[B.mg_conn[pm]].m_first != pm)
             {
                 // a later message of a connection (FlowSender::SendMessage)
@@ -2002,7 +2049,13 @@ struct Model
                 chain_child(x.echain, 2048u + p.len, pc);
                 on_frame(x, p, now);
             }
+#ifdef TW_PROF
+            pr_ev += wall_clock64() - pq;
+#endif
         }
+#ifdef TW_PROF
+        const unsigned long long pr2 = wall_clock64();
+#endif
         x.nic.advance(E0 - 1);
         x.nic.store(t, rt);
         B.pend_next[h] = pn;
@@ -2016,6 +2069,14 @@ struct Model
         int64_t wk = x.nic.size > 0 ? t + 1 : (next == INT64_MAX ? INT64_MAX : next / SLOT_PS);
         B.wake[h] = wk;
         *wake = wk;
+#ifdef TW_PROF
+        const unsigned long long pr3 = wall_clock64();
+        atomicAdd(&R->ev[EV_PROF_H_SETUP], pr1 - pr0);
+        atomicAdd(&R->ev[EV_PROF_H_LOOP], pr2 - pr1);
+        atomicAdd(&R->ev[EV_PROF_H_EVENTS], pr_ev);
+        atomicAdd(&R->ev[EV_PROF_H_END], pr3 - pr2);
+        atomicAdd(&R->ev[EV_PROF_H_COUNT], pr_n);
+#endif
         return (x.nic.sent > 0 || x.nic.size > 0) ? 1 : 0;
     }
 

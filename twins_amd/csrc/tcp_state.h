@@ -100,7 +100,15 @@ enum : int
     EV_RETX_BY_STATE = 8, // 8..12
     EV_MAX_INFLIGHT = 13,
     EV_RTT_OVF = 14,
+    // TW_PROF builds: host lanes' wall-clock ticks summed over hosts and slots: the slot's set-up
+    // (inputs, timer scan), its event loop, of which the events' handling, and its end (active
+    // list compaction); and the host events handled
+    EV_PROF_H_SETUP = 15,
+    EV_PROF_H_LOOP = 16,
+    EV_PROF_H_EVENTS = 17,
+    EV_PROF_H_END = 18,
     EV_RING_OVF = 19,
+    EV_PROF_H_COUNT = 24,
     EV_UNDECIDED = 25,
     EV_PKT_EVENTS = 26, // frames started on a link (packet-events)
     // TW_PROF builds: wall-clock ticks summed over slots: slot period, slowest workgroup's host

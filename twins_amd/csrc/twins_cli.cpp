@@ -265,6 +265,9 @@ int main(int argc, char **argv)
     if (ev[27]) // switch items' phases, summed over items (not per slot maxima)
         printf("prof: switch phases (item-us per slot): window+spray %.2f, count %.2f, scatter %.2f, egress %.2f\n",
                ev[20] / 100.0 / slots, ev[21] / 100.0 / slots, ev[22] / 100.0 / slots, ev[23] / 100.0 / slots);
+    if (ev[27]) // host lanes' phases, summed over hosts (not per slot maxima)
+        printf("prof: host phases (lane-us per slot): set-up %.2f, loop %.2f (events %.2f), end %.2f; events per slot %.2f\n",
+               ev[15] / 100.0 / slots, ev[16] / 100.0 / slots, ev[17] / 100.0 / slots, ev[18] / 100.0 / slots, (double)ev[24] / slots);
     if (ev[5] || ev[6] || ev[7] || ev[14] || ev[19])
         fprintf(stderr, "warning: a capacity overflowed (active/ooo/due/rtt/ring-ovf); the results may differ from ns-3\n");
     if (ev[25])
