@@ -79,28 +79,36 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
     // a barrier wait's limit; the server's claim waits as long as a request may take to come
     unsigned long long wait_limit = WAIT_TICKS;
 
-    // the team barrier; the last workgroup to arrive runs `last` first
-    auto team_barrier = [&](auto last) -> bool {
+    // The team barrier; the last workgroup to arrive runs `last` first. Fenced (a world's start
+    // and end): agent-scope release and acquire, so that the world's records, written with plain
+    // stores, are in memory when it ends and no stale copy is read when it starts. Unfenced (the
+    // barrier between slots): what workgroups hand each other from slot to slot goes through
+    // device-coherent accesses only (coh_load / coh_store, tcp_state.h), so it is enough that
+    // every wave's stores have completed (s_waitcnt) before its workgroup arrives; the caches,
+    // and each entity's records in them, stay warm from slot to slot.
+    auto team_barrier = [&](auto last, bool fenced) -> bool {
+        __builtin_amdgcn_s_waitcnt(0); // (this wave's stores have completed)
         __syncthreads();
         if (tid == 0)
         {
+            if (fenced)
+                __threadfence();
             if (T == 1)
                 last();
             else
             {
-                __threadfence();
                 unsigned g = gen;
                 if (atomicAdd(&ts->arrive, 1u) == (unsigned)T - 1)
                 {
                     last();
                     atomicExch(&ts->arrive, 0u);
-                    __threadfence();
+                    __builtin_amdgcn_s_waitcnt(0); // (the ticket's stores before the release of the others)
                     atomicAdd(&ts->gen, 1u);
                 }
                 else
                 {
                     const unsigned long long w0 = wall_clock64();
-                    while (ld_acquire(&ts->gen) == g)
+                    while (coh_load(&ts->gen) == g)
                     {
                         __builtin_amdgcn_s_sleep(1);
                         if (wall_clock64() - w0 > (SRV ? wait_limit : WAIT_TICKS))
@@ -110,9 +118,10 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
                         }
                     }
                 }
-                __threadfence();
                 gen = g + 1;
             }
+            if (fenced)
+                __threadfence();
             s_err = T > 1 && __hip_atomic_load(&ts->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
@@ -125,7 +134,7 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
         if constexpr (!SRV)
         {
             // the team's next world, from its queue
-            if (team_barrier([&] { ts->world = atomicAdd(Bp->queue_next + P.queue, 1); }))
+            if (team_barrier([&] { ts->world = atomicAdd(Bp->queue_next + P.queue, 1); }, true))
                 return;
             if (tid == 0)
                 s_w = __hip_atomic_load(&ts->world, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -161,7 +170,7 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
                     __builtin_amdgcn_s_sleep(8);
                 }
                 ts->world = got;
-            });
+            }, true);
             wait_limit = WAIT_TICKS;
             if (err)
                 return;
@@ -198,8 +207,10 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
             __syncthreads();
             if (s_done)
             {
-                // every workgroup of the team leaves at the same slot; the first accounts the time
-                // (and, serving, tells the host)
+                // every workgroup of the team leaves at the same slot, after a fenced barrier (the
+                // world's records in memory before its results are read); the first accounts the
+                // time (and, serving, tells the host)
+                (void)team_barrier([] {}, true);
                 if (rank == 0 && tid == 0)
                 {
                     R->ticks += wall_clock64() - c_begin;
@@ -296,7 +307,7 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
                 // the ticket (src/kernel.cuh:79-112): finish, or the next slot with the exact idle skip
                 unsigned a = T > 1 ? atomicExch(&ts->act, 0u) : s_act;
                 unsigned long long nw = T > 1 ? atomicExch(&ts->wake, ~0ull) : s_wake;
-                unsigned cur = a + R->act_cur, prev = R->act_prev;
+                unsigned cur = a + coh_load(&R->act_cur), prev = coh_load(&R->act_prev);
                 bool fin = atomicAdd(&R->flows_done, 0u) >= (unsigned)W.M || t + 1 >= W.max_slots;
                 int64_t next = t + 1;
                 if (cur == 0 && prev == 0 && nw != ~0ull && (int64_t)nw > next)
@@ -304,27 +315,27 @@ __device__ __forceinline__ void world_loop(const BatchT<RUNS> *__restrict__ Bp, 
 #ifdef TW_PROF
                 {
                     const unsigned long long now = wall_clock64();
-                    if (ts->prof[3])
-                        R->ev[EV_PROF_PERIOD] += now - ts->prof[3];
-                    ts->prof[3] = now;
-                    R->ev[EV_PROF_HOSTS] += atomicExch(&ts->prof[0], 0ull);
-                    R->ev[EV_PROF_SWITCHES] += atomicExch(&ts->prof[1], 0ull);
-                    R->ev[EV_PROF_WORK] += atomicExch(&ts->prof[2], 0ull);
+                    const unsigned long long before = atomicExch(&ts->prof[3], now);
+                    if (before)
+                        atomicAdd(&R->ev[EV_PROF_PERIOD], now - before);
+                    atomicAdd(&R->ev[EV_PROF_HOSTS], atomicExch(&ts->prof[0], 0ull));
+                    atomicAdd(&R->ev[EV_PROF_SWITCHES], atomicExch(&ts->prof[1], 0ull));
+                    atomicAdd(&R->ev[EV_PROF_WORK], atomicExch(&ts->prof[2], 0ull));
                 }
 #endif
-                R->act_prev = cur;
-                R->act_cur = 0;
-                R->slot = next;
-                R->slots_run++;
+                coh_store(&R->act_prev, cur);
+                coh_store(&R->act_cur, 0u);
+                coh_store(&R->slot, next);
+                coh_store(&R->slots_run, coh_load(&R->slots_run) + 1);
                 if (fin)
-                    R->finished = 1;
-            });
+                    coh_store(&R->finished, 1);
+            }, false);
             if (err)
             {
                 if (tid == 0)
                 {
-                    R->error = 1;
-                    R->finished = 1;
+                    coh_store(&R->error, 1);
+                    coh_store(&R->finished, 1);
                     if constexpr (SRV)
                         if (rank == 0)
                         {

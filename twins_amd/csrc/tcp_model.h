@@ -7,6 +7,7 @@
 // Build with -ffp-contract=off: the only fused multiply-adds are the explicit fma() calls.
 #pragma once
 #include <cassert>
+#include <cstddef>
 #include <hip/hip_runtime.h>
 #include "exact_math.h"
 #include "tcp_state.h"
@@ -88,16 +89,19 @@ struct Model
     }
     // < 0: a runs first, > 0: b, 0: undecided. Larger codes (earlier parents) first; a complete
     // chain (fewer than RUNS runs) ends at its root, before any other parent; else the roots.
-    static __device__ int chain_cmp(const uint32_t *a, const uint32_t *b)
+    // (a(i), b(i): the chains' words; chain_cmp reads plain memory, frames of link messages are
+    // read device-coherently, word by word as the comparison needs them)
+    template <typename GA, typename GB>
+    static __device__ int chain_cmp_by(GA a, GB b)
     {
         int ia = 0, ib = 0;
-        uint32_t ra = a[0] >> 16, rb = b[0] >> 16;
+        uint32_t ra = a(0) >> 16, rb = b(0) >> 16;
         for (;;)
         {
-            uint32_t ca = a[ia] & 0xFFFF, cb = b[ib] & 0xFFFF;
-            if (ca == 0 && cb != 0 && (a[RUNS] & 0xFFFF) != 0)
+            uint32_t ca = a(ia) & 0xFFFF, cb = b(ib) & 0xFFFF;
+            if (ca == 0 && cb != 0 && (a(RUNS) & 0xFFFF) != 0)
                 return -1;
-            if (cb == 0 && ca != 0 && (b[RUNS] & 0xFFFF) != 0)
+            if (cb == 0 && ca != 0 && (b(RUNS) & 0xFFFF) != 0)
                 return 1;
             if (ca == 0 || cb == 0)
                 break;
@@ -110,19 +114,39 @@ struct Model
             {
                 if (++ia == RUNS)
                     break;
-                ra = a[ia] >> 16;
+                ra = a(ia) >> 16;
             }
             if (rb == 0)
             {
                 if (++ib == RUNS)
                     break;
-                rb = b[ib] >> 16;
+                rb = b(ib) >> 16;
             }
         }
-        uint32_t xa = a[RUNS] << 16 | a[RUNS] >> 16, xb = b[RUNS] << 16 | b[RUNS] >> 16; // the root keys
+        uint32_t xa = a(RUNS) << 16 | a(RUNS) >> 16, xb = b(RUNS) << 16 | b(RUNS) >> 16; // the root keys
         if (xa == 0 || xb == 0 || xa == xb)
             return 0;
         return xa < xb ? -1 : 1;
+    }
+    static __device__ int chain_cmp(const uint32_t *a, const uint32_t *b)
+    {
+        return chain_cmp_by([&](int i) { return a[i]; }, [&](int i) { return b[i]; });
+    }
+    // a link message's frame, read device-coherently: its head (all but the chain), its chain
+    struct FrameHead
+    {
+        int64_t ts;
+        uint32_t flow, sa;
+        uint16_t len, dst;
+        uint8_t flags, to_sender, up, pad;
+    };
+    static_assert(sizeof(FrameHead) == 24 && offsetof(Frame, chain) == 24, "a frame's head");
+    static __device__ __forceinline__ FrameHead msg_head(const Frame *p) { return coh_get(reinterpret_cast<const FrameHead *>(p)); }
+    static __device__ __forceinline__ void msg_chain(Chain &c, const Frame *p)
+    {
+#pragma unroll
+        for (int i = 0; i < CW; i++)
+            c.r[i] = coh_load(&p->chain[i]);
     }
     static __device__ __forceinline__ void count(WorldRun *R, int i, unsigned long long n = 1) { atomicAdd(&R->ev[i], n); }
 
@@ -167,7 +191,7 @@ struct Model
         // the slot's message count word (msg_word) and the queue back to memory
         __device__ __forceinline__ void store(int64_t t, int ring_t)
         {
-            B->msg_cnt[(size_t)ring_t * B->nlinks + lid] = msg_word(t, early, sent);
+            coh_store(B->msg_cnt + (size_t)ring_t * B->nlinks + lid, msg_word(t, early, sent));
             B->q_head[lid] = head;
             B->q_size[lid] = size;
             B->q_link_free[lid] = link_free;
@@ -208,7 +232,8 @@ struct Model
                 link_free = start + (int64_t)o.len * PS_PER_BYTE;
                 o.ts = link_free + PROP_PS; // its arrival at the far end
                 early += o.ts < slot_end + SLOT_PS;
-                out[sent++] = o;
+                coh_put(out + sent, o); // (read by the link's far end in a later slot)
+                sent++;
                 head = head + 1 < cap ? head + 1 : 0;
                 if (--size > 0)
                     head_ts = ring[head].ts;
@@ -336,7 +361,22 @@ struct Model
         conn_open(x, f);
         return *x.ls;
     }
-    static __device__ __forceinline__ void snd_close(Host &x, int f) { rec_move(&x.B->snd[f], x.ls); }
+    // The receiver's host reads a sender's end state (compact_rcv): the 4-byte word holding
+    // SndState::st goes out device-coherently once the sender is done.
+    static constexpr size_t ST_WORD = offsetof(SndState, st) & ~(size_t)3;
+    static constexpr int ST_SHIFT = (int)(offsetof(SndState, st) & 3) * 8;
+    static __device__ __forceinline__ void snd_close(Host &x, int f)
+    {
+        rec_move(&x.B->snd[f], x.ls);
+        if (x.ls->st == S_DONE)
+            coh_store(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(&x.B->snd[f]) + ST_WORD),
+                      *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(x.ls) + ST_WORD));
+    }
+    static __device__ __forceinline__ bool snd_done_seen(const Batch &B, int f)
+    {
+        const uint32_t w = coh_load(reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(&B.snd[f]) + ST_WORD));
+        return (uint8_t)(w >> ST_SHIFT) == S_DONE;
+    }
     static __device__ __forceinline__ RcvState &rcv_open(Host &x, int f)
     {
         rec_move(x.lr, &x.B->rcv[f]);
@@ -404,7 +444,7 @@ struct Model
                     f[j] = l[i0 + j];
                     const RcvState &r = B.rcv[f[j]];
                     const int64_t v[4] = {r.delack_at, r.srv_syn_at, r.r_last_at, r.r_frx_at};
-                    gone[j] = B.snd[f[j]].st == S_DONE && v[0] < 0 && v[2] < 0 && v[3] < 0;
+                    gone[j] = snd_done_seen(B, f[j]) && v[0] < 0 && v[2] < 0 && v[3] < 0;
                     tmin[j] = INT64_MAX;
 #pragma unroll
                     for (int q = 0; q < 4; q++)
@@ -1790,11 +1830,11 @@ struct Model
         return base + (size_t)d.flow * CW;
     }
     // a timer and an arrival at one instant: the one scheduled first runs first (undecided: the timer)
-    static __device__ __forceinline__ bool timer_before_arrival(Host &x, const Due &d, const Frame &p)
+    static __device__ __forceinline__ bool timer_before_arrival(Host &x, const Due &d, const Frame *p)
     {
         Chain arr, pc;
-        chain_load(pc, p.chain);
-        chain_child(arr, 2048u + p.len, pc);
+        msg_chain(pc, p);
+        chain_child(arr, 2048u + msg_head(p).len, pc);
         int r = chain_cmp(due_chain(*x.B, d), arr.r);
         if (r == 0)
             count(x.R, EV_UNDECIDED);
@@ -1819,7 +1859,7 @@ struct Model
         if (t >= 2)
         {
             size_t r = (size_t)((t - 2) % NRING) * B.nlinks + in;
-            const long long v = B.msg_cnt[r];
+            const long long v = coh_load(B.msg_cnt + r);
             if (msg_slot(v) == t - 2)
                 i2 = msg_early(v), n2 = msg_sent(v);
             m2 = B.msg + r * MAX_TX;
@@ -1827,7 +1867,7 @@ struct Model
         if (t >= 1)
         {
             size_t r = (size_t)((t - 1) % NRING) * B.nlinks + in;
-            const long long v = B.msg_cnt[r];
+            const long long v = coh_load(B.msg_cnt + r);
             n1 = msg_slot(v) == t - 1 ? msg_early(v) : 0;
             m1 = B.msg + r * MAX_TX;
         }
@@ -1906,7 +1946,7 @@ struct Model
         // own event is taken (the next reads are then in flight while that event runs): the
         // next arrival, the next timer (due_head, which due_insert keeps), the next application
         // start of the host's order.
-        auto next_arrival = [&]() -> int64_t { return i2 < n2 ? m2[i2].ts : i1 < n1 ? m1[i1].ts : INT64_MAX; };
+        auto next_arrival = [&]() -> int64_t { return i2 < n2 ? coh_load(&m2[i2].ts) : i1 < n1 ? coh_load(&m1[i1].ts) : INT64_MAX; };
         auto next_start = [&](int &m) -> int64_t {
             m = pn < pe ? B.pend_msg[pn] : -1;
             if (m >= 0)
@@ -1971,7 +2011,7 @@ struct Model
                 emit(x, f, F_SYN, 0, 0, SYN_FRAME, false, now);
                 snd_close(x, f);
             }
-            else if (now == td && (now != tr || timer_before_arrival(x, due[x.id], i2 < n2 ? m2[i2] : m1[i1])))
+            else if (now == td && (now != tr || timer_before_arrival(x, due[x.id], i2 < n2 ? m2 + i2 : m1 + i1)))
             {
                 Due d = due[x.id++];
                 x.due_head = x.id < x.nd ? due[x.id].at : INT64_MAX;
@@ -2042,7 +2082,7 @@ struct Model
             }
             else
             {
-                Frame p = i2 < n2 ? m2[i2++] : m1[i1++];
+                Frame p = coh_get(i2 < n2 ? m2 + i2++ : m1 + i1++);
                 tr = next_arrival();
                 Chain pc;
                 chain_load(pc, p.chain);
@@ -2116,7 +2156,7 @@ struct Model
             return dst_pod == (sw - E) / hk ? dst_edge : hk + up;
         return dst_pod;
     }
-    static __device__ int route(const Batch &B, const WorldDesc &W, int sw, const Frame &p, int ep)
+    static __device__ int route(const Batch &B, const WorldDesc &W, int sw, const FrameHead &p, int ep)
     {
         if (W.kind == TOPO_GRAPH)
         {
@@ -2152,14 +2192,15 @@ struct Model
     }
     // simultaneous arrivals: the longer frame first, then the upstream starts' chains, then
     // (undecided) the flow id (src/step.cuh:2691-2704)
-    static __device__ bool frame_before(WorldRun *R, const Frame &a, const Frame &b)
+    static __device__ bool frame_before(WorldRun *R, const Frame *a, const Frame *b)
     {
-        if (a.len != b.len)
-            return a.len > b.len;
-        int c = chain_cmp(a.chain, b.chain);
+        const FrameHead ha = msg_head(a), hb = msg_head(b);
+        if (ha.len != hb.len)
+            return ha.len > hb.len;
+        int c = chain_cmp_by([&](int i) { return coh_load(&a->chain[i]); }, [&](int i) { return coh_load(&b->chain[i]); });
         if (c == 0)
             count(R, EV_UNDECIDED);
-        return c != 0 ? c < 0 : a.flow < b.flow;
+        return c != 0 ? c < 0 : ha.flow < hb.flow;
     }
 
     // -------------------------------------------------------------- one slot of a switch
@@ -2185,7 +2226,9 @@ struct Model
         const Frame *m2, *m1;
         int i2, n2, n1;
         __device__ int count() const { return (n2 - i2) + n1; }
-        __device__ const Frame &at(int j) const { return j < n2 - i2 ? m2[i2 + j] : m1[j - (n2 - i2)]; }
+        // the j-th arrival (a frame another entity wrote in an earlier slot: read it device-coherently)
+        __device__ const Frame *at(int j) const { return j < n2 - i2 ? m2 + i2 + j : m1 + (j - (n2 - i2)); }
+        __device__ FrameHead head(int j) const { return msg_head(at(j)); }
     };
     static __device__ In input_of(const Batch &B, int link, int64_t t, const int *win)
     {
@@ -2225,7 +2268,7 @@ struct Model
     {
         return reinterpret_cast<uint8_t *>(L.wave + SW_LDS_EGRESS) + sw_lane_of(L, x) * EGRESS_LDS;
     }
-    static __device__ int lane_egress(const Batch &B, const WorldDesc &W, const SwLane &x, int j, const Frame &p)
+    static __device__ int lane_egress(const Batch &B, const WorldDesc &W, const SwLane &x, int j, const FrameHead &p)
     {
         int ep = W.n_fail ? epoch_of(B, W, p.ts) : 0;
         if (port_down(B, W, x.swl, x.q, ep))
@@ -2245,13 +2288,13 @@ struct Model
         {
             if (t >= 2)
             {
-                const long long v = B.msg_cnt[(size_t)((t - 2) % NRING) * B.nlinks + in_link];
+                const long long v = coh_load(B.msg_cnt + (size_t)((t - 2) % NRING) * B.nlinks + in_link);
                 if (msg_slot(v) == t - 2)
                     w0 = msg_early(v), w1 = msg_sent(v);
             }
             if (t >= 1)
             {
-                const long long v = B.msg_cnt[(size_t)((t - 1) % NRING) * B.nlinks + in_link];
+                const long long v = coh_load(B.msg_cnt + (size_t)((t - 1) % NRING) * B.nlinks + in_link);
                 w2 = msg_slot(v) == t - 1 ? msg_early(v) : 0;
             }
         }
@@ -2273,7 +2316,7 @@ struct Model
         const int n = in.count();
         for (int j = 0; j < n; j++)
         {
-            int e = lane_egress(B, W, x, j, in.at(j));
+            int e = lane_egress(B, W, x, j, in.head(j));
             if (j < EGRESS_LDS)
                 eg[j] = e >= 0 ? (uint8_t)e : NO_EGRESS;
             if (e >= 0)
@@ -2318,7 +2361,7 @@ struct Model
         auto scatter = [&](auto *ts, auto *ref) {
             for (int j = 0; j < n; j++)
             {
-                const Frame &p = in.at(j);
+                const FrameHead p = in.head(j);
                 const int e = j < EGRESS_LDS ? (eg[j] == NO_EGRESS ? -1 : (int)eg[j]) : lane_egress(B, W, x, j, p);
                 if (e < 0)
                     continue;
@@ -2354,7 +2397,7 @@ struct Model
         if (!x.active)
             return 0;
         const int n = L.cnt[x.q], b0 = L.base[x.q];
-        auto frame_at = [&](uint32_t r) -> const Frame & { return lane_input(B, W, x, t, L, (int)(r >> 16)).at((int)(r & 0xFFFF)); };
+        auto frame_at = [&](uint32_t r) -> const Frame * { return lane_input(B, W, x, t, L, (int)(r >> 16)).at((int)(r & 0xFFFF)); };
         Fifo eg;
         const int rt = (int)(t % NRING);
         eg.load(B, R, x.lid, t, rt);
@@ -2380,7 +2423,7 @@ struct Model
             }
             for (int i = 0; i < n; i++)
             {
-                Frame p = frame_at(ref[i]);
+                Frame p = coh_get(frame_at(ref[i]));
                 Chain up, c;
                 chain_load(up, p.chain);
                 chain_child(c, 2048u + p.len, up);
@@ -2482,7 +2525,7 @@ struct Model
         bool done;
     };
     // the packed entry of arrival j: time offset (20 bits), route count or 255 (a down port)
-    static __device__ uint32_t spray_code(const Batch &B, const WorldDesc &W, const SwLane &x, int64_t S0, const Frame &f)
+    static __device__ uint32_t spray_code(const Batch &B, const WorldDesc &W, const SwLane &x, int64_t S0, const FrameHead &f)
     {
         const int ep = W.n_fail ? epoch_of(B, W, f.ts) : 0;
         const uint32_t nr = port_down(B, W, x.swl, x.q, ep) ? 255u : (uint32_t)routes_to(B, W, x.swl, f.dst, ep);
@@ -2494,7 +2537,7 @@ struct Model
     {
         if (j < SPRAY_CACHE)
             return (uint32_t)L.wave[spray_lane(x, L) * 33 + j];
-        return spray_code(B, W, x, S0, sp.in.at(j));
+        return spray_code(B, W, x, S0, sp.in.head(j));
     }
     static __device__ void spray_init(const Batch &B, const WorldDesc &W, const SwLane &x, int64_t t, const SwLds &L, Spray &sp)
     {
@@ -2510,7 +2553,7 @@ struct Model
         const int lane = spray_lane(x, L);
         const int n = sp.c < SPRAY_CACHE ? sp.c : SPRAY_CACHE;
         for (int j = 0; j < n; j++)
-            L.wave[lane * 33 + j] = (int)spray_code(B, W, x, S0, sp.in.at(j));
+            L.wave[lane * 33 + j] = (int)spray_code(B, W, x, S0, sp.in.head(j));
         if (sp.c > 0)
             sp.head = (uint32_t)L.wave[lane * 33]; // (the lane's own write)
         const TwMrgState &gs = B.sw_rng[W.rng0 + x.swl];
@@ -2554,8 +2597,8 @@ struct Model
         {
             if (L.cnt[q] == (int)SPRAY_NONE || ((uint32_t)L.cnt[q] & 0xFFFFFu) != bt)
                 continue;
-            const Frame &f = lane_input(B, W, x, t, L, q).at(L.base[q]);
-            const Frame &b = lane_input(B, W, x, t, L, best).at(L.base[best]);
+            const Frame *f = lane_input(B, W, x, t, L, q).at(L.base[q]);
+            const Frame *b = lane_input(B, W, x, t, L, best).at(L.base[best]);
             if (frame_before(R, f, b))
                 best = q;
         }
