@@ -85,6 +85,7 @@ def test_noalloc_entry_on_cpu(harness, tmp_path):
     assert r.returncode == 0, r.stderr[-3000:]
     same, n = util.same_times(os.path.join(util.GOLDEN, ref), out)
     assert same == n
-    r = subprocess.run([harness] + args + [f"--flows={fl}", f"--out={out}", "--noalloc=1", "--blocks=40"], cwd=util.GOLDEN,
+    r = subprocess.run([harness] + args + [f"--flows={fl}", f"--out={out}", "--noalloc=1", "--blocks=40", "--team_blocks=40"],
+                       cwd=util.GOLDEN,
                        capture_output=True, text=True, timeout=600, env={**os.environ, "ASAN_OPTIONS": "detect_leaks=0"})
     assert r.returncode == 1 and "static scratch" in r.stderr

@@ -1227,16 +1227,16 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
         // one queue, largest worlds first (the long ones start early); few large worlds are
         // spread over the GPU, many run a workgroup each
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return size_of(x) > size_of(y); });
-        // a team with a wave for every work item of the largest world at six hosts per item, at
-        // most 64 workgroups for a world alone and 32 for each of several. Measured on single
-        // FatTree worlds: k=8 (208 entities) ran 0.64 s on 4 workgroups and 0.39-0.44 s on 9-52;
-        // k=16 (1,344) 0.85-0.95 s on 64 workgroups (251 items on 256 waves) and slower on 80-128
-        // (the team barrier's cost grows with T). Three k=16 worlds side by side (config 2):
-        // 150, 154, 129, 153 and 184 ms per run on teams of 16, 21, 32, 48 and 64.
+        // a team with a wave for every work item of the largest world at two hosts per item, at
+        // most 128 workgroups for a world alone and 48 for each of several. Measured (slot barrier
+        // without cache fences): a FatTree k=16 world alone (k16_a2a50s, DCTCP) ran 0.69, 0.57,
+        // 0.52, 0.49 and 0.47 s on teams of 32, 48, 64, 96 and 128; k8_paper 0.49, 0.40, 0.38 and
+        // 0.35 s on 4, 9, 16 and 32; three k=16 worlds side by side (config 2) 134, 108, 100, 97
+        // and 98 ms per run on 16, 24, 32, 48 and 64.
         const tw_world &big = *b.worlds[order[0]];
         const int waves_per = std::max(1, (cfg.block_threads > 0 ? cfg.block_threads : 256) / 64);
-        const int want = (slot_items_at(big.H, big.S, big.k, 6) + waves_per - 1) / waves_per;
-        const int most = W == 1 ? 64 : 32;
+        const int want = (slot_items_at(big.H, big.S, big.k, 2) + waves_per - 1) / waves_per;
+        const int most = W == 1 ? 128 : 48;
         int T = cfg.team_blocks > 0 ? cfg.team_blocks : (W >= cus ? 1 : std::max(1, std::min({resident / W, most, want})));
         if (cfg.blocks > 0 && cfg.team_blocks <= 0)
             T = std::min(T, cfg.blocks); // (a launch of fewer workgroups than the automatic team)
