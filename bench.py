@@ -219,8 +219,9 @@ def serve_leg(a):
     rblocks = a.serve_rdma_blocks if n_rdma else 0
     # (the TCP loop also leaves an eighth of the CUs free for the uploads' fill kernels)
     cus = resident  # (one slot-kernel workgroup per CU)
-    nm, nl = a.serve_mid_teams, a.serve_large_teams
-    srv = twins_amd.Server(idle_s=2.0, mid_teams=nm, large_teams=nl, small_teams=resident - cus // 8 - rblocks - 16 * nl - 4 * nm)
+    nm, nl, zm, zl = a.serve_mid_teams, a.serve_large_teams, a.serve_mid_size, a.serve_large_size
+    srv = twins_amd.Server(idle_s=2.0, mid_teams=nm, large_teams=nl, mid_size=zm, large_size=zl,
+                           small_teams=resident - cus // 8 - rblocks - zl * nl - zm * nm)
     rsrv = rdma.Server(max_blocks=rblocks) if n_rdma else None
     # every serve_check-th request against the oracle (TCP) or the reference's RDMA build, on the
     # host's cores, each as soon as its result is in (the checks overlap the serving)
@@ -536,8 +537,10 @@ def main():
     ap.add_argument("--serve_check", type=int, default=1, help="every n-th request is checked against the oracle")
     ap.add_argument("--serve_rdma", type=float, default=0.1, help="the share of the serving loop's requests that are RDMA worlds")
     ap.add_argument("--serve_rdma_blocks", type=int, default=64, help="workgroups the RDMA serving loop's launches take")
-    ap.add_argument("--serve_mid_teams", type=int, default=8, help="the TCP serving loop's teams of 4 workgroups (FatTree k=16 worlds)")
-    ap.add_argument("--serve_large_teams", type=int, default=4, help="the TCP serving loop's teams of 16 workgroups (k=32 worlds)")
+    ap.add_argument("--serve_mid_teams", type=int, default=8, help="the TCP serving loop's mid teams (FatTree k=16 worlds)")
+    ap.add_argument("--serve_large_teams", type=int, default=4, help="the TCP serving loop's large teams (k=32 worlds)")
+    ap.add_argument("--serve_mid_size", type=int, default=4, help="workgroups of the TCP serving loop's mid teams")
+    ap.add_argument("--serve_large_size", type=int, default=16, help="workgroups of the TCP serving loop's large teams")
     ap.add_argument("--rdma", type=int, default=1, help="also run config 3, the RDMA model (the \"rdma\" object)")
     ap.add_argument("--refspeed", type=int, default=1, help="also time the reference's published single-world workloads")
     ap.add_argument("--ref_runs", type=int, default=3, help="runs per reference workload (the best is reported)")

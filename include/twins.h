@@ -183,7 +183,7 @@ int tw_world_stats_get(tw_batch *b, int world, tw_world_stats *st);
 int tw_batch_destroy(tw_batch *b);
 
 /* The serving loop: one persistent kernel serves requests (one world each) as they arrive.
- * A request goes to the kernel's queue for its size (teams of 1, 4 or 16 workgroups). It is
+ * A request goes to the kernel's queue for its size (teams of 1, mid_size or large_size workgroups). It is
  * admitted in arrival order within that queue once its device footprint fits the server's arena
  * and the queue holds fewer than 4 resident requests per team; a request waiting for either
  * does not hold up the other queues. Admitted, it is packed into the arena and stays resident
@@ -196,11 +196,13 @@ typedef struct tw_server_cfg
     int32_t chain_runs;       /* 9 or 25 (0: 9) */
     int32_t block_threads;    /* 0: 256 */
     int32_t small_teams;      /* teams of one workgroup (0: what the GPU holds after the others and a free share) */
-    int32_t mid_teams;        /* teams of 4 workgroups, for worlds of 513-4096 hosts and switches (0: 8) */
-    int32_t large_teams;      /* teams of 16, for larger worlds (0: 4) */
+    int32_t mid_teams;        /* mid teams (mid_size workgroups), for worlds of 513-4096 hosts and switches (0: 8) */
+    int32_t large_teams;      /* large teams (large_size workgroups), for larger worlds (0: 4) */
     int32_t max_requests;     /* requests over the server's life (0: 65536) */
     uint64_t arena_bytes;     /* device memory for resident requests (0: 60% of the free memory) */
     double idle_s;            /* a kernel team idle this long leaves; the kernel is relaunched on demand (0: 2 s) */
+    int32_t mid_size;         /* workgroups of a mid team (0: 4) */
+    int32_t large_size;       /* workgroups of a large team (0: 16) */
 } tw_server_cfg;
 enum { TW_REQ_WAITING = 0, TW_REQ_RESIDENT = 1, TW_REQ_DONE = 2, TW_REQ_FAILED = 3, TW_REQ_TAKEN = 4 };
 typedef struct tw_server_stats

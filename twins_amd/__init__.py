@@ -54,7 +54,8 @@ class WorldInfo(ctypes.Structure):
 class ServerCfg(ctypes.Structure):
     _fields_ = [("chain_runs", ctypes.c_int32), ("block_threads", ctypes.c_int32), ("small_teams", ctypes.c_int32),
                 ("mid_teams", ctypes.c_int32), ("large_teams", ctypes.c_int32), ("max_requests", ctypes.c_int32),
-                ("arena_bytes", ctypes.c_uint64), ("idle_s", ctypes.c_double)]
+                ("arena_bytes", ctypes.c_uint64), ("idle_s", ctypes.c_double), ("mid_size", ctypes.c_int32),
+                ("large_size", ctypes.c_int32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -244,9 +245,10 @@ class Server:
     and counters."""
 
     def __init__(self, chain_runs=9, block_threads=0, small_teams=0, mid_teams=0, large_teams=0, max_requests=0,
-                 arena_bytes=0, idle_s=0.0):
+                 arena_bytes=0, idle_s=0.0, mid_size=0, large_size=0):
         L = lib()
-        cfg = ServerCfg(chain_runs, block_threads, small_teams, mid_teams, large_teams, max_requests, arena_bytes, idle_s)
+        cfg = ServerCfg(chain_runs, block_threads, small_teams, mid_teams, large_teams, max_requests, arena_bytes, idle_s,
+                        mid_size, large_size)
         self._h = ctypes.c_void_p()
         self._worlds = {}
         _check(L.tw_server_create(ctypes.byref(cfg), ctypes.byref(self._h)))

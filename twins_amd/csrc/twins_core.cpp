@@ -1227,15 +1227,16 @@ static int pack(tw_batch &b, const tw_pack_cfg &cfg)
         // one queue, largest worlds first (the long ones start early); few large worlds are
         // spread over the GPU, many run a workgroup each
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return size_of(x) > size_of(y); });
-        // a team with a wave for every work item of the largest world at two hosts per item, at
-        // most 128 workgroups for a world alone and 48 for each of several. Measured (slot barrier
-        // without cache fences): a FatTree k=16 world alone (k16_a2a50s, DCTCP) ran 0.69, 0.57,
-        // 0.52, 0.49 and 0.47 s on teams of 32, 48, 64, 96 and 128; k8_paper 0.49, 0.40, 0.38 and
-        // 0.35 s on 4, 9, 16 and 32; three k=16 worlds side by side (config 2) 134, 108, 100, 97
-        // and 98 ms per run on 16, 24, 32, 48 and 64.
+        // a team with a wave for every work item of the largest world, at one host per item for a
+        // world alone (at most 128 workgroups) and two for each of several (at most 48). Measured
+        // (slot barrier without cache fences): a FatTree k=16 world alone (k16_a2a50s, DCTCP) ran
+        // 0.69, 0.57, 0.52, 0.49 and 0.47 s on teams of 32, 48, 64, 96 and 128; k8_paper (DCTCP)
+        // 0.342, 0.344, 0.321 and 0.318 s on 26, 40, 52 and 64 (52: one host per item), k8_a2a30
+        // 0.397, 0.397, 0.360 and 0.362 s; three k=16 worlds side by side (config 2) 134, 108, 100,
+        // 97 and 98 ms per run on 16, 24, 32, 48 and 64.
         const tw_world &big = *b.worlds[order[0]];
         const int waves_per = std::max(1, (cfg.block_threads > 0 ? cfg.block_threads : 256) / 64);
-        const int want = (slot_items_at(big.H, big.S, big.k, 2) + waves_per - 1) / waves_per;
+        const int want = (slot_items_at(big.H, big.S, big.k, W == 1 ? 1 : 2) + waves_per - 1) / waves_per;
         const int most = W == 1 ? 128 : 48;
         int T = cfg.team_blocks > 0 ? cfg.team_blocks : (W >= cus ? 1 : std::max(1, std::min({resident / W, most, want})));
         if (cfg.blocks > 0 && cfg.team_blocks <= 0)
@@ -1832,9 +1833,13 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
     // GPU stays free for the uploads' fill kernels
     const int nl = c.large_teams > 0 ? c.large_teams : 4, nm = c.mid_teams > 0 ? c.mid_teams : 8;
     const int headroom = cus / 8;
-    const int ns = c.small_teams > 0 ? c.small_teams : resident - headroom - 16 * nl - 4 * nm;
-    if (ns < 1 || ns + 4 * nm + 16 * nl > resident)
-        return fail(TW_EINVAL, "server teams (%d + 4 x %d + 16 x %d workgroups) do not fit the %d the GPU holds", ns, nm, nl, resident);
+    const int zm = c.mid_size > 0 ? c.mid_size : 4, zl = c.large_size > 0 ? c.large_size : 16;
+    if (zm > 255 || zl > 255)
+        return fail(TW_EINVAL, "server teams of at most 255 workgroups");
+    const int ns = c.small_teams > 0 ? c.small_teams : resident - headroom - zl * nl - zm * nm;
+    if (ns < 1 || ns + zm * nm + zl * nl > resident)
+        return fail(TW_EINVAL, "server teams (%d + %d x %d + %d x %d workgroups) do not fit the %d the GPU holds", ns, zm, nm, zl, nl,
+                    resident);
     std::vector<TeamPlan> plan;
     std::vector<int32_t> team_of;
     auto add = [&](int n, int T, int q) {
@@ -1842,8 +1847,8 @@ extern "C" int tw_server_create(const tw_server_cfg *cfg, tw_server **out)
             plan.push_back(TeamPlan{0, T, 0, q});
     };
     add(ns, 1, 0);
-    add(nm, 4, 1);
-    add(nl, 16, 2);
+    add(nm, zm, 1);
+    add(nl, zl, 2);
     s->q_teams[0] = ns, s->q_teams[1] = nm, s->q_teams[2] = nl;
     place_teams(plan, team_of);
     s->blocks = (int)team_of.size();

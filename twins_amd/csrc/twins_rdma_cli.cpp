@@ -112,6 +112,10 @@ int main(int argc, char **argv)
         }
         else if (k == "--serial")
             pc.serial = 1; // every switch slot in order on one lane (src/rdma/rmain.cu:1090-1091, 1949)
+        else if (k == "--team_blocks") // (launch shape, no counterpart in the reference: workgroups per world)
+            pc.team_blocks = atoi(v.c_str());
+        else if (k == "--hosts_per_wave") // (likewise: hosts side by side on a wave's lanes)
+            pc.hosts_per_wave = atoi(v.c_str());
         else if (k == "--spread" || k == "--win" || k == "--gwin" || k == "--persist" || k.compare(0, 6, "--rank") == 0 ||
                  k == "--dcap") // (--dcap sizes DAG message buffers; DAG flow files are refused)
             continue;
@@ -248,6 +252,20 @@ int main(int argc, char **argv)
                (unsigned long long)ev[11], (unsigned long long)ev[8], (unsigned long long)ev[9], (unsigned long long)ev[4],
                (unsigned long long)ev[5], (unsigned long long)ev[6], (unsigned long long)ev[7], (unsigned long long)ev[12],
                (unsigned long long)ev[10]);
+        if ((ev[33] || ev[34]) && st.slots > 0)
+        {
+            // (profiling builds: wave-time of host items and of switch items summed over the team, and
+            // the first workgroup's wait at the barrier, per slot; constant clock of 100 MHz)
+            const double k = 0.01 / (double)st.slots;
+            printf("prof: per slot: host items %.2f wave-us, switch items %.2f wave-us, barrier %.2f us (%lld slots)\n",
+                   ev[33] * k, ev[34] * k, ev[35] * k, (long long)st.slots);
+            // the switch slots' phases (ports in, routes, events, ports out), their arrivals, and in the
+            // events of safe slots the egress merge, TransmitCompletes and arrivals (a wave's time each)
+            printf("prof: switch phases per slot (wave-us): in %.2f, routes %.2f, events %.2f, out %.2f; arrivals %.1f; "
+                   "merge %.2f, tx-complete %.2f, arrival events %.2f; safe slots %llu, serial %llu\n",
+                   ev[24] * k, ev[25] * k, ev[26] * k, ev[27] * k, ev[28] / (double)st.slots, ev[29] * k, ev[30] * k, ev[31] * k,
+                   (unsigned long long)ev[36], (unsigned long long)ev[37]);
+        }
         if (!txtrace.empty())
         {
             uint64_t n = 0, seen = 0;
