@@ -40,9 +40,11 @@ typedef struct trd_pack_cfg
     int32_t tail_log2;   /* tail_exact 0: a world's ring of chain tails holds at most 2^tail_log2 nodes (0: 24; a world of E entities
                             2^(log2 E + 14), at least 2^16); tail_exact 1: every world's ring holds 2^tail_log2 nodes (the
                             reference's --tail, src/rdma/rmain.cu:29-31, 1078-1085) */
-    int32_t team_blocks; /* workgroups (of 256 threads) per world (0: from the largest world, at most 32, or 16 beyond 8 worlds) */
-    int32_t hosts_per_wave; /* hosts side by side on a wave's lanes (0: batches of up to 8 worlds 8, or 16 where a world's
-                               work items would outnumber a 32-workgroup team's waves; larger batches 64) */
+    int32_t team_blocks; /* workgroups (of 256 threads) per world (0: batches of up to 8 worlds a workgroup per work item of
+                            the largest world, at most 128 for a world alone or the GPU's share of each world (at least
+                            32); larger batches a wave per work item, at most 16) */
+    int32_t hosts_per_wave; /* hosts side by side on a wave's lanes (0: batches of up to 8 worlds the fewest of 2, 4, 8 or 16
+                               whose items fit a wave each of the largest team; larger batches 64) */
     int32_t tail_exact;  /* see tail_log2 */
     int32_t serial;      /* 1: every switch slot runs its events in order on one lane, never the per-port parallel safe
                             path (the reference's --serial, src/rdma/rmain.cu:1090-1091, 1949); results are the same */

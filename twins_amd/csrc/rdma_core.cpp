@@ -924,18 +924,31 @@ extern "C" int trd_batch_run(trd_batch *b, trd_run_stats *st)
     double kernel_s = 0;
     int64_t prev = -1;
     // the launch shape: hosts per host item (fewer: less divergence among a wave's hosts, more
-    // waves), and one team size, from the largest world's work items (about one per wave, at most 32
-    // workgroups); as many teams as the GPU holds at once. Small batches take 8 hosts per item where
-    // every world's items still fit one per wave of a 32-workgroup team (512-host leaf-spine, HPCC + DCQCN:
-    // 1.19 s of kernel against 1.45 s with 16; with 6 the items outnumber the waves: 1.72 s), else 16.
+    // waves), and one team size. Batches of up to 8 worlds take the fewest hosts per item (2, 4, 8
+    // or 16) whose items still fit a wave each of the largest team - 128 workgroups for a world
+    // alone, the GPU's share of each of up to 8 worlds (at least 32) - and a workgroup per item of
+    // the largest world up to that team. Measured with the slot barrier without cache fences:
+    // FatTree k=8 a2a50s (HPCC) alone 1.14, 0.80 and 0.70 s on 24 workgroups at 8 hosts per item, 64
+    // at 4 and 128 at 2; k8_perm 0.196 and 0.132 s on 24 at 8 and 96 at 2; Jellyfish a2a30 1.44 and
+    // 1.20 s on 12 at 8 and 96 at 2; FatTree k=16 a2a30 (DCQCN) 1.51 s on 112 at 8 and on 192 or 256
+    // alike; the 512-host leaf-spine (HPCC + DCQCN side by side) 1.27 s of kernel on 22 at 8 (a wave
+    // per item), 1.15 s on 48 at 4 and on 128 at 2. Larger batches: 64 hosts per item, teams of 16
+    // workgroups with a wave per item. As many teams as the GPU holds at once.
     const int cap = b->max_teams;
+    const bool small = b->W <= 8;
+    const int most = b->W == 1 ? std::min(128, cap) : small ? std::max(32, cap / b->W) : 16;
     int hpw = b->hpw;
     if (hpw <= 0)
     {
-        hpw = b->W <= 8 ? 8 : 64;
-        for (int i = 0; i < b->W && hpw == 8; i++)
-            if ((b->hs[i] + 7) / 8 + b->ss[i] > 32 * 4)
-                hpw = 16;
+        hpw = small ? 2 : 64;
+        for (bool fits = false; small && !fits && hpw < 16;)
+        {
+            fits = true;
+            for (int i = 0; i < b->W; i++)
+                fits = fits && (b->hs[i] + hpw - 1) / hpw + b->ss[i] <= most * 4;
+            if (!fits)
+                hpw *= 2;
+        }
     }
     vector<int> items(b->W);
     for (int i = 0; i < b->W; i++)
@@ -946,7 +959,7 @@ extern "C" int trd_batch_run(trd_batch *b, trd_run_stats *st)
         int mx = 1;
         for (int v : items)
             mx = std::max(mx, v);
-        T = std::min(b->W <= 8 ? 32 : 16, (mx + 3) / 4); // (large batches: more teams at once)
+        T = std::min(most, small ? mx : (mx + 3) / 4);
     }
     T = std::max(1, std::min(T, cap));
     // worlds by work items, largest first

@@ -17,8 +17,6 @@ constexpr long long LAUNCH_TICKS = 100000000;          // a launch's time: 1 s o
 constexpr unsigned long long WAIT_TICKS = 2000000000ull; // a barrier wait gives up after 20 s
 constexpr int WAVES = LAUNCH_WAVES;                    // waves of a 256-thread workgroup
 
-__device__ __forceinline__ unsigned ld_acquire(const unsigned *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
-
 // (its LDS holds one workgroup per CU: one wave per SIMD, so all registers may be used)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) rdma_team_kernel(const LaunchCfg *__restrict__ lp)
 {
@@ -38,29 +36,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
     const long long t_launch = wall_clock64();
     unsigned gen = 0;
 
-    // the team barrier; the last workgroup to arrive runs `last` first
-    auto barrier = [&](auto last) -> bool {
+    // the team barrier; the last workgroup to arrive runs `last` first. Within a world (fenced
+    // false) it has no cache fences: what the workgroups hand each other goes through coherent
+    // accesses (rdma_model.h), so each wave only waits for its own stores to complete, and the
+    // entities' own records stay in the caches from slot to slot (an agent-scope fence on the
+    // multi-XCD GPU writes back and invalidates the L2). Claiming a world and leaving one are fenced.
+    auto barrier = [&](auto last, bool fenced) -> bool {
+        __builtin_amdgcn_s_waitcnt(0); // (this wave's stores have completed)
         __syncthreads();
         if (tid == 0)
         {
+            if (fenced)
+                __threadfence();
             if (T == 1)
                 last();
             else
             {
-                __threadfence();
                 const unsigned g = gen;
                 if (atomicAdd(&ts->arrive, 1u) == (unsigned)T - 1)
                 {
-                    __threadfence(); // (what the others wrote before arriving, e.g. pending-slot flags)
                     last();
                     atomicExch(&ts->arrive, 0u);
-                    __threadfence();
+                    __builtin_amdgcn_s_waitcnt(0); // (the ticket's stores before the release of the others)
                     atomicAdd(&ts->gen, 1u);
                 }
                 else
                 {
                     const unsigned long long w0 = wall_clock64();
-                    while (ld_acquire(&ts->gen) == g)
+                    while (coh_load(&ts->gen) == g)
                     {
                         __builtin_amdgcn_s_sleep(1);
                         if (wall_clock64() - w0 > WAIT_TICKS)
@@ -70,9 +73,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
                         }
                     }
                 }
-                __threadfence();
                 gen = g + 1;
             }
+            if (fenced)
+                __threadfence();
             s_err = T > 1 && __hip_atomic_load(&ts->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
@@ -87,7 +91,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
                 ts->act = 0;
                 ts->wake = ~0ull;
                 ts->stop = 0;
-            }))
+            }, true))
             return;
         if (tid == 0)
             s_w = __hip_atomic_load(&ts->world, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -192,11 +196,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))
             }
             if (barrier([&] {
                     slot_end(s, wd, r, pend, atomicOr(&ts->act, 0u), atomicMin(&ts->wake, ~0ull));
-                    ts->act = 0;
-                    ts->wake = ~0ull;
+                    coh_store(&ts->act, 0u);
+                    coh_store(&ts->wake, ~0ull);
                     if (wall_clock64() - t_launch > LAUNCH_TICKS)
-                        ts->stop = 1;
-                }))
+                        coh_store(&ts->stop, 1);
+                }, false))
                 return;
 #ifdef RD_PROF
             if (tid == 0 && rank == 0)

@@ -11,8 +11,10 @@
 // their event chains) from what earlier slots filed.
 #pragma once
 #include <cstdint>
+#include <cstddef>
 #include <climits>
 #include <cmath>
+#include "coherent.h"
 
 // Device memory, for the device compile: pointers of the Sim record carry the global address space,
 // so the compiler emits global (not flat) loads and stores through them and what is derived from
@@ -35,6 +37,14 @@
 
 namespace rd
 {
+// Device-coherent accesses (coherent.h). What a world's entities hand each other between slots
+// goes only through these: link frames and their buckets' count words, the pending-slot flags,
+// chain tail nodes, the lead-hash memo and the world's ticket. So a slot's team barrier needs no
+// cache fences; each entity's own records stay in its workgroup's caches from slot to slot.
+using twc::coh_get;
+using twc::coh_load;
+using twc::coh_put;
+using twc::coh_store;
 // ------------------------------------------------------------------ constants (src/rdma/rdma.h:101-106, 119-121)
 constexpr int RUNS = 6;                 // chain runs kept inline
 constexpr int NANC = 4;                 // ancestor lead hashes a chain carries (the memo's depths)
@@ -121,6 +131,102 @@ struct LFrame
     Chain c;    // the chain of the event that started its transmission
     Pkt p;
 };
+// Frames and chains through coherent accesses, word by word as they lie in memory (the words built
+// from the fields in registers: a record copied through memcpy would go to scratch memory).
+static_assert(offsetof(Pkt, flow) == 40 && offsetof(Pkt, seq) == 44 && offsetof(Pkt, size) == 48 && offsetof(Pkt, nhop) == 50 &&
+                  offsetof(Pkt, kind) == 52 && offsetof(Pkt, ecn) == 53 && offsetof(Pkt, cnp) == 54 && offsetof(Pkt, q) == 55 &&
+                  offsetof(Pkt, inport) == 56 && offsetof(Pkt, ord) == 57 && offsetof(Pkt, sport) == 58 && offsetof(Pkt, src) == 60 &&
+                  offsetof(Pkt, dst) == 62,
+              "Pkt layout");
+static_assert(offsetof(Chain, tail) == 8 * RUNS && offsetof(Chain, hash) == 8 * RUNS + 8 && offsetof(Chain, anc) == 8 * RUNS + 16 &&
+                  offsetof(Chain, root) == 8 * RUNS + 16 + 8 * NANC && sizeof(Chain) == 8 * RUNS + 24 + 8 * NANC,
+              "Chain layout");
+RD_HD uint64_t pkt_w6(const Pkt &p)
+{
+    return (uint64_t)p.size | (uint64_t)p.nhop << 16 | (uint64_t)p.kind << 32 | (uint64_t)p.ecn << 40 | (uint64_t)p.cnp << 48 |
+           (uint64_t)p.q << 56;
+}
+RD_HD uint64_t pkt_w7(const Pkt &p)
+{
+    return (uint64_t)p.inport | (uint64_t)p.ord << 8 | (uint64_t)p.sport << 16 | (uint64_t)p.src << 32 | (uint64_t)p.dst << 48;
+}
+template <typename PP>
+RD_HD void pkt_put(PP d, const Pkt &p)
+{
+    auto w = reinterpret_cast<decltype(&d->hop[0])>(d);
+    for (int i = 0; i < MAX_HOP; i++)
+        coh_store(w + i, p.hop[i]);
+    coh_store(w + 5, (uint64_t)p.flow | (uint64_t)p.seq << 32);
+    coh_store(w + 6, pkt_w6(p));
+    coh_store(w + 7, pkt_w7(p));
+}
+template <typename PP>
+RD_HD Pkt pkt_get(PP f)
+{
+    Pkt p;
+    auto w = reinterpret_cast<decltype(&f->hop[0])>(f);
+    for (int i = 0; i < MAX_HOP; i++)
+        p.hop[i] = coh_load(w + i);
+    const uint64_t w5 = coh_load(w + 5), w6 = coh_load(w + 6), w7 = coh_load(w + 7);
+    p.flow = (uint32_t)w5;
+    p.seq = (uint32_t)(w5 >> 32);
+    p.size = (uint16_t)w6;
+    p.nhop = (uint16_t)(w6 >> 16);
+    p.kind = (uint8_t)(w6 >> 32);
+    p.ecn = (uint8_t)(w6 >> 40);
+    p.cnp = (uint8_t)(w6 >> 48);
+    p.q = (uint8_t)(w6 >> 56);
+    p.inport = (uint8_t)w7;
+    p.ord = (uint8_t)(w7 >> 8);
+    p.sport = (uint16_t)(w7 >> 16);
+    p.src = (uint16_t)(w7 >> 32);
+    p.dst = (uint16_t)(w7 >> 48);
+    return p;
+}
+static_assert(offsetof(Run, cnt) == 4 && sizeof(Run) == 8 && offsetof(Chain, pad) == offsetof(Chain, root) + 4, "Run, Chain words");
+template <typename CP>
+RD_HD void chain_put(CP d, const Chain &c)
+{
+    auto w = reinterpret_cast<decltype(&d->tail)>(d);
+    for (int i = 0; i < RUNS; i++)
+        coh_store(w + i, (uint64_t)c.r[i].code | (uint64_t)c.r[i].cnt << 32);
+    coh_store(w + RUNS, c.tail);
+    coh_store(w + RUNS + 1, c.hash);
+    for (int i = 0; i < NANC; i++)
+        coh_store(w + RUNS + 2 + i, c.anc[i]);
+    coh_store(w + RUNS + 2 + NANC, (uint64_t)c.root);
+}
+template <typename CP>
+RD_HD Chain chain_get(CP f)
+{
+    Chain c;
+    auto w = reinterpret_cast<decltype(&f->tail)>(f);
+    for (int i = 0; i < RUNS; i++)
+    {
+        const uint64_t x = coh_load(w + i);
+        c.r[i].code = (uint32_t)x;
+        c.r[i].cnt = (uint32_t)(x >> 32);
+    }
+    c.tail = coh_load(w + RUNS);
+    c.hash = coh_load(w + RUNS + 1);
+    for (int i = 0; i < NANC; i++)
+        c.anc[i] = coh_load(w + RUNS + 2 + i);
+    c.root = (uint32_t)coh_load(w + RUNS + 2 + NANC);
+    c.pad = 0;
+    return c;
+}
+// what a receiver looks at first: the arrival time, the size and the ordinal
+struct FrameHead
+{
+    int64_t ts;
+    uint32_t size;
+    uint32_t ord;
+};
+RD_HD FrameHead frame_head(const LFrame RD_G *f)
+{
+    auto w = reinterpret_cast<const uint64_t RD_G *>(&f->p);
+    return FrameHead{coh_load(&f->ts), (uint32_t)(uint16_t)coh_load(w + 6), (uint32_t)(uint8_t)(coh_load(w + 7) >> 8)};
+}
 
 struct Params
 {
@@ -481,7 +587,7 @@ RD_HD int bucket(const Sim &s, int link, int64_t slot, uint64_t *base)
 {
     const Link &l = s.li[link];
     int b = (int)(slot % (int64_t)l.nb);
-    int64_t v = s.msg_cnt[l.coff + b];
+    const int64_t v = coh_load(s.msg_cnt + l.coff + b);
     *base = l.moff + (uint64_t)b * l.cap;
     return (v >> 16) == slot ? (int)(v & 0xFFFF) : 0;
 }
@@ -514,11 +620,11 @@ RD_HD void ob_flush(const Sim &s, OutBucket &o)
     if (o.hi == 0)
         return;
     o.last = (o.lo / s.D) << 16 | o.n;
-    s.msg_cnt[o.cnt] = o.last;
+    coh_store(s.msg_cnt + o.cnt, o.last);
     o.hi = 0;
 }
 // a place in the bucket of the frame's arrival slot (null: full, counted)
-RD_HD LFrame *ob_push(const Sim &s, uint32_t *pend, int pend_n, int64_t ts, OutBucket &o)
+RD_HD LFrame RD_G *ob_push(const Sim &s, uint32_t *pend, int pend_n, int64_t ts, OutBucket &o)
 {
     if (ts >= o.hi)
     {
@@ -530,7 +636,7 @@ RD_HD LFrame *ob_push(const Sim &s, uint32_t *pend, int pend_n, int64_t ts, OutB
         o.hi = o.lo + s.D;
         o.base = o.moff + (uint64_t)b * o.cap;
         o.n = (o.last >> 16) == a ? (uint32_t)(o.last & 0xFFFF) : 0;
-        pend[a % pend_n] = 1;
+        coh_store(pend + a % pend_n, 1u);
     }
     if (o.n >= o.cap)
     {
@@ -590,11 +696,11 @@ RD_HD void cpush(const Sim &s, TailAlloc &ta, Chain *dst, uint32_t code, const C
                 ta.end = ta.next + TAIL_CHUNK;
             }
             const uint64_t pos = ta.next++;
-            TailNode &n = s.tail[tail_slot(s, pos)];
-            n.code = par.r[RUNS - 1].code;
-            n.cnt = par.r[RUNS - 1].cnt;
-            n.next = par.tail;
-            n.pos = pos;
+            TailNode RD_G *n = s.tail + tail_slot(s, pos);
+            coh_store(&n->code, par.r[RUNS - 1].code);
+            coh_store(&n->cnt, par.r[RUNS - 1].cnt);
+            coh_store(&n->next, par.tail);
+            coh_store(&n->pos, pos);
             c.tail = pos;
         }
     }
@@ -622,20 +728,21 @@ RD_HD void rr_fetch(const Sim &s, RunReader &r)
 {
     if (r.at < RUNS)
     {
-        const Run &x = r.c->r[r.at];
+        const Run x = coh_get(&r.c->r[r.at]);
         r.code = x.code;
         r.left = x.cnt;
         if (x.code || r.at + 1 < RUNS)
             return; // an inline run, or the chain's end within them
         r.at = RUNS + 1; // every inline run is full: the tail follows
-        r.node = r.c->tail;
+        r.node = coh_load(&r.c->tail);
     }
     if (!r.node)
     {
         r.code = 0;
         return;
     }
-    const TailNode n = s.tail[tail_slot(s, r.node)];
+    const TailNode RD_G *tn = s.tail + tail_slot(s, r.node);
+    const TailNode n = {coh_load(&tn->code), coh_load(&tn->cnt), coh_load(&tn->next), coh_load(&tn->pos)};
     if (n.pos != r.node)
     {
         r.lost = true; // the ring has wrapped over it
@@ -657,10 +764,11 @@ RD_HD void rr_open(const Sim &s, RunReader &r, const Chain &ch, uint32_t code0)
         rr_fetch(s, r);
         return;
     }
-    const bool joins = ch.r[0].code == code0;
+    const Run r0 = coh_get(&ch.r[0]);
+    const bool joins = r0.code == code0;
     r.code = code0;
     r.at = joins ? 0 : -1;
-    r.left = joins ? ch.r[0].cnt + 1 : 1;
+    r.left = joins ? r0.cnt + 1 : 1;
 }
 RD_HD void rr_step(const Sim &s, RunReader &r)
 {
@@ -669,7 +777,7 @@ RD_HD void rr_step(const Sim &s, RunReader &r)
     else if (++r.at == RUNS)
     {
         r.at = RUNS + 1;
-        r.node = r.c->tail;
+        r.node = coh_load(&r.c->tail);
     }
     rr_fetch(s, r);
 }
@@ -688,6 +796,7 @@ RD_HD int memo_find(const Sim &s, uint64_t ha, uint64_t hb)
     const uint64_t k = memo_key(ha, hb);
     for (uint64_t i = 0; i < 4; i++)
     {
+        // (a plain read: a slot goes once from 0 to its key, so a stale 0 only costs a walk)
         const unsigned long long v = s.memo[(k + i) & s.memo_mask];
         if (!v)
             break; // (an empty slot ends the probe)
@@ -740,7 +849,10 @@ RD_COLD int order_walk(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, c
     const bool a_end = !A.code, b_end = !B.code;
     *inexact = (a_end && A.lost) || (b_end && B.lost);
     if (a_end && b_end)
-        return a.root && b.root && a.root != b.root ? (a.root < b.root ? -1 : 1) : deep;
+    {
+        const uint32_t a_root = coh_load(&a.root), b_root = coh_load(&b.root);
+        return a_root && b_root && a_root != b_root ? (a_root < b_root ? -1 : 1) : deep;
+    }
     if ((a_end ? A : B).lost)
         return deep;
     *by_lead = true;
@@ -749,9 +861,13 @@ RD_COLD int order_walk(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb, c
 // the lead hashes of the event that code0 pushed on c would be, then of its ancestors
 RD_HD void lead_hashes(uint64_t *h, uint32_t code0, const Chain &c)
 {
-    h[0] = code0 ? chash(c.hash, code0) : c.hash;
+    uint64_t anc[NANC];
+    for (int i = 0; i < NANC; i++)
+        anc[i] = coh_load(&c.anc[i]);
+    const uint64_t ch = coh_load(&c.hash);
+    h[0] = code0 ? chash(ch, code0) : ch;
     for (int i = 1; i <= NANC; i++)
-        h[i] = !code0 ? c.anc[i - 1] : i == 1 ? c.hash : c.anc[i - 2];
+        h[i] = !code0 ? anc[i - 1] : i == 1 ? ch : anc[i - 2];
 }
 // order_walk behind two shortcuts (src/rdma/rstep.cuh:504-575): lineages in lockstep (equal lead
 // hashes, distinct roots) run in their roots' rank order; and the memo, probed at each depth
@@ -762,8 +878,9 @@ RD_COLD int order_events(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb,
     lead_hashes(ha, pa, a);
     lead_hashes(hb, pb, b);
     *inexact = false;
-    if (a.root && b.root && a.root != b.root && ha[0] == hb[0])
-        return s.root_rank[a.root - 1] < s.root_rank[b.root - 1] ? -1 : 1;
+    const uint32_t a_root = coh_load(&a.root), b_root = coh_load(&b.root);
+    if (a_root && b_root && a_root != b_root && ha[0] == hb[0])
+        return s.root_rank[a_root - 1] < s.root_rank[b_root - 1] ? -1 : 1;
     RunReader A, B;
     rr_open(s, A, a, pa);
     rr_open(s, B, b, pb);
@@ -792,14 +909,17 @@ RD_COLD int order_events(const Sim &s, uint32_t pa, const Chain &a, uint32_t pb,
 constexpr int HEAD = 3;
 RD_HD void head_runs(Run *out, uint32_t code0, const Chain &c)
 {
-    const bool joins = code0 && c.r[0].code == code0;
+    Run h[HEAD];
+    for (int i = 0; i < HEAD; i++)
+        h[i] = coh_get(&c.r[i]);
+    const bool joins = code0 && h[0].code == code0;
     const int shift = code0 && !joins ? 1 : 0;
     for (int i = shift; i < HEAD; i++)
-        out[i] = c.r[i - shift];
+        out[i] = h[i - shift];
     if (code0)
     {
         out[0].code = code0;
-        out[0].cnt = joins ? c.r[0].cnt + 1 : 1;
+        out[0].cnt = joins ? h[0].cnt + 1 : 1;
     }
 }
 RD_HD int heads_cmp(const Run *a, const Run *b)
@@ -825,6 +945,8 @@ RD_HD int heads_cmp(const Run *a, const Run *b)
 }
 // does (ta, the event code pa pushes on a) run before (tb, the one pb pushes on b)? Times first,
 // then the chains; an undecided tie keeps b (src/rdma/rstep.cuh:635-657)
+// does (ta, the event code pa pushes on a) run before (tb, the one pb pushes on b)? Times first,
+// then the chains; an undecided tie keeps b (src/rdma/rstep.cuh:635-657)
 RD_COLD bool before(const Sim &s, int64_t ta, uint32_t pa, const Chain &a, int64_t tb, uint32_t pb, const Chain &b)
 {
     if (ta != tb)
@@ -841,10 +963,12 @@ RD_COLD bool before(const Sim &s, int64_t ta, uint32_t pa, const Chain &a, int64
 // (the quick test src/rdma/rstep.cuh:626-633 applies to arrivals, TransmitCompletes and sends)
 RD_COLD bool before_q(const Sim &s, int64_t t, uint32_t pa, const Chain &a, uint32_t pb, const Chain &b)
 {
-    const uint64_t ha = pa ? chash(a.hash, pa) : a.hash, hb = pb ? chash(b.hash, pb) : b.hash;
+    const uint64_t a_hash = coh_load(&a.hash), b_hash = coh_load(&b.hash);
+    const uint32_t a_root = coh_load(&a.root), b_root = coh_load(&b.root);
+    const uint64_t ha = pa ? chash(a_hash, pa) : a_hash, hb = pb ? chash(b_hash, pb) : b_hash;
     int r;
-    if (a.root && b.root && a.root != b.root && ha == hb)
-        r = s.root_rank[a.root - 1] < s.root_rank[b.root - 1] ? -1 : 1;
+    if (a_root && b_root && a_root != b_root && ha == hb)
+        r = s.root_rank[a_root - 1] < s.root_rank[b_root - 1] ? -1 : 1;
     else
     {
         Run xa[HEAD], xb[HEAD];
@@ -992,7 +1116,9 @@ RD_HD bool qp_window_full(const Params &P, const SndQP &q)
 }
 
 // a transmission-trace record (src/rdma/rstep.cuh:915-942): out of line, off the path when tracing is off
-RD_COLD void trace_tx(const Sim &s, int64_t now, int node, int dev, const Pkt &p)
+// (the packet by value, as its words 6 and 7 and its sequence number: a packet passed by reference
+// to this out-of-line function would be kept in scratch memory on the transmit path)
+RD_COLD void trace_tx(const Sim &s, int64_t now, int node, int dev, uint32_t seq, uint64_t w6, uint64_t w7)
 {
     const TxTrace &t = s.txt_rec;
     if (now < t.from || now > t.until)
@@ -1000,19 +1126,21 @@ RD_COLD void trace_tx(const Sim &s, int64_t now, int node, int dev, const Pkt &p
     const unsigned long long i = at_add64(t.cnt, 1ull);
     if (i >= t.cap)
         return;
+    const uint8_t kind = (uint8_t)(w6 >> 32), q = (uint8_t)(w6 >> 56);
+    const uint16_t sport = (uint16_t)(w7 >> 16), src = (uint16_t)(w7 >> 32), dst = (uint16_t)(w7 >> 48);
     TxRec r;
     r.ts = now;
     r.node = node;
     r.dev = dev;
-    r.kind = p.kind;
-    r.size = p.size;
-    r.ecn = p.ecn;
+    r.kind = kind;
+    r.size = (uint16_t)w6;
+    r.ecn = (uint8_t)(w6 >> 40);
     r.pad = 0;
-    const bool ctrl = p.kind >= K_PAUSE, fwd = p.kind == K_DATA;
-    r.a = ctrl ? p.q : fwd ? p.src : p.dst;
-    r.b = ctrl ? 0 : fwd ? p.dst : p.src;
-    r.sport = ctrl ? 0 : p.sport;
-    r.seq = ctrl ? 0 : p.seq;
+    const bool ctrl = kind >= K_PAUSE, fwd = kind == K_DATA;
+    r.a = ctrl ? q : fwd ? src : dst;
+    r.b = ctrl ? 0 : fwd ? dst : src;
+    r.sport = ctrl ? 0 : sport;
+    r.seq = ctrl ? 0 : seq;
     t.rec[i] = r;
 }
 
@@ -1027,20 +1155,19 @@ RD_HD void nic_emit(NicSlot &n, const Pkt &p)
     const int done_ord = n.ord; // the TransmitComplete's ordinal; the arrival's is the next
     n.ord += 2;
     n.nout++;
-    LFrame *fr = ob_push(s, n.pend, n.pend_n, lands, n.ob);
+    LFrame RD_G *fr = ob_push(s, n.pend, n.pend_n, lands, n.ob);
     if (fr)
     {
-        fr->ts = lands;
-        fr->p = p;
-        fr->p.ord = (uint8_t)(done_ord < 254 ? done_ord + 1 : 255);
+        Pkt q = p;
+        q.ord = (uint8_t)(done_ord < 254 ? done_ord + 1 : 255);
+        coh_store(&fr->ts, lands);
+        pkt_put(&fr->p, q);
+        if (s.txt_rec.rec)
+            trace_tx(s, n.now, n.h, 1, q.seq, pkt_w6(q), pkt_w7(q));
     }
     cpush(s, n.st.ta, &s.h_tc[n.h], ccode(busy, done_ord), n.cur);
     if (fr)
-    {
-        fr->c = n.cur;
-        if (s.txt_rec.rec)
-            trace_tx(s, n.now, n.h, 1, fr->p);
-    }
+        chain_put(&fr->c, n.cur);
 }
 
 // The egress pump (DequeueAndTransmit + RdmaEgressQueue): when the link is free, the ACK queue
@@ -1563,15 +1690,32 @@ RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, ui
     int di = 0, ai = 0;
     int sf = -1, sf_at = -1;
     int64_t sf_t = INT64_MAX;
+    // the next arrival's frame (written by the link's sender in an earlier slot: coherent reads):
+    // its time and packet read once, together (its chain when it runs)
+    const LFrame RD_G *af = nullptr;
+    FrameHead ah = {INT64_MAX, 0, 0};
+    Pkt apk;
+    int a_at = -1;
     for (;;)
     {
-        const LFrame *af = ai < an ? &s.msg[abase + ai] : nullptr;
-        if (af && af->p.size != in_sz)
+        if (a_at != ai)
         {
-            in_sz = af->p.size;
+            a_at = ai;
+            af = ai < an ? s.msg + abase + ai : nullptr;
+            if (af)
+            {
+                apk = pkt_get(&af->p);
+                ah = FrameHead{coh_load(&af->ts), apk.size, apk.ord};
+            }
+            else
+                ah = FrameHead{INT64_MAX, 0, 0};
+        }
+        if (af && ah.size != in_sz)
+        {
+            in_sz = ah.size;
             in_tx = link_tx(s, lin, in_sz);
         }
-        const uint32_t acode = af ? ccode(in_tx + in_delay, af->p.ord) : 0;
+        const uint32_t acode = af ? ccode(in_tx + in_delay, ah.ord) : 0;
         while (di < ndue && due_now(s, due[di]) != due[di].at)
             di++; // cancelled or moved
         if (sf_at != c.st.next_start)
@@ -1582,7 +1726,7 @@ RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, ui
         }
         // candidate times: 0 arrival, 1 TransmitComplete, 2 m_nextSend, 3 DCQCN timer, 4 flow start
         int64_t tt[5];
-        tt[0] = af ? af->ts : INT64_MAX;
+        tt[0] = ah.ts;
         tt[1] = c.st.tc_at >= 0 && c.st.tc_at < t1 ? c.st.tc_at : INT64_MAX;
         tt[2] = c.st.ns_at >= 0 && c.st.ns_at < t1 ? c.st.ns_at : INT64_MAX;
         tt[3] = di < ndue ? due[di].at : INT64_MAX;
@@ -1616,13 +1760,13 @@ RD_HD bool host_step(const Sim &s, int h, int64_t slot, unsigned *flows_done, ui
             count(s, EV_LATE);
         c.now = bt;
         if (bk == 0)
-            cpush(s, c.st.ta, &c.cur, acode, af->c);
+            cpush(s, c.st.ta, &c.cur, acode, chain_get(&af->c));
         else
             c.cur = bk == 1 ? s.h_tc[h] : bk == 2 ? s.h_ns[h] : bk == 3 ? due_chain(s, due[di]) : s.fi[sf].start_chain;
         c.ord = 0;
         if (bk == 0)
         {
-            const Pkt p = af->p;
+            const Pkt p = apk;
             ai++;
             if (p.kind == K_DATA)
                 nic_on_data(c, p);
@@ -1726,9 +1870,9 @@ RD_HD const Chain &sw_event_chain(SwSlot &c)
     {
         c.cur_ok = true;
         if (!c.cur_code)
-            c.cur = *c.cur_src;
+            c.cur = *c.cur_src; // (a TransmitComplete's chain, the port's own)
         else
-            cpush(*c.s, c.pt[c.ep].ta, &c.cur, c.cur_code, *c.cur_src);
+            cpush(*c.s, c.pt[c.ep].ta, &c.cur, c.cur_code, chain_get(c.cur_src)); // (an arrival's, in its frame)
     }
     return c.cur;
 }
@@ -1796,21 +1940,20 @@ RD_HD void port_emit(SwSlot &c, int port, const Pkt &p)
     const int done_ord = c.ord;
     c.ord += 2;
     r.nout++;
-    LFrame *fr = ob_push(s, c.pend, c.pend_n, lands, r.ob);
+    LFrame RD_G *fr = ob_push(s, c.pend, c.pend_n, lands, r.ob);
     if (fr)
     {
-        fr->ts = lands;
-        fr->p = p;
-        fr->p.ord = (uint8_t)(done_ord < 254 ? done_ord + 1 : 255);
+        Pkt q = p;
+        q.ord = (uint8_t)(done_ord < 254 ? done_ord + 1 : 255);
+        coh_store(&fr->ts, lands);
+        pkt_put(&fr->p, q);
+        if (s.txt_rec.rec)
+            trace_tx(s, c.now, c.node, port + 1, q.seq, pkt_w6(q), pkt_w7(q));
     }
     const Chain &ev = sw_event_chain(c);
     cpush(s, c.pt[c.ep].ta, &c.tc[port], ccode(busy, done_ord), ev);
     if (fr)
-    {
-        fr->c = ev;
-        if (s.txt_rec.rec)
-            trace_tx(s, c.now, c.node, port + 1, fr->p);
-    }
+        chain_put(&fr->c, ev);
 }
 // a frame into one of the port's queues (0: control, else data); the data queue's head is kept
 // in the port's slot state
@@ -1997,9 +2140,9 @@ RD_HD void sw_peek(const Sim &s, const SwSlot &c, int p)
     w.ats = INT64_MAX;
     if (w.ai < w.an)
     {
-        const LFrame &f = s.msg[w.abase + w.ai];
-        w.ats = f.ts;
-        w.acode = ccode(port_in_tx(s, w, f.p.size) + w.idelay, f.p.ord);
+        const FrameHead h = frame_head(s.msg + w.abase + w.ai);
+        w.ats = h.ts;
+        w.acode = ccode(port_in_tx(s, w, h.size) + w.idelay, h.ord);
     }
 }
 
@@ -2101,10 +2244,10 @@ RD_HD void sw_serial_events(const Sim &s, SwSlot &c, int deg, int64_t t1)
             port_pump(c, p);
             continue;
         }
-        const LFrame &f = s.msg[c.w[p].abase + c.w[p].ai];
-        const Pkt pk = f.p;
+        const LFrame RD_G *f = s.msg + c.w[p].abase + c.w[p].ai;
+        const Pkt pk = pkt_get(&f->p);
         c.cur_code = c.w[p].acode;
-        c.cur_src = &f.c;
+        c.cur_src = &f->c;
         c.w[p].ai++;
         sw_peek(s, c, p);
         if (pk.kind == K_PAUSE || pk.kind == K_RESUME)
@@ -2197,11 +2340,12 @@ typedef SwWorkT<KW> SwWorkWide; // in device memory, a wave's (switches of KS + 
 
 // the arrival code and chain of arrival j (of ingress p) in the workspace
 template <typename WK>
-RD_HD const LFrame &sw_arr(const Sim &s, const WK &k, int p, int j) { return s.msg[k.w[p].abase + (uint64_t)(j - k.first[p])]; }
-RD_HD uint32_t sw_acode(const Sim &s, const SwSlot &c, int p, const LFrame &f)
+RD_HD const LFrame RD_G *sw_arr(const Sim &s, const WK &k, int p, int j) { return s.msg + k.w[p].abase + (uint64_t)(j - k.first[p]); }
+RD_HD uint32_t sw_acode(const Sim &s, const SwSlot &c, int p, const LFrame RD_G *f)
 {
     const PortRun &r = c.w[p];
-    return ccode(port_in_tx(s, r, f.p.size) + r.idelay, f.p.ord);
+    const FrameHead h = frame_head(f);
+    return ccode(port_in_tx(s, r, h.size) + r.idelay, h.ord);
 }
 
 // egress port o's events of a safe slot (src/rdma/rstep.cuh:2806-3005)
@@ -2245,8 +2389,8 @@ RD_HD void sw_egress_events(const Sim &s, SwSlot &c, WK &k, int o, int deg, int6
                 b4 = k.ats[j] < k.ats[kb];
             else
             {
-                const LFrame &fa = sw_arr(s, k, p, j), &fb = sw_arr(s, k, bp, kb);
-                b4 = before_q(s, k.ats[j], sw_acode(s, c, p, fa), fa.c, sw_acode(s, c, bp, fb), fb.c);
+                const LFrame RD_G *fa = sw_arr(s, k, p, j), *fb = sw_arr(s, k, bp, kb);
+                b4 = before_q(s, k.ats[j], sw_acode(s, c, p, fa), fa->c, sw_acode(s, c, bp, fb), fb->c);
             }
             if (b4)
                 bp = p, kb = j;
@@ -2258,8 +2402,8 @@ RD_HD void sw_egress_events(const Sim &s, SwSlot &c, WK &k, int o, int deg, int6
         if (ta == INT64_MAX && !htc)
             break;
         bool arr;
-        const LFrame *fa = kb >= 0 ? &sw_arr(s, k, bp, kb) : nullptr;
-        const uint32_t acode = fa ? sw_acode(s, c, bp, *fa) : 0;
+        const LFrame RD_G *fa = kb >= 0 ? sw_arr(s, k, bp, kb) : nullptr;
+        const uint32_t acode = fa ? sw_acode(s, c, bp, fa) : 0;
         if (!htc || ta == INT64_MAX)
             arr = !htc;
         else if (ta != tc)
@@ -2287,7 +2431,7 @@ RD_HD void sw_egress_events(const Sim &s, SwSlot &c, WK &k, int o, int deg, int6
         c.now = ta;
         c.cur_src = &fa->c;
         c.cur_code = acode;
-        const Pkt pk = fa->p;
+        const Pkt pk = pkt_get(&fa->p);
         if (pk.kind == K_PAUSE || pk.kind == K_RESUME)
         {
             if (pk.kind == K_PAUSE)
@@ -2352,17 +2496,18 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
             for (int i = 0; i < k.cnt[p]; i++)
             {
                 const int j = f0 + i;
-                const LFrame &f = s.msg[k.w[p].abase + i];
-                k.ats[j] = f.ts;
+                const LFrame RD_G *f = s.msg + k.w[p].abase + i;
+                k.ats[j] = coh_load(&f->ts);
+                const Pkt pk = pkt_get(&f->p);
                 int q = 0;
-                if (f.p.kind == K_PAUSE || f.p.kind == K_RESUME)
+                if (pk.kind == K_PAUSE || pk.kind == K_RESUME)
                     k.own[j] = (int8_t)p;
                 else
                 {
-                    const int out = sw_route(s, *c.P, x, f.p, &q);
+                    const int out = sw_route(s, *c.P, x, pk, &q);
                     k.own[j] = (int8_t)out;
                     if (out >= 0 && q != 0)
-                        k.abytes[p] += f.p.size;
+                        k.abytes[p] += pk.size;
                 }
                 k.q[j] = (uint8_t)q;
             }
@@ -2489,17 +2634,19 @@ RD_HD bool switch_has_input(const Sim &s, int x, int64_t slot)
 // skip to the slot of the earliest host event or pending arrival; with none, the run is quiescent.
 RD_HD void slot_end(const Sim &s, const World &wd, WorldRun &r, uint32_t *pend, unsigned act, unsigned long long wake)
 {
-    const int64_t slot = r.slot;
+    // (the ticket and the pending-slot flags are read and written coherently: every workgroup of
+    // the team reads the ticket after the barrier, and every entity sets flags)
+    const int64_t slot = coh_load(&r.slot);
     // (an atomic read: the lanes' completions were counted by atomics, in L2)
     if (at_add32(&r.flows_done, 0u) >= (uint32_t)wd.nf || slot + 1 >= wd.max_slots)
-        r.finished = 1;
-    pend[slot % wd.pend_n] = 0;
+        coh_store(&r.finished, 1);
+    coh_store(pend + slot % wd.pend_n, 0u);
     int64_t next = slot + 1;
     if (act == 0)
     {
         int64_t ws = wake != ~0ull ? (int64_t)(wake / (unsigned long long)s.D) : INT64_MAX;
         for (int i = 1; i < wd.pend_n; i++)
-            if (pend[(slot + i) % wd.pend_n])
+            if (coh_load(pend + (slot + i) % wd.pend_n))
             {
                 ws = slot + i < ws ? slot + i : ws;
                 break;
@@ -2507,10 +2654,10 @@ RD_HD void slot_end(const Sim &s, const World &wd, WorldRun &r, uint32_t *pend, 
         if (ws != INT64_MAX && ws > next)
             next = ws < wd.max_slots ? ws : wd.max_slots;
         if (ws == INT64_MAX)
-            r.finished = 1;
+            coh_store(&r.finished, 1);
     }
-    r.slots_run++;
-    r.slot = next;
+    coh_store(&r.slots_run, coh_load(&r.slots_run) + 1);
+    coh_store(&r.slot, next);
 }
 
 } // namespace rd

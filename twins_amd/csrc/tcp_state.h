@@ -12,6 +12,7 @@
 // a few lines; queues, hosts, ports and link messages are arrays indexed by global id.
 #pragma once
 #include <cstdint>
+#include "coherent.h"
 
 // Device memory, for the device compile: pointers of the batch record carry the global address
 // space, so the compiler emits global (not flat) loads and stores through them and what is derived
@@ -31,75 +32,11 @@
 namespace tw
 {
 
-// Device-coherent accesses: relaxed atomics at agent scope, which every XCD sees in one order.
-// The data a team's workgroups hand each other from slot to slot - link messages and their count
-// words, the world's ticket, a sender's end state - goes only through these, so the slot's team
-// barrier needs no acquire/release fence: on a GPU of several XCDs an agent-scope fence writes
-// back and invalidates the L2 (buffer_wbl2 / buffer_inv), and every slot would start with cold
-// caches for the entities' own records. CPU builds: plain accesses.
-template <typename T>
-TW_HD inline T coh_load(const T *p)
-{
-    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "a 4- or 8-byte word");
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#else
-    return *p;
-#endif
-}
-template <typename T>
-TW_HD inline void coh_store(T *p, T v)
-{
-    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "a 4- or 8-byte word");
-#if defined(__HIP_DEVICE_COMPILE__)
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#else
-    *p = v;
-#endif
-}
-#if defined(__HIP_DEVICE_COMPILE__)
-// (the same through pointers that carry the global address space)
-template <typename T>
-inline __device__ T coh_load(const TW_G T *p)
-{
-    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "a 4- or 8-byte word");
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-inline __device__ void coh_store(TW_G T *p, T v)
-{
-    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "a 4- or 8-byte word");
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-#endif
-// a record of whole 8-byte words, word by word (the words' loads or stores issued together)
-template <typename R>
-TW_HD inline R coh_get(const R *p)
-{
-    static_assert(sizeof(R) % 8 == 0, "whole 8-byte words");
-    R r;
-    const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(R) / 8); i++)
-    {
-        const uint64_t w = coh_load(q + i);
-        __builtin_memcpy(reinterpret_cast<char *>(&r) + 8 * i, &w, 8);
-    }
-    return r;
-}
-template <typename R>
-TW_HD inline void coh_put(R *p, const R &r)
-{
-    static_assert(sizeof(R) % 8 == 0, "whole 8-byte words");
-    uint64_t *d = reinterpret_cast<uint64_t *>(p);
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(R) / 8); i++)
-    {
-        uint64_t w;
-        __builtin_memcpy(&w, reinterpret_cast<const char *>(&r) + 8 * i, 8);
-        coh_store(d + i, w);
-    }
-}
+// Device-coherent accesses (coherent.h): what a team's workgroups hand each other from slot to slot
+using twc::coh_get;
+using twc::coh_load;
+using twc::coh_put;
+using twc::coh_store;
 
 // A slot's work items on a team's waves: host items of `per` hosts side by side (one serial
 // event loop per lane) and switch items of min(per, 64 / k) switches (a lane per port). The work
