@@ -88,6 +88,26 @@ inline void coh_put(R *p, const R &r)
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
+// 16 bytes at byte offset `off` of a buffer, with the cache policy of an agent-scope atomic store
+// (sc1: written through, device-coherent): half the store instructions of 8-byte atomic stores
+inline __device__ void coh_store16(__amdgpu_buffer_rsrc_t r, uint32_t off, uint64_t a, uint64_t b)
+{
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    const v4u v = {(unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 16 /* sc1 */);
+}
+// a raw buffer over `bytes` bytes from p (at most 2^31), made wave-uniform: p and bytes must be
+// the same on every lane
+inline __device__ __amdgpu_buffer_rsrc_t coh_buffer(const void *p, uint32_t bytes)
+{
+    const uint64_t a = (uint64_t)p;
+    // (readfirstlane returns an int: each half is widened through uint32_t, not sign-extended)
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((unsigned)a), hi = (uint32_t)__builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    const uint64_t u = lo | hi << 32;
+    return __builtin_amdgcn_make_buffer_rsrc((void *)u, 0, (int)__builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
+#endif
+#if defined(__HIP_DEVICE_COMPILE__)
 // (records through pointers that carry the global address space)
 template <typename R>
 inline __device__ R coh_get(const __attribute__((address_space(1))) R *p)

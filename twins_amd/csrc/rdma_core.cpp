@@ -818,6 +818,8 @@ static int pack(trd_batch &b, trd_world *const *worlds, int W, const trd_pack_cf
     TRY(b.alloc(&s.q3, (size_t)NP * s.q3_cap));
     TRY(b.upload_c(&s.li, li));
     TRY(b.alloc(&s.msg, nframes));
+    // (16-byte coherent frame stores address the rings with 32-bit offsets)
+    s.msg_bytes = (uint64_t)nframes * sizeof(rd::LFrame) <= 0x7FFFFFF0ull ? (uint32_t)((uint64_t)nframes * sizeof(rd::LFrame)) : 0;
     TRY(b.upload(&s.msg_cnt, vector<int64_t>(ncnt, -(1ll << 16))));
     {
         vector<int32_t> txt(rates.size() * rd::TXT_N);

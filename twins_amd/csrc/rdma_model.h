@@ -484,6 +484,7 @@ struct Sim
     int q0_cap, q3_cap;
     const Link RD_G *li;
     LFrame RD_G *msg;
+    uint32_t msg_bytes; // the link rings' size when below 2^31 (frames written 16 bytes a store), else 0
     int64_t RD_G *msg_cnt;
     const int32_t RD_G *txt; // [R][TXT_N]
     uint32_t RD_G *pend;
@@ -644,6 +645,50 @@ RD_HD LFrame RD_G *ob_push(const Sim &s, uint32_t *pend, int pend_n, int64_t ts,
         return nullptr;
     }
     return s.msg + o.base + o.n++;
+}
+
+// A frame's packet and chain, written coherently: 16 bytes a store (sc1 buffer stores) where the
+// link rings fit a 32-bit offset, else 8-byte atomic stores; plain stores in the CPU build.
+RD_HD void frame_put_pkt(const Sim &s, LFrame RD_G *fr, const Pkt &q)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (s.msg_bytes)
+    {
+        const __amdgpu_buffer_rsrc_t r = twc::coh_buffer((const void *)s.msg, s.msg_bytes);
+        const uint32_t o = (uint32_t)((const char RD_G *)&fr->p - (const char RD_G *)s.msg);
+        twc::coh_store16(r, o, q.hop[0], q.hop[1]);
+        twc::coh_store16(r, o + 16, q.hop[2], q.hop[3]);
+        twc::coh_store16(r, o + 32, q.hop[4], (uint64_t)q.flow | (uint64_t)q.seq << 32);
+        twc::coh_store16(r, o + 48, pkt_w6(q), pkt_w7(q));
+        return;
+    }
+#endif
+    pkt_put(&fr->p, q);
+}
+RD_HD void frame_put_chain(const Sim &s, LFrame RD_G *fr, const Chain &c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (s.msg_bytes)
+    {
+        const __amdgpu_buffer_rsrc_t r = twc::coh_buffer((const void *)s.msg, s.msg_bytes);
+        const uint32_t o = (uint32_t)((const char RD_G *)&fr->c - (const char RD_G *)s.msg);
+        uint64_t w[RUNS + 2 + NANC + 1];
+        for (int i = 0; i < RUNS; i++)
+            w[i] = (uint64_t)c.r[i].code | (uint64_t)c.r[i].cnt << 32;
+        w[RUNS] = c.tail;
+        w[RUNS + 1] = c.hash;
+        for (int i = 0; i < NANC; i++)
+            w[RUNS + 2 + i] = c.anc[i];
+        w[RUNS + 2 + NANC] = (uint64_t)c.root;
+        constexpr int NW = RUNS + 2 + NANC + 1;
+        for (int i = 0; i + 1 < NW; i += 2)
+            twc::coh_store16(r, o + 8 * i, w[i], w[i + 1]);
+        if (NW & 1)
+            coh_store(reinterpret_cast<uint64_t RD_G *>(&fr->c) + NW - 1, w[NW - 1]);
+        return;
+    }
+#endif
+    chain_put(&fr->c, c);
 }
 
 // ------------------------------------------------------------------ event order (src/rdma/rstep.cuh:155-660)
@@ -1161,13 +1206,13 @@ RD_HD void nic_emit(NicSlot &n, const Pkt &p)
         Pkt q = p;
         q.ord = (uint8_t)(done_ord < 254 ? done_ord + 1 : 255);
         coh_store(&fr->ts, lands);
-        pkt_put(&fr->p, q);
+        frame_put_pkt(s, fr, q);
         if (s.txt_rec.rec)
             trace_tx(s, n.now, n.h, 1, q.seq, pkt_w6(q), pkt_w7(q));
     }
     cpush(s, n.st.ta, &s.h_tc[n.h], ccode(busy, done_ord), n.cur);
     if (fr)
-        chain_put(&fr->c, n.cur);
+        frame_put_chain(s, fr, n.cur);
 }
 
 // The egress pump (DequeueAndTransmit + RdmaEgressQueue): when the link is free, the ACK queue
@@ -1946,14 +1991,14 @@ RD_HD void port_emit(SwSlot &c, int port, const Pkt &p)
         Pkt q = p;
         q.ord = (uint8_t)(done_ord < 254 ? done_ord + 1 : 255);
         coh_store(&fr->ts, lands);
-        pkt_put(&fr->p, q);
+        frame_put_pkt(s, fr, q);
         if (s.txt_rec.rec)
             trace_tx(s, c.now, c.node, port + 1, q.seq, pkt_w6(q), pkt_w7(q));
     }
     const Chain &ev = sw_event_chain(c);
     cpush(s, c.pt[c.ep].ta, &c.tc[port], ccode(busy, done_ord), ev);
     if (fr)
-        chain_put(&fr->c, ev);
+        frame_put_chain(s, fr, ev);
 }
 // a frame into one of the port's queues (0: control, else data); the data queue's head is kept
 // in the port's slot state
