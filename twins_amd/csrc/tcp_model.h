@@ -2339,15 +2339,31 @@ struct Model
         }
         L.cnt[x.q] = off;
     }
-    // 4: the start of egress q's region
+    // 4: the start of egress q's region: the exclusive prefix sum of the switch's region sizes over
+    // its ports. On the GPU the switch's k lanes are consecutive in the wavefront (port q on its
+    // first lane + q) and every lane of the wave runs this phase, so a wave64 scan does it in six
+    // shuffle steps; a CPU check, which runs the lanes one after another, sums in a loop.
     static __device__ void sw_base(const SwLane &x, const SwLds &L)
     {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const int v = x.active ? L.cnt[x.q] : 0;
+        int incl = v;
+        for (int d = 1; d < 64; d <<= 1)
+        {
+            const int u = __shfl_up(incl, d);
+            if (x.active && x.q >= d)
+                incl += u;
+        }
+        if (x.active)
+            L.base[x.q] = incl - v;
+#else
         if (!x.active)
             return;
         int b = 0;
         for (int j = 0; j < x.q; j++)
             b += L.cnt[j];
         L.base[x.q] = b;
+#endif
     }
     // 5: every arrival of ingress q into its egress's region, in arrival order
     static __device__ void sw_scatter(const Batch &B, const WorldDesc &W, const SwLane &x, int64_t t, const SwLds &L)
