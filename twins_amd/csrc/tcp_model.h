@@ -1946,7 +1946,14 @@ struct Model
         // own event is taken (the next reads are then in flight while that event runs): the
         // next arrival, the next timer (due_head, which due_insert keeps), the next application
         // start of the host's order.
-        auto next_arrival = [&]() -> int64_t { return i2 < n2 ? coh_load(&m2[i2].ts) : i1 < n1 ? coh_load(&m1[i1].ts) : INT64_MAX; };
+        // (the next arrival's whole frame is read with its time, so taking it waits for no load)
+        Frame fa;
+        auto next_arrival = [&]() -> int64_t {
+            if (i2 >= n2 && i1 >= n1)
+                return INT64_MAX;
+            fa = coh_get(i2 < n2 ? m2 + i2 : m1 + i1);
+            return fa.ts;
+        };
         auto next_start = [&](int &m) -> int64_t {
             m = pn < pe ? B.pend_msg[pn] : -1;
             if (m >= 0)
@@ -2082,7 +2089,11 @@ struct Model
             }
             else
             {
-                Frame p = coh_get(i2 < n2 ? m2 + i2++ : m1 + i1++);
+                const Frame p = fa;
+                if (i2 < n2)
+                    i2++;
+                else
+                    i1++;
                 tr = next_arrival();
                 Chain pc;
                 chain_load(pc, p.chain);
