@@ -2359,6 +2359,7 @@ struct SwWorkT
     int16_t mc[K][K]; // [egress][ingress]: the ingress's next arrival for the egress (-1: none)
     int16_t onext[LK];  // an arrival's next one of the same ingress and egress (-1: none)
     int64_t ats[LK];
+    uint32_t acode[LK]; // its event code (the in-link's tx time and delay, its ordinal)
     int8_t own[LK];     // the egress port it belongs to in a safe slot (-1: none)
     uint8_t q[LK];
     int total, safe;
@@ -2383,15 +2384,9 @@ typedef SwWorkT<KW> SwWorkWide; // in device memory, a wave's (switches of KS + 
     } while (0)
 #endif
 
-// the arrival code and chain of arrival j (of ingress p) in the workspace
+// arrival j's frame (of ingress p) in its link's ring (its code is in the workspace: k.acode[j])
 template <typename WK>
 RD_HD const LFrame RD_G *sw_arr(const Sim &s, const WK &k, int p, int j) { return s.msg + k.w[p].abase + (uint64_t)(j - k.first[p]); }
-RD_HD uint32_t sw_acode(const Sim &s, const SwSlot &c, int p, const LFrame RD_G *f)
-{
-    const PortRun &r = c.w[p];
-    const FrameHead h = frame_head(f);
-    return ccode(port_in_tx(s, r, h.size) + r.idelay, h.ord);
-}
 
 // egress port o's events of a safe slot (src/rdma/rstep.cuh:2806-3005)
 #if defined(RD_PROF) && defined(__HIP_DEVICE_COMPILE__)
@@ -2435,7 +2430,7 @@ RD_HD void sw_egress_events(const Sim &s, SwSlot &c, WK &k, int o, int deg, int6
             else
             {
                 const LFrame RD_G *fa = sw_arr(s, k, p, j), *fb = sw_arr(s, k, bp, kb);
-                b4 = before_q(s, k.ats[j], sw_acode(s, c, p, fa), fa->c, sw_acode(s, c, bp, fb), fb->c);
+                b4 = before_q(s, k.ats[j], k.acode[j], fa->c, k.acode[kb], fb->c);
             }
             if (b4)
                 bp = p, kb = j;
@@ -2448,7 +2443,7 @@ RD_HD void sw_egress_events(const Sim &s, SwSlot &c, WK &k, int o, int deg, int6
             break;
         bool arr;
         const LFrame RD_G *fa = kb >= 0 ? sw_arr(s, k, bp, kb) : nullptr;
-        const uint32_t acode = fa ? sw_acode(s, c, bp, fa) : 0;
+        const uint32_t acode = fa ? k.acode[kb] : 0;
         if (!htc || ta == INT64_MAX)
             arr = !htc;
         else if (ta != tc)
@@ -2544,6 +2539,7 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
                 const LFrame RD_G *f = s.msg + k.w[p].abase + i;
                 k.ats[j] = coh_load(&f->ts);
                 const Pkt pk = pkt_get(&f->p);
+                k.acode[j] = ccode(port_in_tx(s, k.w[p], pk.size) + k.w[p].idelay, pk.ord);
                 int q = 0;
                 if (pk.kind == K_PAUSE || pk.kind == K_RESUME)
                     k.own[j] = (int8_t)p;
