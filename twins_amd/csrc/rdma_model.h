@@ -2354,6 +2354,7 @@ struct SwWorkT
     Port pt[K];
     PortRun w[K];
     PortCfg pc[K]; // (a copy: no load of a port's configuration per event)
+    Chain tc[K];   // the ports' pending TransmitCompletes' chains (copies, written back at the slot's end)
     int cnt[K], first[K];
     uint32_t abytes[K], ing0[K];
     int16_t mc[K][K]; // [egress][ingress]: the ingress's next arrival for the egress (-1: none)
@@ -2500,6 +2501,7 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
     SwSlot c;
     sw_ctx(c, s, x, slot, pend, pend_n, k.pt, k.w);
     c.pc = k.pc;
+    c.tc = k.tc;
 #if defined(RD_PROF) && defined(__HIP_DEVICE_COMPILE__)
     long long pt0 = wall_clock64();
 #endif
@@ -2508,6 +2510,7 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
     {
         k.pc[p] = s.pc[si.pbase + p];
         k.pt[p] = s.ps[si.pbase + p];
+        k.tc[p] = s.p_tc[si.pbase + p];
         k.ing0[p] = k.pt[p].ing;
         k.abytes[p] = 0;
         sw_port_open(s, c, p, slot);
@@ -2614,6 +2617,7 @@ RD_HD bool switch_slot(const Sim &s, int x, int64_t slot, uint32_t *pend, int pe
     {
         act |= sw_port_close(s, c, p, &nout);
         s.ps[si.pbase + p] = k.pt[p];
+        s.p_tc[si.pbase + p] = k.tc[p];
     }
     if (act)
         at_add32(&k.act, 1u);
